@@ -1,0 +1,112 @@
+"""ctypes binding of libdogpup_hip.so (the C ABI declared in include/dogpup.h).
+
+The library is loaded from dogpup_amd/lib/ (built in-tree by
+``python -m dogpup_amd.build``).  There is no fallback: if the library is
+missing or fails to load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libdogpup_hip.so"
+
+OK = 0
+ERR_ARG, ERR_HIP, ERR_UNSUPPORTED, ERR_NOMEM = -1, -2, -3, -4
+RHS_CONVERGED, RHS_MAXIT, RHS_BREAKDOWN, RHS_ZERO = 0, 1, 2, 3
+MODE_C128, MODE_C64_REFINE, MODE_C128_PLAIN = 0, 1, 2
+
+_lock = threading.Lock()
+_lib = None
+
+vp, ip, dp, i32, f64 = C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double
+P = C.POINTER
+
+# name -> argtypes (all return int)
+SIGNATURES = {
+    "dogpup_last_error": None,
+    "dogpup_version": [],
+    "dogpup_device_count": [P(C.c_int)],
+    "dogpup_smv_call": [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, vp],
+    "dogpup_fsaimv_call": [vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp],
+    "dogpup_fsaip_call": [vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp],
+    "dogpup_smv_mex": [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32],
+    "dogpup_fsaimv_mex": [vp, vp, vp, vp, vp, i32, i32, i32, i32],
+    "dogpup_fsaip_gen_mex": [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32],
+    "dogpup_assemble": [i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, f64, vp, vp, vp, vp],
+    "dogpup_solver_create": [P(vp), i32, i32, i32, i32, i32, i32, vp, vp, i32, vp, vp, vp, vp, vp],
+    "dogpup_solver_destroy": [vp],
+    "dogpup_solver_set_matrix": [vp, vp, vp, vp, vp],
+    "dogpup_solver_solve": [vp, vp, vp, f64, i32, vp, vp, vp, vp],
+    "dogpup_solver_stats": [vp, vp, i32],
+    "dogpup_solver_get_fsai": [vp, vp, vp],
+    "dogpup_layout_to_solver": [vp, vp, i32, i32, i32, i32, vp],
+    "dogpup_layout_from_solver": [vp, vp, i32, i32, i32, i32, vp],
+    "dogpup_point_sources": [vp, i32, i32, i32, i32, vp, vp, vp, vp],
+    "dogpup_td2fc": [vp, vp, vp, vp, i32, i32, i32, i32, vp],
+    "dogpup_fc2td": [vp, vp, vp, vp, i32, i32, i32, vp],
+    "dogpup_fc2tg": [vp, vp, vp, i32, i32, i32, f64, vp],
+    "dogpup_jacobian_tg": [vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32, vp],
+    "dogpup_meas": [vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp],
+    "dogpup_solver_set_profile": [vp, i32],
+    "dogpup_solver_profile": [vp, vp, i32],
+}
+
+
+class DogpupError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dogpup error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load the library (once); raises if it is not built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(f"{LIB_PATH} is not built; run `python -m dogpup_amd.build`")
+            L = C.CDLL(str(LIB_PATH))
+            for name, args in SIGNATURES.items():
+                fn = getattr(L, name, None)
+                if fn is None:
+                    continue
+                if name == "dogpup_last_error":
+                    fn.restype = C.c_char_p
+                    fn.argtypes = []
+                else:
+                    fn.restype = C.c_int
+                    fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+def exported_symbols():
+    L = lib()
+    return [n for n in SIGNATURES if hasattr(L, n)]
+
+
+def call(name, *args):
+    """Call an entry point and raise DogpupError on a non-zero status."""
+    L = lib()
+    fn = getattr(L, name)
+    rc = fn(*args)
+    if rc != OK:
+        raise DogpupError(rc, L.dogpup_last_error().decode(errors="replace"))
+    return rc
+
+
+def ptr(t):
+    """Device/host pointer of a torch tensor or numpy array (None -> NULL)."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return C.c_void_p(t.data_ptr())
+    return C.c_void_p(t.ctypes.data)
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)

@@ -1,0 +1,167 @@
+// FSAI factor setup on gfx950.
+//
+// For every (pattern row i, frequency k) the factor row g_i of G_k ~ L_k^-1
+// solves A_k[P_i, P_i] x = e_m (P_i ascending, diagonal last, m = |P_i|) and
+// is scaled to g_i = x / sqrt(x_m) with the principal complex square root
+// (behaviour of DOGPUP/cuda_lib/mex_sources/FSAIP_CUDA.cu:92-160: symmetric
+// local matrix gathered from the lower triangle, Gaussian elimination with
+// partial pivoting on the modulus, back substitution; A_k = Re A + i*k*Im A,
+// FSAIP_CUDA.cu:132-133).
+//
+// MI355X design: one lane per (row, frequency); the lane's m x (m+1) complex
+// augmented system lives in LDS laid out [entry][lane] so the 64 lanes of a
+// wavefront hit 64 different banks.  Rows are bucketed by m on the host side
+// of the launch (one launch per LDS size class), so a Kuhn mesh (m <= 8)
+// needs a single class.
+#include "common.hpp"
+#include "fsai.hpp"
+
+namespace dogpup {
+
+template <int MMAX, int BS>
+__global__ __launch_bounds__(BS) void k_fsai_rows(
+    const double* __restrict__ are, const double* __restrict__ aim, int a_il,
+    const int* __restrict__ rptrA, const int* __restrict__ cptrA, const int* __restrict__ rptrG,
+    const int* __restrict__ cptrG, int nrows, int nfreqs, const double* __restrict__ fscale,
+    int mlo, FsaiOut out) {
+  constexpr int W = MMAX + 1;
+  __shared__ double sre[MMAX * W * BS];
+  __shared__ double sim[MMAX * W * BS];
+  const int tid = threadIdx.x;
+  const long long t = (long long)blockIdx.x * BS + tid;
+  if (t >= (long long)nrows * nfreqs) return;
+  const int row = (int)(t / nfreqs);
+  const int k = (int)(t % nfreqs);
+  const int gs = rptrG[row];
+  const int m = rptrG[row + 1] - gs;
+  if (m <= mlo || m > MMAX) return;
+  const double fk = fscale ? fscale[k] : (double)k;
+#define AR(r, c) sre[((r) * W + (c)) * BS + tid]
+#define AI(r, c) sim[((r) * W + (c)) * BS + tid]
+  for (int r = 0; r < m; ++r)
+    for (int c = 0; c <= m; ++c) { AR(r, c) = 0.0; AI(r, c) = 0.0; }
+  // gather A_k[P, P] from the lower-triangle entries (P[a] >= P[b] for b <= a)
+  for (int a = 0; a < m; ++a) {
+    const int ra = cptrG[gs + a];
+    const int s0 = rptrA[ra], s1 = rptrA[ra + 1];
+    int p = s0;
+    for (int b = 0; b <= a; ++b) {
+      const int cb = cptrG[gs + b];
+      // columns of row ra are ascending and P is ascending: resume the scan
+      while (p < s1 && cptrA[p] < cb) ++p;
+      if (p < s1 && cptrA[p] == cb) {
+        double vr, vi;
+        if (a_il) { vr = are[2LL * p]; vi = are[2LL * p + 1]; }
+        else { vr = are[p]; vi = aim[p]; }
+        vi *= fk;
+        AR(a, b) = vr; AI(a, b) = vi;
+        AR(b, a) = vr; AI(b, a) = vi;
+      }
+    }
+  }
+  AR(m - 1, m) = 1.0;
+  // forward elimination with partial pivoting on the modulus
+  for (int i = 0; i < m; ++i) {
+    int piv = i;
+    double best = sqrt(AR(i, i) * AR(i, i) + AI(i, i) * AI(i, i));
+    for (int j = i + 1; j < m; ++j) {
+      double mj = sqrt(AR(j, i) * AR(j, i) + AI(j, i) * AI(j, i));
+      if (mj > best) { best = mj; piv = j; }
+    }
+    if (piv != i) {
+      for (int c = i; c <= m; ++c) {
+        double tr = AR(i, c), ti = AI(i, c);
+        AR(i, c) = AR(piv, c); AI(i, c) = AI(piv, c);
+        AR(piv, c) = tr; AI(piv, c) = ti;
+      }
+    }
+    const cpx<double> d{AR(i, i), AI(i, i)};
+    for (int j = i + 1; j < m; ++j) {
+      const cpx<double> f = cdiv(cpx<double>{AR(j, i), AI(j, i)}, d);
+      for (int c = i; c <= m; ++c) {
+        const cpx<double> pr = cmul(f, cpx<double>{AR(i, c), AI(i, c)});
+        AR(j, c) -= pr.re;
+        AI(j, c) -= pr.im;
+      }
+    }
+  }
+  // back substitution; the solution overwrites the augmented column
+  for (int i = m - 1; i >= 0; --i) {
+    double sr = 0.0, si = 0.0;
+    for (int j = i + 1; j < m; ++j) {
+      const cpx<double> pr = cmul(cpx<double>{AR(i, j), AI(i, j)}, cpx<double>{AR(j, m), AI(j, m)});
+      sr += pr.re;
+      si += pr.im;
+    }
+    const cpx<double> x = cdiv(cpx<double>{AR(i, m) - sr, AI(i, m) - si}, cpx<double>{AR(i, i), AI(i, i)});
+    AR(i, m) = x.re;
+    AI(i, m) = x.im;
+  }
+  // principal square root of x_m
+  const double xr = AR(m - 1, m), xi = AI(m - 1, m);
+  const double mod = sqrt(xr * xr + xi * xi);
+  cpx<double> q;
+  q.re = sqrt((mod + xr) / 2);
+  if (xi == 0.0) {
+    if (xr < 0.0) { q.re = 0.0; q.im = sqrt(-xr); }
+    else q.im = 0.0;
+  } else {
+    q.im = (xi > 0 ? 1.0 : -1.0) * sqrt((mod - xr) / 2);
+  }
+  for (int i = 0; i < m; ++i) {
+    const cpx<double> g = cdiv(cpx<double>{AR(i, m), AI(i, m)}, q);
+    const long long j = gs + i;
+    if (out.layout == FSAI_OUT_REF) {
+      out.re[k + (long long)nfreqs * j] = g.re;
+      out.im[k + (long long)nfreqs * j] = g.im;
+    } else {
+      double* o = out.re + 2 * ((long long)k * out.nnzG + j);
+      o[0] = g.re;
+      o[1] = g.im;
+    }
+  }
+#undef AR
+#undef AI
+}
+
+__global__ void k_max_row_len(const int* __restrict__ rptr, int n, int* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int m = 0;
+  if (i < n) m = rptr[i + 1] - rptr[i];
+  // wave reduction then one atomic per wave
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+int fsai_launch(const double* are, const double* aim, int a_il, const int* rptrA,
+                const int* cptrA, const int* rptrG, const int* cptrG, int nrows, int nfreqs,
+                const double* fscale, FsaiOut out, hipStream_t st) {
+  if (nrows <= 0 || nfreqs <= 0) return DOGPUP_OK;
+  int* dmax = nullptr;
+  DP_HIP(hipMalloc((void**)&dmax, sizeof(int)));
+  DP_HIP(hipMemsetAsync(dmax, 0, sizeof(int), st));
+  k_max_row_len<<<div_up(nrows, 256), 256, 0, st>>>(rptrG, nrows, dmax);
+  DP_LAUNCH_CHECK();
+  int hmax = 0;
+  DP_HIP(hipMemcpyAsync(&hmax, dmax, sizeof(int), hipMemcpyDeviceToHost, st));
+  DP_HIP(hipStreamSynchronize(st));
+  DP_HIP(hipFree(dmax));
+  if (hmax > 32) return fail(DOGPUP_ERR_UNSUPPORTED, "FSAI pattern row with %d entries (max 32)", hmax);
+  const long long total = (long long)nrows * nfreqs;
+  k_fsai_rows<8, 64><<<div_up(total, 64), 64, 0, st>>>(are, aim, a_il, rptrA, cptrA, rptrG, cptrG,
+                                                        nrows, nfreqs, fscale, 0, out);
+  DP_LAUNCH_CHECK();
+  if (hmax > 8) {
+    k_fsai_rows<16, 16><<<div_up(total, 16), 16, 0, st>>>(are, aim, a_il, rptrA, cptrA, rptrG,
+                                                          cptrG, nrows, nfreqs, fscale, 8, out);
+    DP_LAUNCH_CHECK();
+  }
+  if (hmax > 16) {
+    k_fsai_rows<32, 4><<<div_up(total, 4), 4, 0, st>>>(are, aim, a_il, rptrA, cptrA, rptrG,
+                                                       cptrG, nrows, nfreqs, fscale, 16, out);
+    DP_LAUNCH_CHECK();
+  }
+  return DOGPUP_OK;
+}
+
+}  // namespace dogpup

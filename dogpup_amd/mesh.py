@@ -1,0 +1,317 @@
+"""DMesh / DOptode: the reference's dMesh and dOptode API for the hot path.
+
+``DMesh`` mirrors ``DOGPUP/dMesh.m:1-247`` (constructor, update_properties,
+add_optode, flu_solve, meas_flu, adj_flu_solve, J_complex) and ``DOptode``
+mirrors ``DOGPUP/dOptode.m:34-120``.  Differences of form, not of meaning:
+indices (elements, links, optodes) are 0-based; arrays are torch tensors on
+the GPU; solver outputs stay device-resident in solver layout (see
+``FluenceField``) and are converted to the reference's NN x NF x NS layout on
+request.  All numerical work of the path (assembly, FSAI factors, BiCGSTAB,
+detector sampling, transforms, Jacobian) runs in libdogpup_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib, topology
+from .transforms import td2fc
+
+DEV = "cuda"
+MODES = {"c128": _lib.MODE_C128, "c64": _lib.MODE_C64_REFINE, "mixed": _lib.MODE_C64_REFINE,
+         "plain": _lib.MODE_C128_PLAIN, "fp64": _lib.MODE_C128}
+
+
+def _i32(a):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=DEV)
+
+
+def _f64(a):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=DEV)
+
+
+def group_width(ncols: int) -> int:
+    """Columns per solver group: the smallest of 8/16/32/64 holding ncols,
+    64 for more (several column blocks per frequency)."""
+    for w in (8, 16, 32, 64):
+        if ncols <= w:
+            return w
+    return 64
+
+
+class DOptode:
+    """Sources, detectors, links and the source spectrum (dOptode.m:34-74)."""
+
+    def __init__(self, s_pos, d_pos, link, avg_pow, tpsf, t_axis, nf):
+        self.link = np.asarray(link, np.int32).reshape(-1, 2)
+        self.Nf = int(nf)
+        t_axis = np.asarray(t_axis, float).reshape(-1)
+        self.tAxis = t_axis
+        self.dt = t_axis[1] - t_axis[0]
+        T = t_axis[-1]                                   # dOptode.m:57
+        self.fAxis = np.arange(nf) * 2.0 * np.pi / T     # :58-59
+        if nf < 2:
+            raise ValueError("Nf must be at least 2 (df = fAxis(2) - fAxis(1))")
+        self.df = self.fAxis[1] - self.fAxis[0]          # :60
+        self.d_positions = np.asarray(d_pos, float).reshape(-1, 3)
+        self.s_positions = np.asarray(s_pos, float).reshape(-1, 3)
+        self.update_tpsf(avg_pow, tpsf, t_axis, nf)
+        self.s_bary = None
+        self.d_bary = None
+
+    def update_tpsf(self, avg_pow, tpsf, t_axis, nf):
+        """dOptode.m:101-120: TPSF scaled to mean avg_pow, s_fpsf = td2fc."""
+        tpsf = np.asarray(tpsf, float).reshape(-1)
+        self.s_avgPow = float(avg_pow)
+        self.s_tpsf = tpsf * (avg_pow / tpsf.mean())
+        sf = td2fc(torch.as_tensor(self.s_tpsf[None, :]), self.fAxis, t_axis, 1)
+        self.s_fpsf = sf[0].cpu().numpy()
+
+    def snap2mesh(self, mesh: "DMesh"):
+        """dOptode.m:77-97: detectors snapped to the surface, sources snapped
+        then moved 1/min(musp) along the inward normal."""
+        d_pos, _ = topology.snap_to_surface(mesh.node, mesh.face, mesh.bnd, self.d_positions)
+        d_id, d_w = topology.locate_points(mesh.node, mesh.elem, d_pos)
+        s_pos, s_dir = topology.snap_to_surface(mesh.node, mesh.face, mesh.bnd, self.s_positions)
+        s_in = s_pos + (1.0 / mesh.musp.min()) * s_dir
+        s_id, s_w = topology.locate_points(mesh.node, mesh.elem, s_in)
+        if (d_id < 0).any() or (s_id < 0).any():
+            raise ValueError("an optode could not be located inside the mesh")
+        self.d_positions, self.d_dirs = d_pos, None
+        self.s_positions, self.s_dirs = s_in, s_dir
+        self.d_bary = (d_id, topology.clean_bary(d_w))
+        self.s_bary = (s_id, topology.clean_bary(s_w))
+        return self
+
+
+@dataclass
+class FluenceField:
+    """Device-resident fluence in solver layout: complex128
+    [NF*NCB][NN][NSP], column c of frequency k at group k*NCB + c//NSP."""
+
+    data: torch.Tensor
+    nn: int
+    nf: int
+    ncols: int
+    nsp: int
+    iters: np.ndarray = None
+    relres: np.ndarray = None
+    status: np.ndarray = None
+    stats: dict = field(default_factory=dict)
+
+    def to_reference_layout(self) -> torch.Tensor:
+        """NN x NF x NS complex128 (the reference's phi layout)."""
+        out = torch.empty((self.ncols, self.nf, self.nn), dtype=torch.complex128, device=DEV)
+        _lib.call("dogpup_layout_from_solver", _lib.ptr(torch.view_as_real(self.data)),
+                  _lib.ptr(torch.view_as_real(out)), self.nn, self.nf, self.ncols, self.nsp,
+                  _lib.stream_ptr())
+        return out.permute(2, 1, 0)
+
+
+class DMesh:
+    """Mesh, optical properties and forward matrices (dMesh.m:48-88)."""
+
+    cVac = topology.C_VAC
+
+    def __init__(self, node, elem, nr, props, reference_faces=True):
+        self.node = np.asarray(node, float)
+        self.elem = np.asarray(elem, np.int64)
+        self.nr = float(nr)
+        nn = self.node.shape[0]
+        props = np.atleast_2d(np.asarray(props, float))
+        self.mua = np.broadcast_to(props[:, 0], (nn,)).copy()      # dMesh.m:68-73
+        self.musp = np.broadcast_to(props[:, 1], (nn,)).copy()
+        self.kappa = 1.0 / (3.0 * (self.mua + self.musp))          # :74
+        self.reference_faces = reference_faces
+        self.optode = None
+        self._solvers = {}
+        self.prepare_mesh()
+        self.update_properties()
+
+    # -- geometry (dMesh.m:945-1030) ------------------------------------------
+    def prepare_mesh(self):
+        nn = self.node.shape[0]
+        self.c = self.cVac / self.nr
+        self.face, self.area = topology.true_faces(self.node, self.elem)
+        self.bnd = np.zeros(nn, np.int32)
+        self.bnd[np.unique(self.face.ravel())] = 1
+        self.R = topology.robin_factor(self.nr) * self.bnd
+        self.bface = topology.robin_faces(self.elem, self.bnd, self.reference_faces, self.face)
+        self.pattern = topology.system_pattern(nn, self.elem, self.bface)
+        rG, cG = topology.fsai_pattern(self.pattern.rptr, self.pattern.cptr)
+        rGT, cGT, perm = topology.transpose_pattern(rG, cG)
+        self.g_pattern = (rG, cG)
+        d = self._dev = {}
+        d["node"] = _f64(self.node)
+        d["elem"] = _i32(self.elem)
+        d["bface"] = _i32(self.bface)
+        d["R"] = _f64(self.R)
+        d["rA"], d["cA"] = _i32(self.pattern.rptr), _i32(self.pattern.cptr)
+        d["slot_ptr"], d["slot_src"] = _i32(self.pattern.slot_ptr), _i32(self.pattern.slot_src)
+        d["rG"], d["cG"] = _i32(rG), _i32(cG)
+        d["rGT"], d["cGT"], d["permGT"] = _i32(rGT), _i32(cGT), _i32(perm)
+        self.nnzA = int(self.pattern.cptr.size)
+        self.nnzG = int(cG.size)
+
+    @property
+    def nn(self):
+        return self.node.shape[0]
+
+    # -- assembly (dMesh.m:80-88, 1033-1194) -----------------------------------
+    def update_properties(self, props=None):
+        if props is not None:
+            props = np.asarray(props, float)
+            self.mua[:] = props[:, 0]
+            self.musp[:] = props[:, 1]
+            self.kappa[:] = 1.0 / (3.0 * (self.mua + self.musp))
+        d = self._dev
+        ne = self.elem.shape[0]
+        self.K = torch.empty(self.nnzA, dtype=torch.float64, device=DEV)   # Re A (full symmetric)
+        self.M = torch.empty(self.nnzA, dtype=torch.float64, device=DEV)   # mass matrix values
+        work = torch.empty(ne * 17, dtype=torch.float64, device=DEV)
+        _lib.call("dogpup_assemble", self.nn, ne, int(self.bface.shape[0]), self.nnzA,
+                  _lib.ptr(d["node"]), _lib.ptr(d["elem"]), _lib.ptr(d["bface"]),
+                  _lib.ptr(_f64(self.mua)), _lib.ptr(_f64(self.kappa)), _lib.ptr(d["R"]),
+                  _lib.ptr(d["slot_ptr"]), _lib.ptr(d["slot_src"]), 1.0, _lib.ptr(work),
+                  _lib.ptr(self.K), _lib.ptr(self.M), _lib.stream_ptr())
+        self._matrix_version = getattr(self, "_matrix_version", 0) + 1
+        return self
+
+    def system_csr(self):
+        """(rptr, cptr, K, M) of the full symmetric system on the device;
+        A_k = K + i*k*(df/c)*M."""
+        d = self._dev
+        return d["rA"], d["cA"], self.K, self.M
+
+    # -- optodes -------------------------------------------------------------
+    def add_optode(self, optode: DOptode):
+        optode.snap2mesh(self)
+        self.optode = optode
+        return self
+
+    def clear_optode(self):
+        self.optode = None
+        return self
+
+    def _bary_dev(self, which):
+        eid, w = self.optode.s_bary if which == "s" else self.optode.d_bary
+        nodes = self.elem[eid]                      # N x 4
+        return _i32(nodes), _f64(w)
+
+    # -- solver -----------------------------------------------------------------
+    def solver(self, ncols, mode="mixed"):
+        key = (ncols, mode)
+        s = self._solvers.get(key)
+        if s is None:
+            s = FluenceSolver(self, ncols, MODES[mode])
+            self._solvers[key] = s
+        return s
+
+    def solve_point_sources(self, which="s", mode="mixed", tol=None, maxit=1000, extra=None):
+        """Solve for the sources ('s'), detectors ('d') or both ('sd', forward
+        and adjoint batched in one launch: columns 0..NS-1 sources,
+        NS..NS+ND-1 detectors)."""
+        parts = [self._bary_dev(w) for w in which]
+        nodes = torch.cat([p[0] for p in parts])
+        w = torch.cat([p[1] for p in parts])
+        ncols = nodes.shape[0]
+        S = self.solver(ncols, mode)
+        spec = torch.as_tensor(self.optode.s_fpsf, dtype=torch.complex128, device=DEV)
+        b = S.empty_block()
+        _lib.call("dogpup_point_sources", _lib.ptr(torch.view_as_real(b)), self.nn, S.nf, ncols, S.nsp,
+                  _lib.ptr(nodes), _lib.ptr(w), _lib.ptr(torch.view_as_real(spec)), _lib.stream_ptr())
+        if tol is None:
+            tol = 1e-12 if MODES[mode] != _lib.MODE_C64_REFINE else 1e-10
+        return S.solve(b, tol=tol, maxit=maxit)
+
+    # -- forward / adjoint (dMesh.m:106-202) ------------------------------------
+    def flu_solve(self, display=False, mode="mixed", tol=None):
+        """phi (FluenceField, NS columns) and data (NM x NF complex)."""
+        phi = self.solve_point_sources("s", mode, tol)
+        return phi, self.meas_flu(phi)
+
+    def adj_flu_solve(self, display=False, mode="mixed", tol=None):
+        return self.solve_point_sources("d", mode, tol)
+
+    def meas_flu(self, phi: FluenceField):
+        """data(m,k) = (D phi_k)(d_m, s_m) (dMesh.m:144-171); NM x NF."""
+        dn, dw = self._bary_dev("d")
+        link = _i32(self.optode.link)
+        nm = link.shape[0]
+        out = torch.empty((phi.nf, nm), dtype=torch.complex128, device=DEV)
+        _lib.call("dogpup_meas", _lib.ptr(torch.view_as_real(phi.data)), _lib.ptr(torch.view_as_real(out)),
+                  _lib.ptr(dn), _lib.ptr(dw), _lib.ptr(link), self.nn, phi.nf, phi.ncols, phi.nsp, nm,
+                  int(dn.shape[0]), _lib.stream_ptr())
+        return out.t()
+
+
+class FluenceSolver:
+    """Device-resident batched FSAI-BiCGSTAB for one mesh, column count and
+    mode (wraps dogpup_solver_*)."""
+
+    def __init__(self, mesh: DMesh, ncols: int, mode: int):
+        if mesh.optode is None:
+            raise ValueError("add an optode first (the frequency axis comes from it)")
+        self.mesh = mesh
+        self.nf = mesh.optode.Nf
+        self.ncols = ncols
+        self.nsp = group_width(ncols)
+        self.ncb = -(-ncols // self.nsp)
+        self.mode = mode
+        d = mesh._dev
+        h = C.c_void_p()
+        _lib.call("dogpup_solver_create", C.byref(h), mesh.nn, self.nf, ncols, self.nsp, mode,
+                  mesh.nnzA, _lib.ptr(d["rA"]), _lib.ptr(d["cA"]), mesh.nnzG, _lib.ptr(d["rG"]),
+                  _lib.ptr(d["cG"]), _lib.ptr(d["rGT"]), _lib.ptr(d["cGT"]), _lib.ptr(d["permGT"]))
+        self.h = h
+        self._version = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib.lib().dogpup_solver_destroy(self.h)
+        except Exception:
+            pass
+
+    def empty_block(self):
+        return torch.zeros((self.nf * self.ncb, self.mesh.nn, self.nsp), dtype=torch.complex128,
+                           device=DEV)
+
+    def ensure_matrix(self):
+        m = self.mesh
+        if self._version == m._matrix_version:
+            return
+        fscale = _f64(np.arange(self.nf) * m.optode.df / m.c)    # A_k = K + i k df/c M
+        _lib.call("dogpup_solver_set_matrix", self.h, _lib.ptr(m.K), _lib.ptr(m.M), _lib.ptr(fscale),
+                  _lib.stream_ptr())
+        torch.cuda.current_stream().synchronize()
+        self._version = m._matrix_version
+
+    def solve(self, b: torch.Tensor, tol=1e-10, maxit=1000, x: torch.Tensor | None = None):
+        self.ensure_matrix()
+        if x is None:
+            x = torch.empty_like(b)
+        n = self.nf * self.ncols
+        iters = np.zeros(n, np.int32)
+        relres = np.zeros(n, np.float64)
+        status = np.zeros(n, np.int32)
+        _lib.call("dogpup_solver_solve", self.h, _lib.ptr(torch.view_as_real(b)),
+                  _lib.ptr(torch.view_as_real(x)), float(tol), int(maxit), _lib.ptr(iters),
+                  _lib.ptr(relres), _lib.ptr(status), _lib.stream_ptr())
+        st = np.zeros(4)
+        _lib.call("dogpup_solver_stats", self.h, _lib.ptr(st), 4)
+        shape = (self.nf, self.ncols)
+        return FluenceField(x, self.mesh.nn, self.nf, self.ncols, self.nsp, iters.reshape(shape),
+                            relres.reshape(shape), status.reshape(shape),
+                            {"max_iters": st[0], "group_iters": st[1], "sweeps": st[2],
+                             "inner_iters": st[3]})
+
+    def fsai_values(self) -> torch.Tensor:
+        """FSAI factors of every frequency, complex128 [NF][nnzG]."""
+        self.ensure_matrix()
+        g = torch.empty((self.nf, self.mesh.nnzG), dtype=torch.complex128, device=DEV)
+        _lib.call("dogpup_solver_get_fsai", self.h, _lib.ptr(torch.view_as_real(g)), _lib.stream_ptr())
+        return g

@@ -1,0 +1,154 @@
+"""Synthetic slab meshes and optode layouts for the frequency-domain diffusion path.
+
+The reference builds its meshes with iso2mesh; the Kuhn split used here has the
+behaviour of ``DOGPUP/meshing/iso2mesh/meshgrid6.m:35-67``: every lattice cube is
+cut into the 6 tetrahedra that share its main diagonal, and the lattice is
+numbered with x fastest (MATLAB ``ndgrid`` order).  Node numbering is 0-based
+here; element orientation is made positive (as ``meshreorient`` does).
+
+Everything is seeded so the oracle and the HIP path see identical inputs.
+"""
+from __future__ import annotations
+
+import itertools
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class SlabMesh:
+    """A Kuhn-split box mesh plus its nodal optical properties."""
+
+    node: np.ndarray          # NN x 3 float64 (mm)
+    elem: np.ndarray          # NE x 4 int32, 0-based
+    mua: np.ndarray           # NN float64 (1/mm)
+    musp: np.ndarray          # NN float64 (1/mm)
+    nr: float                 # refractive index
+    shape: tuple              # (nx, ny, nz) lattice points
+    h: float                  # lattice spacing (mm)
+    extent: tuple             # (Lx, Ly, Lz) box size (mm)
+    meta: dict = field(default_factory=dict)
+
+
+def kuhn_lattice(nx: int, ny: int, nz: int, h: float):
+    """Regular lattice nodes and the 6-tet Kuhn split of every cube.
+
+    Tetrahedron for a permutation (p1, p2, p3) of the axes holds the cube
+    corners v with v[p1] <= v[p2] <= v[p3] (``meshgrid6.m:35-42``).
+    """
+    xs = np.arange(nx) * h
+    ys = np.arange(ny) * h
+    zs = np.arange(nz) * h
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    node = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)  # x fastest
+    stride = np.array([1, nx, nx * ny], dtype=np.int64)
+
+    ci, cj, ck = np.meshgrid(np.arange(nx - 1), np.arange(ny - 1), np.arange(nz - 1), indexing="ij")
+    base = (ci.ravel(order="F") * stride[0] + cj.ravel(order="F") * stride[1]
+            + ck.ravel(order="F") * stride[2]).astype(np.int64)
+    # cube corner offsets (bit b of corner id -> axis b)
+    corners = np.array([[(c >> a) & 1 for a in range(3)] for c in range(8)], dtype=np.int64)
+    tets = []
+    for perm in itertools.permutations(range(3)):
+        verts = [c for c in range(8)
+                 if corners[c][perm[0]] <= corners[c][perm[1]] <= corners[c][perm[2]]]
+        assert len(verts) == 4
+        off = corners[verts] @ stride
+        tets.append(base[:, None] + off[None, :])
+    elem = np.concatenate(tets, axis=0)
+    return node, elem
+
+
+def orient_positive(node: np.ndarray, elem: np.ndarray) -> np.ndarray:
+    """Swap the last two vertices of negatively oriented tetrahedra."""
+    a = node[elem[:, 1]] - node[elem[:, 0]]
+    b = node[elem[:, 2]] - node[elem[:, 0]]
+    c = node[elem[:, 3]] - node[elem[:, 0]]
+    det = np.einsum("ij,ij->i", np.cross(a, b), c)
+    elem = elem.copy()
+    neg = det < 0
+    elem[neg, 2], elem[neg, 3] = elem[neg, 3].copy(), elem[neg, 2].copy()
+    return elem
+
+
+def slab(extent=(60.0, 60.0, 30.0), h=2.0, mua=0.01, musp=1.0, nr=1.4,
+         jitter=0.0, seed=0, inclusions=()):
+    """Kuhn slab ``extent`` mm with spacing ``h``.
+
+    ``jitter`` moves interior nodes by U(-jitter, jitter) mm per axis.
+    ``inclusions`` is a list of (centre xyz, radius, mua) spheres; nodes inside
+    take the inclusion's absorption.
+    """
+    nx, ny, nz = (int(round(e / h)) + 1 for e in extent)
+    node, elem = kuhn_lattice(nx, ny, nz, h)
+    rng = np.random.default_rng(seed)
+    if jitter > 0:
+        ijk = np.stack(np.unravel_index(np.arange(node.shape[0]), (nz, ny, nx))[::-1], axis=1)
+        interior = np.all((ijk > 0) & (ijk < np.array([nx - 1, ny - 1, nz - 1])), axis=1)
+        node = node.copy()
+        node[interior] += rng.uniform(-jitter, jitter, size=(int(interior.sum()), 3))
+    elem = orient_positive(node, elem).astype(np.int32)
+    mua_n = np.full(node.shape[0], float(mua))
+    musp_n = np.full(node.shape[0], float(musp))
+    for centre, radius, mua_inc in inclusions:
+        inside = np.linalg.norm(node - np.asarray(centre, float)[None, :], axis=1) <= radius
+        mua_n[inside] = mua_inc
+    return SlabMesh(node=node, elem=elem, mua=mua_n, musp=musp_n, nr=float(nr),
+                    shape=(nx, ny, nz), h=float(h), extent=tuple(float(e) for e in extent),
+                    meta={"seed": seed, "jitter": jitter, "inclusions": list(inclusions)})
+
+
+def top_grid(extent, nx, ny, x0, y0, dx, dy):
+    """Points on the top face z = Lz on an nx-by-ny grid (x fastest)."""
+    xs = x0 + dx * np.arange(nx)
+    ys = y0 + dy * np.arange(ny)
+    Y, X = np.meshgrid(ys, xs, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), np.full(X.size, extent[2])], axis=1)
+
+
+def all_pairs(ns: int, nd: int) -> np.ndarray:
+    """Link table [source, detector] (0-based) for every pair, source fastest."""
+    s, d = np.meshgrid(np.arange(ns), np.arange(nd), indexing="xy")
+    return np.stack([s.ravel(), d.ravel()], axis=1).astype(np.int32)
+
+
+# --- named workloads ------------------------------------------------------------
+
+def slab_60x60x30_h2(seed=0):
+    """60x60x30 mm slab, 2 mm Kuhn grid (15,376 nodes), interior jitter 0.2 mm,
+    mua 0.01/mm, musp 1.0/mm, n 1.4; 16 sources and 16 detectors on 4x4 grids
+    of the top face."""
+    m = slab((60.0, 60.0, 30.0), 2.0, 0.01, 1.0, 1.4, jitter=0.2, seed=seed)
+    src = top_grid(m.extent, 4, 4, 12.0, 12.0, 12.0, 12.0)
+    det = top_grid(m.extent, 4, 4, 18.0, 18.0, 12.0, 12.0)
+    return m, src, det, all_pairs(16, 16)
+
+
+def slab_80x80x40_h1(seed=0, inclusion=True):
+    """80x80x40 mm slab, 1 mm Kuhn grid (269,001 nodes, 1,536,000 tets),
+    mua 0.01/mm, musp 1.0/mm, n 1.4, one 6 mm sphere of mua 0.03/mm at a seeded
+    depth of 10-25 mm below the top face; 32 sources and 32 detectors on two
+    interleaved 4x8 grids of the top face."""
+    rng = np.random.default_rng(seed)
+    depth = rng.uniform(10.0, 25.0)
+    inc = [((40.0, 40.0, 40.0 - depth), 6.0, 0.03)] if inclusion else []
+    m = slab((80.0, 80.0, 40.0), 1.0, 0.01, 1.0, 1.4, jitter=0.0, seed=seed, inclusions=inc)
+    src = top_grid(m.extent, 4, 8, 16.0, 5.0, 16.0, 10.0)
+    det = top_grid(m.extent, 4, 8, 24.0, 10.0, 16.0, 10.0)
+    return m, src, det, all_pairs(32, 32)
+
+
+def gaussian_tpsf(t, fwhm=250e-12, centre_sigmas=6.0):
+    """Gaussian source pulse sampled on ``t`` (s)."""
+    sd = fwhm / (2.0 * np.sqrt(2.0 * np.log(2.0)))
+    return np.exp(-((t - centre_sigmas * sd) ** 2) / (2.0 * sd ** 2))
+
+
+def boxcar_gates(t, n_gates=8, width=500e-12, spacing=200e-12, start=0.0):
+    """n_gates boxcar time gates of ``width`` s starting ``spacing`` s apart."""
+    g = np.zeros((n_gates, t.size))
+    for i in range(n_gates):
+        t0 = start + i * spacing
+        g[i] = ((t >= t0) & (t < t0 + width)).astype(float)
+    return g

@@ -1,0 +1,234 @@
+/*
+ * dogpup.h -- C ABI of libdogpup_hip.so, the MI355X (gfx950) implementation of
+ * DOGPUP's frequency-domain diffuse-optics hot path.
+ *
+ * Conventions
+ *  - Every entry point returns an int status: DOGPUP_OK (0) or a negative
+ *    DOGPUP_ERR_*; dogpup_last_error() gives the message of the last failure on
+ *    the calling thread.  No C++ exception crosses this boundary.
+ *  - CSR arrays are int32 and 0-based, as the reference's sparse_csr produces
+ *    (DOGPUP/dMesh.m:1388-1414).
+ *  - "dev" pointers are device (HBM) pointers, "host" pointers are host memory.
+ *    `stream` is a hipStream_t (NULL = default stream); device entry points
+ *    are asynchronous on it unless stated otherwise.
+ *  - Complex data are interleaved (re, im) doubles or floats unless an entry
+ *    point mirrors a reference launcher with planar re/im arrays.
+ *  - Reference layout ("MATLAB layout") of a multi-RHS block: NN x NS x NF
+ *    column-major, element (row, col, freq) at freq*NS*NN + col*NN + row
+ *    (smv_CUDA.cu:32,41).
+ *  - Solver layout of a multi-RHS block: groups of NSP columns, one group per
+ *    (frequency, column block): element (g, node, c) at (g*NN + node)*NSP + c,
+ *    g = freq*NCB + col/NSP, c = col % NSP, NSP in {8,16,32,64}.
+ */
+#ifndef DOGPUP_H
+#define DOGPUP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DOGPUP_OK 0
+#define DOGPUP_ERR_ARG -1
+#define DOGPUP_ERR_HIP -2
+#define DOGPUP_ERR_UNSUPPORTED -3
+#define DOGPUP_ERR_NOMEM -4
+
+/* per-RHS solver status */
+#define DOGPUP_RHS_CONVERGED 0
+#define DOGPUP_RHS_MAXIT 1
+#define DOGPUP_RHS_BREAKDOWN 2
+#define DOGPUP_RHS_ZERO 3 /* zero right-hand side: x = 0 */
+
+/* solver precision / structure modes */
+#define DOGPUP_MODE_C128 0          /* complex128 fused BiCGSTAB, per-RHS convergence */
+#define DOGPUP_MODE_C64_REFINE 1    /* complex64 fused BiCGSTAB inside FP64 iterative refinement */
+#define DOGPUP_MODE_C128_PLAIN 2    /* complex128, reference structure: unfused passes, lockstep */
+
+const char* dogpup_last_error(void);
+int dogpup_version(void);
+int dogpup_device_count(int* count);
+
+/* ---------------------------------------------------------------------------
+ * Reference-compatible launchers: device pointers, planar FP64 re/im arrays,
+ * reference layout.  Same argument order and meaning as the reference's
+ * launchers, plus a trailing stream.
+ * ------------------------------------------------------------------------- */
+
+/* Replaces smv_call (DOGPUP/cuda_lib/mex_sources/smvMex.hpp:4-15,
+ * smv_CUDA.cu:59-82): y(:,c,k) = (Re A + i*fAxis[k]*Im A) x(:,c,k). */
+int dogpup_smv_call(double* d_out_real, double* d_out_imag, const double* d_x_real,
+                    const double* d_x_imag, const double* d_fAxis, const double* d_valA_real,
+                    const double* d_valA_imag, const int* d_rPtr, const int* d_cPtr, int nrows,
+                    int ncols, int nfreqs, void* stream);
+
+/* Replaces FSAImv_call (FSAImvMex.hpp:4-15, FSAImv_CUDA.cu:58-81):
+ * y(:,c,k) = G_k x(:,c,k), G_k values at valG[nnz*k + i]. */
+int dogpup_fsaimv_call(double* d_out_real, double* d_out_imag, const double* d_x_real,
+                       const double* d_x_imag, const double* d_valG_real,
+                       const double* d_valG_imag, const int* d_rPtr, const int* d_cPtr, int nnz,
+                       int nrows, int ncols, int nfreqs, void* stream);
+
+/* Replaces FSAIP_call (FSAIP.hpp:4-13, FSAIP_CUDA.cu:162-180): FSAI factor rows
+ * of A_k = Re A + i*k*Im A (A given as its lower triangle tril(A) in CSR) for
+ * k = 0..nfreqs-1 on the pattern (rPtrG, cPtrG) (ascending columns, diagonal
+ * last).  Output NF x nnzG planar, frequency fastest: out[k + nfreqs*j].
+ * Rows with more than 32 pattern entries return DOGPUP_ERR_UNSUPPORTED. */
+int dogpup_fsaip_call(double* d_out_real, double* d_out_imag, const double* d_valA_real,
+                      const double* d_valA_imag, const int* d_rPtrA, const int* d_cPtrA,
+                      const int* d_rPtrG, const int* d_cPtrG, int nfreqs, int nrows,
+                      void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Mex-level entry points: host arrays in the interleaved-complex MATLAB
+ * layout (R2018a complex API), synchronous.  A MATLAB mex gateway for AMD
+ * forwards its mxArrays here (gpuArray is CUDA-only).
+ * ------------------------------------------------------------------------- */
+
+/* y = smv_mex(x, f, A, rPtrA, cPtrA)  (smv_mexgateway.cpp:8-100).
+ * x, y: 2*nrows*ncols*nfreqs doubles; f: nfreqs doubles; A: 2*nnz doubles. */
+int dogpup_smv_mex(double* h_y, const double* h_x, const double* h_f, const double* h_A,
+                   const int* h_rPtrA, const int* h_cPtrA, int nrows, int ncols, int nfreqs,
+                   int nnz);
+
+/* y = FSAImv_mex(x, G, rPtrG, cPtrG)  (FSAImv_mexgateway.cpp:8-100).
+ * G: nnzG x nfreqs complex, column-major (2*nnzG*nfreqs doubles). */
+int dogpup_fsaimv_mex(double* h_y, const double* h_x, const double* h_G, const int* h_rPtrG,
+                      const int* h_cPtrG, int nrows, int ncols, int nfreqs, int nnzG);
+
+/* valG = FSAIP_gen_mex(valA, rPtrA, cPtrA, rPtrG, cPtrG, nfreqs)
+ * (FSAIP_mexgateway.cpp:8-85).  valA: tril(A) CSR values (2*nnzA doubles);
+ * output NF x nnzG complex column-major (2*nfreqs*nnzG doubles). */
+int dogpup_fsaip_gen_mex(double* h_valG, const double* h_valA, const int* h_rPtrA,
+                         const int* h_cPtrA, const int* h_rPtrG, const int* h_cPtrG,
+                         int nrows, int nnzA, int nnzG, int nfreqs);
+
+/* ---------------------------------------------------------------------------
+ * FEM assembly (replaces gen_fwdmat, dMesh.m:1033-1194, and the formation of
+ * the full complex system in fluGPU, dMesh.m:1241-1242).
+ *
+ * Produces the values of the full symmetric A on a CSR pattern prepared by
+ * the host: valA_re = K (stiffness(kappa) + absorption mass(mua) + Robin
+ * term), valA_im = scale_im * M (scale_im = df/c, unit frequency).
+ * slot_ptr (nnz+1) / slot_src (contributions) list, per CSR slot, the element
+ * (code e*16 + a*4 + b) and Robin-face (code NE*16 + f*9 + a*3 + b)
+ * contributions that sum into it.  work: NE*17 doubles of device scratch.
+ * ------------------------------------------------------------------------- */
+int dogpup_assemble(int nn, int ne, int nbf, int nnz, const double* d_node, const int* d_elem,
+                    const int* d_bface, const double* d_mua, const double* d_kappa,
+                    const double* d_R, const int* d_slot_ptr, const int* d_slot_src,
+                    double scale_im, double* d_work, double* d_valA_re, double* d_valA_im,
+                    void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Batched FSAI-BiCGSTAB over every (frequency, column) right-hand side
+ * (replaces gpuBicstab_FSAIP, dMesh.m:1247-1348, and FSAIP_gen's factor call,
+ * dMesh.m:1384).  A solver handle keeps the pattern, the matrix, the FSAI
+ * factors of every frequency and all workspace resident in HBM.
+ * ------------------------------------------------------------------------- */
+typedef struct dogpup_solver* dogpup_solver_t;
+
+/* d_rptrA/d_cptrA: full symmetric CSR of A (sorted columns).  d_rptrG/d_cptrG:
+ * FSAI pattern (lower, ascending, diagonal last).  d_rptrGT/d_cptrGT/d_permGT:
+ * CSR of G^T and the map from G^T slots to G slots.  ncols = columns per
+ * frequency (sources); nsp = group width (8/16/32/64, >= columns per block).
+ * The handle keeps the pattern pointers (caller keeps them alive). */
+int dogpup_solver_create(dogpup_solver_t* out, int nn, int nfreq, int ncols, int nsp, int mode,
+                         int nnzA, const int* d_rptrA, const int* d_cptrA, int nnzG,
+                         const int* d_rptrG, const int* d_cptrG, const int* d_rptrGT,
+                         const int* d_cptrGT, const int* d_permGT);
+int dogpup_solver_destroy(dogpup_solver_t s);
+
+/* Set A (FP64 values on the handle's pattern; frequency k uses
+ * valA_re + i*fscale[k]*valA_im) and compute the FSAI factors of every
+ * frequency (FP64), G^T, and their complex64 copies when the mode needs them.
+ * d_fscale: nfreq doubles (the reference uses k = 0..NF-1). */
+int dogpup_solver_set_matrix(dogpup_solver_t s, const double* d_valA_re, const double* d_valA_im,
+                             const double* d_fscale, void* stream);
+
+/* Solve A_k x = b for every RHS.  b, x: complex128 in solver layout (device).
+ * tol: relative residual ||b - A x|| / ||b|| per RHS (FP64 true residual in
+ * C64_REFINE mode); maxit: BiCGSTAB iterations (all sweeps together in
+ * C64_REFINE).  Outputs (host, nfreq*ncols each, index freq*ncols + col):
+ * iterations, final relative residual, status.  Synchronous. */
+int dogpup_solver_solve(dogpup_solver_t s, const double* d_b, double* d_x, double tol, int maxit,
+                        int* h_iters, double* h_relres, int* h_status, void* stream);
+
+/* Counters of the last solve: number of vector-kernel passes weighted by the
+ * columns they moved, for roofline accounting.  h_out[0] = BiCGSTAB
+ * iterations (max), [1] = sum over groups of active iterations (c64 or
+ * c128), [2] = FP64 refinement sweeps, [3] = inner sweeps. */
+int dogpup_solver_stats(dogpup_solver_t s, double* h_out, int n);
+
+/* Per-launch timing of the next solves (HIP events on the solve's stream).
+ * dogpup_solver_profile fills h_out[3*c + {0,1,2}] = {milliseconds,
+ * algorithmic bytes, launches} for kernel class c: 0 = G/G^T SpMV, 1 = A
+ * SpMV, 2 = BiCGSTAB vector passes, 3 = FP64 refinement passes. */
+int dogpup_solver_set_profile(dogpup_solver_t s, int on);
+int dogpup_solver_profile(dogpup_solver_t s, double* h_out, int n);
+
+/* Copy the FSAI factors held by the handle (FP64, [freq][nnzG] interleaved
+ * complex, 2*nfreq*nnzG doubles) into d_out. */
+int dogpup_solver_get_fsai(dogpup_solver_t s, double* d_out, void* stream);
+
+/* Layout transforms between MATLAB layout (NN x NF x NS column-major,
+ * interleaved complex128) and solver layout. */
+int dogpup_layout_to_solver(const double* d_src, double* d_dst, int nn, int nfreq, int ncols,
+                            int nsp, void* stream);
+int dogpup_layout_from_solver(const double* d_src, double* d_dst, int nn, int nfreq, int ncols,
+                              int nsp, void* stream);
+
+/* Point-source right-hand sides in solver layout (flu_solve / adj_flu_solve
+ * source vectors, dMesh.m:124-129,191-196): b(g,node,c) = sum_q w[c][q] *
+ * spectrum[freq] for node = nodes[c][q]; columns >= ncols are zero. */
+int dogpup_point_sources(double* d_b, int nn, int nfreq, int ncols, int nsp, const int* d_nodes,
+                         const double* d_w, const double* d_spectrum, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Fourier-series transforms along the leading axis of a block with `ncol`
+ * contiguous columns (the reference permutes the transformed dimension to
+ * the front the same way).  f: angular frequencies (rad/s), t: time axis (s),
+ * device arrays.  Synchronous on `stream`.
+ * ------------------------------------------------------------------------- */
+
+/* Replaces td2fc (DOGPUP/td2fc.m:1-39): out(k,col) = dt/P sum_t in(t,col)
+ * exp(-i f_k t), P = t(end).  in: NT x ncol real (in_complex = 0) or complex;
+ * out: NF x ncol complex. */
+int dogpup_td2fc(const double* d_in, double* d_out, const double* d_f, const double* d_t, int nt,
+                 int nf, int ncol, int in_complex, void* stream);
+
+/* Replaces fc2td (fc2td.m:1-37): out(t,col) = 2 Re sum_k w_k in(k,col)
+ * exp(i f_k t), w_0 = 1/2.  in: NF x ncol complex; out: NT x ncol real. */
+int dogpup_fc2td(const double* d_in, double* d_out, const double* d_f, const double* d_t, int nf,
+                 int nt, int ncol, void* stream);
+
+/* Replaces fc2tg (fc2tg.m:1-38): out(g,col) = nt_factor * 2 Re sum_k w_k
+ * conj(gates(g,k)) in(k,col).  gates: NG x NF complex (row-major);
+ * nt_factor = Nt (number of time samples). */
+int dogpup_fc2tg(const double* d_in, double* d_out, const double* d_gates, int nf, int ng, int ncol,
+                 double nt_factor, void* stream);
+
+/* Replaces the detector sampling of meas_flu (dMesh.m:144-171):
+ * out(m,k) = sum_q dw[d_m][q] phi(dnodes[d_m][q], k, s_m).  phi: solver
+ * layout (NS columns); dnodes/dw: ND x 4; link: NM x 2 int32 (source,
+ * detector), 0-based; out: NM x NF complex128, m fastest. */
+int dogpup_meas(const double* d_phi, double* d_out, const int* d_dnodes, const double* d_dw,
+                const int* d_link, int nn, int nf, int ncols, int nsp, int nm, int nd, void* stream);
+
+/* Replaces J_complex followed by fc2tg (dMesh.m:206-247, wghtd_LM_recon.m:
+ * 58-61) on the nodal basis (m2g = I):
+ * out[v][g][m] = scale(m,g) * sum_k Re(coef[g][k] phi(v,k,s_m) phi(v,k,d_m)),
+ * coef[g][k] = -2 Nt w_k conj(gate(g,k)) dxyz^3 / s_fpsf(k) (complex128,
+ * NG x NF).  d_field: forward and adjoint fluence of one batched solve,
+ * complex128 solver layout (ncols columns per frequency, group width nsp);
+ * link: NM x 2 int32 = (source column, detector column) in that field;
+ * scale: NG x NM float (m fastest) or NULL; out: float, NM x NG x NV
+ * column-major (node v = 0..nv-1).  Limits: ng <= 16,
+ * (nf*ncols + ng*nf)*8 <= 60 KiB. */
+int dogpup_jacobian_tg(const double* d_field, const double* d_coef, const int* d_link,
+                       const float* d_scale, float* d_out, int nn, int nf, int ng, int ncols, int nsp,
+                       int nm, int nv, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DOGPUP_H */

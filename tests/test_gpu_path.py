@@ -1,0 +1,203 @@
+"""GPU parity of the HIP path against the FP64 oracle (small meshes).
+
+Every call goes through libdogpup_hip.so's C ABI; the oracle is the checker.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import rel, small_slab
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+@pytest.fixture(scope="module")
+def setup():
+    _need_gpu()
+    from dogpup_amd import meshgen, mesh as dm
+    from oracle import fem, optode as oopt, transforms as otr
+    m = small_slab(seed=3, inclusions=[((10.0, 10.0, 5.0), 3.0, 0.03)])
+    t = np.arange(0, 501) * 20e-12
+    nf = 4
+    src = meshgen.top_grid(m.extent, 2, 2, 5.0, 5.0, 10.0, 10.0)
+    det = meshgen.top_grid(m.extent, 3, 2, 4.0, 7.0, 6.0, 6.0)
+    link = meshgen.all_pairs(4, 6)
+    tpsf = meshgen.gaussian_tpsf(t, 250e-12)
+    D = dm.DMesh(m.node, m.elem, m.nr, np.stack([m.mua, m.musp], 1))
+    opt = dm.DOptode(src, det, link, 1e3, tpsf, t, nf)
+    D.add_optode(opt)
+    om = fem.make_mesh(m.node, m.elem, m.nr, m.mua, m.musp)
+    f, df = otr.freq_axis(t, nf)
+    A = fem.full_system(om, df)
+    return dict(m=m, D=D, opt=opt, om=om, A=A, t=t, f=f, df=df, nf=nf, link=link, src=src, det=det)
+
+
+def dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a)).to("cuda", dtype)
+
+
+def test_assembly_matches_oracle(setup):
+    D, A = setup["D"], setup["A"]
+    om = setup["om"]
+    rA, cA, K, M = D.system_csr()
+    Kg = sp.csr_matrix((K.cpu().numpy(), cA.cpu().numpy(), rA.cpu().numpy()), shape=A.shape)
+    Mg = sp.csr_matrix((M.cpu().numpy(), cA.cpu().numpy(), rA.cpu().numpy()), shape=A.shape)
+    Kf = (om.K + sp.triu(om.K, 1).T).tocsr()
+    Mf = (om.M + sp.triu(om.M, 1).T).tocsr()
+    assert abs(Kg - Kf).max() <= 1e-13 * abs(Kf).max()
+    assert abs(Mg - Mf).max() <= 1e-13 * abs(Mf).max()
+
+
+def test_compat_smv_fsaimv(setup):
+    from dogpup_amd import _lib
+    from oracle import fsai as ofsai, solver as osol
+    A = setup["A"].tocsr()
+    A.sort_indices()
+    n, nf, ns = A.shape[0], 3, 5
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(n, ns, nf)) + 1j * rng.normal(size=(n, ns, nf))
+    f = np.array([0.0, 1.0, 2.5])
+    ref = osol.smv(A, f, X)
+    # planar device launcher (smv_call), reference layout x(row, col, freq)
+    xr = dev(np.transpose(X, (2, 1, 0)).real.copy())
+    xi = dev(np.transpose(X, (2, 1, 0)).imag.copy())
+    yr, yi = torch.empty_like(xr), torch.empty_like(xi)
+    _lib.call("dogpup_smv_call", _lib.ptr(yr), _lib.ptr(yi), _lib.ptr(xr), _lib.ptr(xi), _lib.ptr(dev(f)),
+              _lib.ptr(dev(A.data.real)), _lib.ptr(dev(A.data.imag)), _lib.ptr(dev(A.indptr, torch.int32)),
+              _lib.ptr(dev(A.indices, torch.int32)), n, ns, nf, None)
+    torch.cuda.synchronize()
+    got = np.transpose(yr.cpu().numpy() + 1j * yi.cpu().numpy(), (2, 1, 0))
+    assert rel(got, ref) < 1e-14
+    # mex-level (host interleaved, MATLAB column-major NN x NS x NF)
+    xh = np.asfortranarray(X).ravel(order="F")
+    yh = np.zeros(2 * xh.size)
+    Ah = np.ascontiguousarray(A.data)
+    _lib.call("dogpup_smv_mex", _lib.ptr(yh), _lib.ptr(xh.view(np.float64)), _lib.ptr(f), _lib.ptr(Ah.view(np.float64)),
+              _lib.ptr(A.indptr.astype(np.int32)), _lib.ptr(A.indices.astype(np.int32)), n, ns, nf, A.nnz)
+    got2 = yh.view(np.complex128).reshape((n, ns, nf), order="F")
+    assert rel(got2, ref) < 1e-14
+    # FSAImv with per-frequency values
+    rG, cG, valG = ofsai.fsaip_gen_batched(A, nf)
+    refG = osol.fsaimv(rG, cG, valG, X)
+    Gh = np.asfortranarray(valG).ravel(order="F")
+    yh = np.zeros(2 * xh.size)
+    _lib.call("dogpup_fsaimv_mex", _lib.ptr(yh), _lib.ptr(xh.view(np.float64)), _lib.ptr(Gh.view(np.float64)),
+              _lib.ptr(rG), _lib.ptr(cG), n, ns, nf, cG.size)
+    got3 = yh.view(np.complex128).reshape((n, ns, nf), order="F")
+    assert rel(got3, refG) < 1e-14
+
+
+def test_compat_fsaip_gen(setup):
+    """FSAIP_gen_mex vs the oracle's per-row partial-pivot GE (brute force)."""
+    from dogpup_amd import _lib
+    from oracle import fsai as ofsai
+    A = setup["A"]
+    nf = 3
+    rG, cG, valG = ofsai.fsaip_gen(A, nf)
+    At = sp.tril(A).tocsr()
+    At.sort_indices()
+    out = np.zeros(2 * nf * cG.size)
+    vA = np.ascontiguousarray(At.data)
+    _lib.call("dogpup_fsaip_gen_mex", _lib.ptr(out), _lib.ptr(vA.view(np.float64)),
+              _lib.ptr(At.indptr.astype(np.int32)), _lib.ptr(At.indices.astype(np.int32)), _lib.ptr(rG),
+              _lib.ptr(cG), A.shape[0], At.nnz, cG.size, nf)
+    got = out.view(np.complex128).reshape((nf, cG.size), order="F").T
+    assert rel(got, valG) < 1e-13
+
+
+@pytest.mark.parametrize("mode,tol,bound", [("fp64", 1e-12, 1e-9), ("mixed", 1e-10, 1e-7),
+                                            ("plain", 1e-12, 1e-9)])
+def test_solver_vs_direct(setup, mode, tol, bound):
+    from oracle import optode as oopt, solver as osol
+    D, A, om = setup["D"], setup["A"], setup["om"]
+    opt = setup["opt"]
+    sid, sw = opt.s_bary
+    Q = oopt.source_matrix(om, sid, sw, opt.s_fpsf)
+    ref = osol.direct(A, Q)
+    phi = D.solve_point_sources("s", mode=mode, tol=tol)
+    got = phi.to_reference_layout().cpu().numpy()
+    assert (phi.status == 0).all(), phi.status
+    for k in range(setup["nf"]):
+        for s in range(Q.shape[2]):
+            assert rel(got[:, k, s], ref[:, k, s]) < bound, (mode, k, s)
+
+
+def test_solver_fsai_factors(setup):
+    from oracle import fsai as ofsai
+    D, A = setup["D"], setup["A"]
+    S = D.solver(4, "fp64")
+    S.ensure_matrix()
+    nf = setup["nf"]
+    fscale = np.arange(nf) * setup["df"] / D.c
+    rG, cG, valG = ofsai.fsaip_gen_batched(sp.csr_matrix(A.real) + 1j * sp.csr_matrix(A.imag / (setup["df"] / D.c)), nf, fscale=fscale)
+    got = S.fsai_values().cpu().numpy().T
+    assert (np.asarray(D.g_pattern[0]) == rG).all() and (np.asarray(D.g_pattern[1]) == cG).all()
+    assert rel(got, valG) < 1e-12
+
+
+def test_meas_and_transforms(setup):
+    from dogpup_amd import transforms as tr
+    from oracle import optode as oopt, solver as osol, transforms as otr
+    D, A, om, opt = setup["D"], setup["A"], setup["om"], setup["opt"]
+    sid, sw = opt.s_bary
+    did, dw = opt.d_bary
+    Q = oopt.source_matrix(om, sid, sw, opt.s_fpsf)
+    ref_phi = osol.direct(A, Q)
+    ref = oopt.meas_flu(om, did, dw, setup["link"], ref_phi)
+    phi, data = D.flu_solve(mode="fp64")
+    assert rel(data.cpu().numpy(), ref) < 1e-9
+    t, f = setup["t"], setup["f"]
+    # td2fc of the source pulse
+    ref_sf = otr.td2fc(opt.s_tpsf[None, :], f, t, 1)[0]
+    assert rel(opt.s_fpsf, ref_sf) < 1e-13
+    td = tr.fc2td(torch.as_tensor(ref), f, t[:300], 1).cpu().numpy()
+    assert rel(td, otr.fc2td(ref, f, t[:300], 1)) < 1e-12
+    gates = np.stack([((t >= a) & (t < a + 500e-12)).astype(float) for a in (0.0, 200e-12, 400e-12)])
+    gfc = otr.td2fc(gates, f, t, 1)
+    gfc_gpu = tr.td2fc(torch.as_tensor(gates), f, t, 1).cpu().numpy()
+    assert rel(gfc_gpu, gfc) < 1e-13
+    tg = tr.fc2tg(torch.as_tensor(ref), gfc, f, t.size, 1).cpu().numpy()
+    assert rel(tg, otr.fc2tg(ref, gfc, f, t.size, 1)) < 1e-12
+
+
+def test_jacobian_tg(setup):
+    from dogpup_amd import _lib
+    from oracle import jacobian as ojac, optode as oopt, solver as osol, transforms as otr
+    D, A, om, opt = setup["D"], setup["A"], setup["om"], setup["opt"]
+    t, f, nf = setup["t"], setup["f"], setup["nf"]
+    sid, sw = opt.s_bary
+    did, dw = opt.d_bary
+    phi_ref = osol.direct(A, oopt.source_matrix(om, sid, sw, opt.s_fpsf))
+    phiA_ref = osol.direct(A, oopt.source_matrix(om, did, dw, opt.s_fpsf))
+    gates = np.stack([((t >= a) & (t < a + 500e-12)).astype(float) for a in (0.0, 200e-12, 400e-12)])
+    gfc = otr.td2fc(gates, f, t, 1)
+    link = setup["link"]
+    dx = 2.0
+    ref = ojac.j_tg(phi_ref, phiA_ref, link, opt.s_fpsf, dx, gfc, f, t.size)   # NM x NG x NV
+    fw = D.solve_point_sources("sd", mode="fp64")
+    ns, nd = len(sid), len(did)
+    w = np.where(np.arange(nf) == 0, 0.5, 1.0)
+    coef = -2.0 * t.size * w[None, :] * np.conj(gfc) * dx ** 3 / opt.s_fpsf[None, :]
+    ng, nm, nn = gfc.shape[0], link.shape[0], D.nn
+    # forward and adjoint live in one block: columns 0..ns-1 and ns..ns+nd-1
+    phi_s = torch.view_as_real(fw.data)
+    out = torch.empty((nn, ng, nm), dtype=torch.float32, device="cuda")
+    link_all = np.stack([link[:, 0], link[:, 1] + ns], 1)
+    _lib.call("dogpup_jacobian_tg", _lib.ptr(phi_s),
+              _lib.ptr(torch.view_as_real(dev(coef, torch.complex128))), _lib.ptr(dev(link_all, torch.int32)),
+              None, _lib.ptr(out), nn, nf, ng, ns + nd, fw.nsp, nm, nn, _lib.stream_ptr())
+    got = out.cpu().numpy()                      # [v][g][m]
+    assert rel(np.transpose(got, (2, 1, 0)), ref) < 1e-5
+    # the package wrapper gives the same tensor
+    from dogpup_amd.jacobian import jacobian_tg
+    J2, _ = jacobian_tg(D, torch.as_tensor(gfc), mode="fp64", tol=1e-12, dxyz=dx, field=fw)
+    assert torch.equal(J2, out)
