@@ -97,13 +97,27 @@ def call(name, *args):
     return rc
 
 
+class _Ptr(C.c_void_p):
+    """void* that keeps the array it points into alive for as long as the
+    argument tuple of the call holds it (temporaries stay valid during the
+    call; device work queued on torch's current stream stays valid after it,
+    by the caching allocator's stream ordering)."""
+
+
 def ptr(t):
     """Device/host pointer of a torch tensor or numpy array (None -> NULL)."""
     if t is None:
         return None
     if hasattr(t, "data_ptr"):
-        return C.c_void_p(t.data_ptr())
-    return C.c_void_p(t.ctypes.data)
+        if not t.is_contiguous():
+            raise ValueError("non-contiguous tensor passed to the C ABI")
+        p = _Ptr(t.data_ptr())
+    else:
+        if not t.flags["C_CONTIGUOUS"]:
+            raise ValueError("non-contiguous array passed to the C ABI")
+        p = _Ptr(t.ctypes.data)
+    p.keep = t
+    return p
 
 
 def stream_ptr(stream=None):

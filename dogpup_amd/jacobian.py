@@ -45,7 +45,7 @@ def tg_coefficients(optode, gate_fc, dxyz):
     w = np.where(np.arange(nf) == 0, 0.5, 1.0)
     g = gate_fc.cpu().numpy() if torch.is_tensor(gate_fc) else np.asarray(gate_fc)
     coef = -2.0 * optode.tAxis.size * w[None, :] * np.conj(g) * dxyz ** 3 / optode.s_fpsf[None, :]
-    return torch.as_tensor(coef, dtype=torch.complex128, device=DEV)
+    return torch.as_tensor(np.ascontiguousarray(coef), dtype=torch.complex128, device=DEV)
 
 
 def jacobian_tg(mesh, gate_fc, mode="mixed", tol=1e-10, dxyz=None, scale=None, field=None):
@@ -77,4 +77,4 @@ def jacobian_tg(mesh, gate_fc, mode="mixed", tol=1e-10, dxyz=None, scale=None, f
               _lib.stream_ptr())
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    return out, {"solve_s": t1 - t0, "kernel_s": t2 - t1, "field": field}
+    return out, {"solve_s": t1 - t0, "kernel_s": t2 - t1, "field": field, "coef": coef, "link": link}

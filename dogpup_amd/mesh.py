@@ -172,9 +172,10 @@ class DMesh:
         self.K = torch.empty(self.nnzA, dtype=torch.float64, device=DEV)   # Re A (full symmetric)
         self.M = torch.empty(self.nnzA, dtype=torch.float64, device=DEV)   # mass matrix values
         work = torch.empty(ne * 17, dtype=torch.float64, device=DEV)
+        mua_d, kappa_d = _f64(self.mua), _f64(self.kappa)
         _lib.call("dogpup_assemble", self.nn, ne, int(self.bface.shape[0]), self.nnzA,
                   _lib.ptr(d["node"]), _lib.ptr(d["elem"]), _lib.ptr(d["bface"]),
-                  _lib.ptr(_f64(self.mua)), _lib.ptr(_f64(self.kappa)), _lib.ptr(d["R"]),
+                  _lib.ptr(mua_d), _lib.ptr(kappa_d), _lib.ptr(d["R"]),
                   _lib.ptr(d["slot_ptr"]), _lib.ptr(d["slot_src"]), 1.0, _lib.ptr(work),
                   _lib.ptr(self.K), _lib.ptr(self.M), _lib.stream_ptr())
         self._matrix_version = getattr(self, "_matrix_version", 0) + 1

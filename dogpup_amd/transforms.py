@@ -69,14 +69,14 @@ def fc2tg(fun_fc, gate_fc, f, nt, dim):
         raise ValueError("Function dimensions and given dimension mismatch")
     if f.size != src.shape[dim]:
         raise ValueError("Size along given dimension does not match given fourier coefficients")
-    g = _dev(gate_fc, torch.complex128)
+    g = _dev(gate_fc, torch.complex128).contiguous()
     if g.dim() != 2 or g.shape[1] != f.size:
         raise ValueError("gateFC must be NG x NF")
     x, shp = _front(_dev(src, torch.complex128), dim)
     ncol = int(np.prod(shp[1:], dtype=np.int64))
     out = torch.empty((g.shape[0],) + tuple(shp[1:]), dtype=torch.float64, device=DEV)
     _lib.call("dogpup_fc2tg", _lib.ptr(torch.view_as_real(x)), _lib.ptr(out),
-              _lib.ptr(torch.view_as_real(g.contiguous())), f.size, g.shape[0], ncol, float(nt),
+              _lib.ptr(torch.view_as_real(g)), f.size, g.shape[0], ncol, float(nt),
               _lib.stream_ptr())
     return torch.movedim(out, 0, dim)
 
