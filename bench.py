@@ -91,6 +91,26 @@ def timed_solves(D, which, mode, tol, steps, warmup, profile=False):
     return dt, phi, prof
 
 
+def copy_bandwidth(nbytes=2 << 30, reps=5):
+    """Device-to-device copy rate of a plain copy of nbytes (read + write
+    counted), the practical HBM ceiling the kernels are compared with."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
 def release(D):
     import torch
     D._solvers.clear()
@@ -191,7 +211,8 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-jacobian", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--tol", type=float, default=1e-8,
+                    help="relative FP64 residual of the mixed solve (chosen to meet 1e-6 fluence error)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
@@ -199,6 +220,8 @@ def main(argv=None):
         return 2
     import torch
     assert torch.cuda.is_available(), "bench needs a GPU"
+    copy_gbs = copy_bandwidth()
+    log(f"device copy {copy_gbs:.0f} GB/s")
     D, opt, t = build_workload(a.small)
     ns = len(opt.s_bary[0])
     nrhs = ns * opt.Nf
@@ -210,6 +233,7 @@ def main(argv=None):
     value = nrhs / dt_mixed
     extra["mixed_s_per_solve"] = dt_mixed
     extra["mixed_max_iters"] = int(phi_mixed.iters.max())
+    extra["mixed_iters_per_freq"] = phi_mixed.iters.max(axis=1).tolist()
     extra["mixed_status"] = np.unique(phi_mixed.status).tolist()
     extra["mixed_relres_max"] = float(phi_mixed.relres.max())
     x_mixed = phi_mixed.data.clone()
@@ -224,6 +248,7 @@ def main(argv=None):
                                for c in range(4) if prof[c, 2] > 0}
     all_gbs = prof[:, 1].sum() / (prof[:, 0].sum() * 1e-3) / 1e9
     extra["all_kernels_GBps"] = all_gbs
+    extra["copy_GBps"] = copy_gbs
 
     if not a.no_fp64:
         log("FP64 fused solve")
@@ -232,6 +257,7 @@ def main(argv=None):
                               phi64.data.permute(0, 2, 1).reshape(-1, D.nn))
         extra["fp64_fused_s_per_solve"] = dt64
         extra["fp64_fused_max_iters"] = int(phi64.iters.max())
+        extra["fp64_iters_per_freq"] = phi64.iters.max(axis=1).tolist()
         extra["mixed_vs_fp64_rel_err_max"] = float(err.max())
         extra["speedup_mixed_vs_fp64_fused"] = dt64 / dt_mixed
         del phi64
@@ -271,7 +297,7 @@ def main(argv=None):
         "ms_per_step": dt_mixed * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64+f64-refine", "data": "synthetic",
         "config": {"workload": "slab 80x80x40 mm, 1 mm Kuhn grid (269,001 nodes), 32 sources x 64 "
-                               "frequencies = 2048 RHS, FSAI-BiCGSTAB to 1e-10 (FP64 residual)"
+                               "frequencies = 2048 RHS, FSAI-BiCGSTAB to a 1e-8 FP64 relative residual"
                    if not a.small else "slab 60x60x30 mm 2 mm grid, 16 sources x 8 frequencies",
                    "nodes": D.nn, "rhs": nrhs, "tol": a.tol},
         "roofline": {"bound": "hbm", "kernel": names[cls], "achieved": achieved, "peak": HBM_PEAK_GBS,

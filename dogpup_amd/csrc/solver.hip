@@ -32,9 +32,27 @@ enum { MAT_A = 0, MAT_G = 1 };
 enum { EPI_STORE = 0, EPI_STORE_DOT = 1, EPI_DOT2 = 2 };
 enum { ST_ACTIVE = -1, ST_SKIP = -2 };
 
+// Row tiles are assigned so that each XCD sweeps one contiguous range of
+// tiles: workgroups are dispatched round-robin over the 8 XCDs, and
+// gridDim.x is a multiple of 8, so blockIdx.x % 8 names the XCD.  A tile's
+// stencil neighbours then sit in the same XCD's L2.
+__device__ inline int xcd_tile(int bx) { return (bx & 7) * (gridDim.x >> 3) + (bx >> 3); }
+
 // ------------------------------------------------------------------ SpMV ---
 // y = M x for every group; MAT_A: M = v0 + i*fscale[f]*v1 (planar, shared by
 // all frequencies); MAT_G: M = per-frequency complex values v0[f][nnz].
+//
+// The block walks its row tile in chunks of kChunkRows rows.  Each chunk's
+// CSR column indices and values are first copied into LDS with coalesced
+// loads; the row loop then reads them from LDS, so the only vector-memory
+// instruction per nonzero is the 16-byte-per-lane gather of x (the matrix
+// entry of a nonzero is shared by the LPR lanes of its row, and fetching it
+// per wavefront from global memory costs as many address cycles as the
+// gather itself).  A chunk with more than kChunkCap nonzeros falls back to
+// reading the CSR arrays directly.
+constexpr int kChunkRows = 64;
+constexpr int kChunkCap = 2048;
+
 template <typename T, int NSP, int MAT, int EPI>
 __global__ __launch_bounds__(256) void k_spmv(
     const int* __restrict__ rptr, const int* __restrict__ cptr, const T* __restrict__ v0,
@@ -44,9 +62,13 @@ __global__ __launch_bounds__(256) void k_spmv(
     const int* __restrict__ grp_active) {
   using G = Geo<T, NSP>;
   constexpr int CPL = G::CPL;
+  static_assert(kChunkRows % G::RPB == 0, "chunk");
   const int g = blockIdx.y, bx = blockIdx.x;
   if (grp_active && !grp_active[g]) return;
   __shared__ double sh[2 * 4 * NSP];
+  __shared__ int srp[kChunkRows + 1];
+  __shared__ int scol[kChunkCap];
+  __shared__ T sval[2 * kChunkCap];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int rsub = lane / G::LPR, cl = lane % G::LPR, c0 = cl * CPL;
   const int f = (g0 + g) / ncb;
@@ -59,59 +81,99 @@ __global__ __launch_bounds__(256) void k_spmv(
   Acc<CPL> d1, d2;
   d1.zero();
   d2.zero();
-  for (int st = 0; st < G::STEPS; ++st) {
-    const int row = bx * kTile + st * G::RPB + wv * G::RPW + rsub;
-    if (row >= nn) break;
-    T ar[CPL], ai[CPL];
-#pragma unroll
-    for (int e = 0; e < CPL; ++e) { ar[e] = T(0); ai[e] = T(0); }
-    const int s0 = rptr[row], s1 = rptr[row + 1];
-    int p = s0;
-    for (; p + 4 <= s1; p += 4) {
-      int cc[4];
-      T mr[4], mi[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        cc[q] = cptr[p + q];
-        if (MAT == MAT_A) { mr[q] = v0[p + q]; mi[q] = fk * v1[p + q]; }
-        else { mr[q] = gv[2 * (long long)(p + q)]; mi[q] = gv[2 * (long long)(p + q) + 1]; }
+  const int tile0 = xcd_tile(bx) * kTile;
+  const int tile1 = min(tile0 + kTile, nn);
+  for (int r0 = tile0; r0 < tile1; r0 += kChunkRows) {
+    const int r1 = min(r0 + kChunkRows, nn);
+    __syncthreads();  // the previous chunk's LDS reads are done
+    for (int q = threadIdx.x; q <= r1 - r0; q += 256) srp[q] = rptr[r0 + q];
+    __syncthreads();
+    const int es = srp[0], ne = srp[r1 - r0] - es;
+    const bool staged = ne <= kChunkCap;
+    if (staged) {
+      for (int q = threadIdx.x; q < ne; q += 256) {
+        scol[q] = cptr[es + q];
+        if (MAT == MAT_A) {
+          sval[2 * q] = v0[es + q];
+          sval[2 * q + 1] = fk * v1[es + q];
+        } else {
+          sval[2 * q] = gv[2 * (long long)(es + q)];
+          sval[2 * q + 1] = gv[2 * (long long)(es + q) + 1];
+        }
       }
-      T xr[4][CPL], xi[4][CPL];
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int st = 0; st < kChunkRows / G::RPB; ++st) {
+      const int lr = st * G::RPB + wv * G::RPW + rsub;
+      const int row = r0 + lr;
+      if (row >= r1) continue;
+      T ar[CPL], ai[CPL];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) Pack<T>::load(xg + (size_t)cc[q] * NSP * 2, xr[q], xi[q]);
+      for (int e = 0; e < CPL; ++e) { ar[e] = T(0); ai[e] = T(0); }
+      const int s0 = srp[lr] - es, s1 = srp[lr + 1] - es;
+      int p = s0;
+      for (; p + 4 <= s1; p += 4) {
+        int cc[4];
+        T mr[4], mi[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 4; ++q) {
+          if (staged) {
+            cc[q] = scol[p + q];
+            mr[q] = sval[2 * (p + q)];
+            mi[q] = sval[2 * (p + q) + 1];
+          } else {
+            const long long pg = es + p + q;
+            cc[q] = cptr[pg];
+            if (MAT == MAT_A) { mr[q] = v0[pg]; mi[q] = fk * v1[pg]; }
+            else { mr[q] = gv[2 * pg]; mi[q] = gv[2 * pg + 1]; }
+          }
+        }
+        T xr[4][CPL], xi[4][CPL];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Pack<T>::load(xg + (size_t)cc[q] * NSP * 2, xr[q], xi[q]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int e = 0; e < CPL; ++e) {
+            ar[e] += mr[q] * xr[q][e] - mi[q] * xi[q][e];
+            ai[e] += mr[q] * xi[q][e] + mi[q] * xr[q][e];
+          }
+      }
+      for (; p < s1; ++p) {
+        int cc;
+        T mr, mi;
+        if (staged) {
+          cc = scol[p];
+          mr = sval[2 * p];
+          mi = sval[2 * p + 1];
+        } else {
+          const long long pg = es + p;
+          cc = cptr[pg];
+          if (MAT == MAT_A) { mr = v0[pg]; mi = fk * v1[pg]; }
+          else { mr = gv[2 * pg]; mi = gv[2 * pg + 1]; }
+        }
+        T xr[CPL], xi[CPL];
+        Pack<T>::load(xg + (size_t)cc * NSP * 2, xr, xi);
 #pragma unroll
         for (int e = 0; e < CPL; ++e) {
-          ar[e] += mr[q] * xr[q][e] - mi[q] * xi[q][e];
-          ai[e] += mr[q] * xi[q][e] + mi[q] * xr[q][e];
+          ar[e] += mr * xr[e] - mi * xi[e];
+          ai[e] += mr * xi[e] + mi * xr[e];
         }
-    }
-    for (; p < s1; ++p) {
-      const int cc = cptr[p];
-      T mr, mi;
-      if (MAT == MAT_A) { mr = v0[p]; mi = fk * v1[p]; }
-      else { mr = gv[2 * (long long)p]; mi = gv[2 * (long long)p + 1]; }
-      T xr[CPL], xi[CPL];
-      Pack<T>::load(xg + (size_t)cc * NSP * 2, xr, xi);
-#pragma unroll
-      for (int e = 0; e < CPL; ++e) {
-        ar[e] += mr * xr[e] - mi * xi[e];
-        ai[e] += mr * xi[e] + mi * xr[e];
       }
-    }
-    const size_t off = ((gbase + row) * NSP + c0) * 2;
-    if (EPI != EPI_DOT2) Pack<T>::store(y + off, ar, ai);
-    if (EPI != EPI_STORE) {
-      T wr[CPL], wi[CPL];
-      Pack<T>::load(w + off, wr, wi);
+      const size_t off = ((gbase + row) * NSP + c0) * 2;
+      if (EPI != EPI_DOT2) Pack<T>::store(y + off, ar, ai);
+      if (EPI != EPI_STORE) {
+        T wr[CPL], wi[CPL];
+        Pack<T>::load(w + off, wr, wi);
 #pragma unroll
-      for (int e = 0; e < CPL; ++e) {
-        d1.re[e] += (double)wr[e] * ar[e] - (double)wi[e] * ai[e];
-        d1.im[e] += (double)wr[e] * ai[e] + (double)wi[e] * ar[e];
-        if (EPI == EPI_DOT2) {
-          d2.re[e] += (double)ar[e] * ar[e] - (double)ai[e] * ai[e];
-          d2.im[e] += 2.0 * (double)ar[e] * ai[e];
+        for (int e = 0; e < CPL; ++e) {
+          d1.re[e] += (double)wr[e] * ar[e] - (double)wi[e] * ai[e];
+          d1.im[e] += (double)wr[e] * ai[e] + (double)wi[e] * ar[e];
+          if (EPI == EPI_DOT2) {
+            d2.re[e] += (double)ar[e] * ar[e] - (double)ai[e] * ai[e];
+            d2.im[e] += 2.0 * (double)ar[e] * ai[e];
+          }
         }
       }
     }
@@ -147,7 +209,7 @@ __device__ inline void load_coef(const double* __restrict__ c, long long rhs0, T
   const size_t gbase = (size_t)g * nn;                                         \
   (void)sh; (void)rhs0;                                                        \
   for (int st = 0; st < G::STEPS; ++st) {                                      \
-    const int row = bx * kTile + st * G::RPB + wv * G::RPW + rsub;             \
+    const int row = xcd_tile(bx) * kTile + st * G::RPB + wv * G::RPW + rsub;   \
     if (row >= nn) break;                                                      \
     const size_t off = ((gbase + row) * NSP + c0) * 2;
 
@@ -317,7 +379,7 @@ __global__ __launch_bounds__(256) void k_residual64(
   d1.zero();
   d2.zero();
   for (int st = 0; st < G::STEPS; ++st) {
-    const int row = bx * kTile + st * G::RPB + wv * G::RPW + rsub;
+    const int row = xcd_tile(bx) * kTile + st * G::RPB + wv * G::RPW + rsub;
     if (row >= nn) break;
     const size_t off = ((gbase + row) * NSP + c);
     double yr = 0.0, yi = 0.0;
@@ -448,22 +510,43 @@ struct BState {
   int* counter;    // 4
 };
 
-__device__ inline void sum_part(const double* __restrict__ part, long long g, int c, int nsp,
-                                int nbx, double& re, double& im) {
-  double sr = 0.0, si = 0.0;
-  for (int bx = 0; bx < nbx; ++bx) {
+// Sum of the per-block partials of every column of group g.  All threads of
+// the (kScalThreads-wide) block call it; thread c < nsp receives column c's
+// sum.  Each column is reduced by kScalThreads/nsp threads in parallel.
+constexpr int kScalThreads = 256;
+__device__ inline void block_sum_part(const double* __restrict__ part, long long g, int nsp, int nbx,
+                                      double& re, double& im) {
+  __shared__ double sr[kScalThreads], si[kScalThreads];
+  const int t = threadIdx.x, c = t % nsp, q = t / nsp, nq = kScalThreads / nsp;
+  double a = 0.0, b = 0.0;
+  for (int bx = q; bx < nbx; bx += nq) {
     const long long i = 2 * ((g * nbx + bx) * nsp + c);
-    sr += part[i];
-    si += part[i + 1];
+    a += part[i];
+    b += part[i + 1];
   }
-  re = sr;
-  im = si;
+  sr[t] = a;
+  si[t] = b;
+  __syncthreads();
+  if (t < nsp) {
+    for (int k = 1; k < nq; ++k) {
+      a += sr[t + k * nsp];
+      b += si[t + k * nsp];
+    }
+    re = a;
+    im = b;
+  }
+  __syncthreads();
 }
 
 __device__ inline void group_flag(BState s, int g, bool active) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
   const unsigned long long m = __ballot(active);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(&cnt, __popcll(m));
+  __syncthreads();
   if (threadIdx.x == 0) {
-    const int n = __popcll(m);
+    const int n = cnt;
     s.grp_active[g] = n > 0;
     if (n) {
       atomicAdd(s.counter, n);
@@ -472,17 +555,19 @@ __device__ inline void group_flag(BState s, int g, bool active) {
   }
 }
 
-// kind 0: init from (part1 = |b|^2, part2 = b.b); `mask` (optional outer
-// status) leaves non-active columns skipped; ncols_valid marks padding.
-__global__ void k_sc_init(BState s, const double* __restrict__ part1, const double* __restrict__ part2,
-                          const int* __restrict__ mask, int nsp, int nbx, int ncb, int ncols, int g0) {
+// init from (part1 = |b|^2, part2 = b.b); `mask` (optional outer status)
+// leaves non-active columns skipped; columns >= ncols are padding.
+__global__ __launch_bounds__(kScalThreads) void k_sc_init(BState s, const double* __restrict__ part1,
+                                                          const double* __restrict__ part2,
+                                                          const int* __restrict__ mask, int nsp,
+                                                          int nbx, int ncb, int ncols, int g0) {
   const int g = blockIdx.x, c = threadIdx.x;
+  double n2 = 0, dummy = 0, rr = 0, ri = 0;
+  block_sum_part(part1, g, nsp, nbx, n2, dummy);
+  block_sum_part(part2, g, nsp, nbx, rr, ri);
   bool active = false;
   if (c < nsp) {
     const long long i = (long long)g * nsp + c;
-    double n2, dummy, rr, ri;
-    sum_part(part1, g, c, nsp, nbx, n2, dummy);
-    sum_part(part2, g, c, nsp, nbx, rr, ri);
     const int col = ((g0 + g) % ncb) * nsp + c;
     s.iters[i] = 0;
     s.half[i] = 0;
@@ -507,13 +592,15 @@ __global__ void k_sc_init(BState s, const double* __restrict__ part1, const doub
   group_flag(s, g, active);
 }
 
-__global__ void k_sc_alpha(BState s, const double* __restrict__ part1, int nsp, int nbx) {
+__global__ __launch_bounds__(kScalThreads) void k_sc_alpha(BState s, const double* __restrict__ part1,
+                                                           int nsp, int nbx) {
   const int g = blockIdx.x, c = threadIdx.x;
+  if (!s.grp_active[g]) return;
+  double dr = 0, di = 0;
+  block_sum_part(part1, g, nsp, nbx, dr, di);
   if (c >= nsp) return;
   const long long i = (long long)g * nsp + c;
   if (s.status[i] != ST_ACTIVE) return;
-  double dr, di;
-  sum_part(part1, g, c, nsp, nbx, dr, di);
   if ((dr == 0.0 && di == 0.0) || !isfinite(dr) || !isfinite(di)) {
     s.status[i] = DOGPUP_RHS_BREAKDOWN;
     return;
@@ -523,13 +610,15 @@ __global__ void k_sc_alpha(BState s, const double* __restrict__ part1, int nsp, 
   s.alpha[2 * i + 1] = a.im;
 }
 
-__global__ void k_sc_s(BState s, const double* __restrict__ part1, int nsp, int nbx) {
+__global__ __launch_bounds__(kScalThreads) void k_sc_s(BState s, const double* __restrict__ part1, int nsp,
+                                                       int nbx) {
   const int g = blockIdx.x, c = threadIdx.x;
+  if (!s.grp_active[g]) return;
+  double n2 = 0, dummy = 0;
+  block_sum_part(part1, g, nsp, nbx, n2, dummy);
   if (c >= nsp) return;
   const long long i = (long long)g * nsp + c;
   if (s.status[i] != ST_ACTIVE) return;
-  double n2, dummy;
-  sum_part(part1, g, c, nsp, nbx, n2, dummy);
   const double rs = sqrt(n2) / s.bnorm[i];
   if (rs < s.tol[i]) {
     s.half[i] = 1;
@@ -537,17 +626,19 @@ __global__ void k_sc_s(BState s, const double* __restrict__ part1, int nsp, int 
   }
 }
 
-__global__ void k_sc_omega(BState s, const double* __restrict__ part1, const double* __restrict__ part2,
-                           int nsp, int nbx) {
+__global__ __launch_bounds__(kScalThreads) void k_sc_omega(BState s, const double* __restrict__ part1,
+                                                           const double* __restrict__ part2, int nsp,
+                                                           int nbx) {
   const int g = blockIdx.x, c = threadIdx.x;
+  if (!s.grp_active[g]) return;
+  double nr = 0, ni = 0, dr = 0, di = 0;
+  block_sum_part(part1, g, nsp, nbx, nr, ni);
+  block_sum_part(part2, g, nsp, nbx, dr, di);
   if (c >= nsp) return;
   const long long i = (long long)g * nsp + c;
   if (s.status[i] != ST_ACTIVE) return;
   double om_r = 0.0, om_i = 0.0;
   if (!s.half[i]) {
-    double nr, ni, dr, di;
-    sum_part(part1, g, c, nsp, nbx, nr, ni);
-    sum_part(part2, g, c, nsp, nbx, dr, di);
     if ((dr == 0.0 && di == 0.0) || !isfinite(dr) || !isfinite(di) || !isfinite(nr) || !isfinite(ni)) {
       s.bd[i] = 1;
     } else {
@@ -560,16 +651,19 @@ __global__ void k_sc_omega(BState s, const double* __restrict__ part1, const dou
   s.omega[2 * i + 1] = om_i;
 }
 
-__global__ void k_sc_beta(BState s, const double* __restrict__ part1, const double* __restrict__ part2,
-                          int nsp, int nbx, int maxit) {
+__global__ __launch_bounds__(kScalThreads) void k_sc_beta(BState s, const double* __restrict__ part1,
+                                                          const double* __restrict__ part2, int nsp,
+                                                          int nbx, int maxit) {
   const int g = blockIdx.x, c = threadIdx.x;
+  if (!s.grp_active[g]) return;   // inactive groups stay inactive
+  double n2 = 0, dummy = 0, rr = 0, ri = 0;
+  block_sum_part(part2, g, nsp, nbx, n2, dummy);
+  block_sum_part(part1, g, nsp, nbx, rr, ri);
   bool active = false;
   if (c < nsp) {
     const long long i = (long long)g * nsp + c;
     if (s.status[i] == ST_ACTIVE) {
       const int it = ++s.iters[i];
-      double n2, dummy, rr, ri;
-      sum_part(part2, g, c, nsp, nbx, n2, dummy);
       const double rn = sqrt(n2) / s.bnorm[i];
       if (s.half[i]) {
         s.status[i] = DOGPUP_RHS_CONVERGED;
@@ -582,22 +676,18 @@ __global__ void k_sc_beta(BState s, const double* __restrict__ part1, const doub
           s.status[i] = DOGPUP_RHS_CONVERGED;
         } else if (it >= maxit) {
           s.status[i] = DOGPUP_RHS_MAXIT;
+        } else if ((rr == 0.0 && ri == 0.0) || (om_r == 0.0 && om_i == 0.0)) {
+          s.status[i] = DOGPUP_RHS_BREAKDOWN;
         } else {
-          sum_part(part1, g, c, nsp, nbx, rr, ri);
-          if ((rr == 0.0 && ri == 0.0) || (om_r == 0.0 && om_i == 0.0)) {
-            s.status[i] = DOGPUP_RHS_BREAKDOWN;
-          } else {
-            const cpx<double> rho_n{rr, ri};
-            const cpx<double> q = cdiv(rho_n, cpx<double>{s.rho[2 * i], s.rho[2 * i + 1]});
-            const cpx<double> a_over_w =
-                cdiv(cpx<double>{s.alpha[2 * i], s.alpha[2 * i + 1]}, cpx<double>{om_r, om_i});
-            const cpx<double> b = cmul(q, a_over_w);
-            s.beta[2 * i] = b.re;
-            s.beta[2 * i + 1] = b.im;
-            s.rho[2 * i] = rr;
-            s.rho[2 * i + 1] = ri;
-            active = true;
-          }
+          const cpx<double> q = cdiv(cpx<double>{rr, ri}, cpx<double>{s.rho[2 * i], s.rho[2 * i + 1]});
+          const cpx<double> a_over_w =
+              cdiv(cpx<double>{s.alpha[2 * i], s.alpha[2 * i + 1]}, cpx<double>{om_r, om_i});
+          const cpx<double> b = cmul(q, a_over_w);
+          s.beta[2 * i] = b.re;
+          s.beta[2 * i + 1] = b.im;
+          s.rho[2 * i] = rr;
+          s.rho[2 * i + 1] = ri;
+          active = true;
         }
       }
     }
@@ -616,15 +706,17 @@ struct RState {
 
 // After a residual pass: decide each column's outer status and its inner
 // tolerance.  part1 = |r64|^2.
-__global__ void k_rf_check(RState o, BState in, const double* __restrict__ part1, int nsp, int nbx,
-                           int ncb, int ncols, int g0, int sweep, double tol, double tol_in,
-                           int maxit) {
+__global__ __launch_bounds__(kScalThreads) void k_rf_check(RState o, BState in,
+                                                           const double* __restrict__ part1, int nsp,
+                                                           int nbx, int ncb, int ncols, int g0,
+                                                           int sweep, double tol, double tol_in,
+                                                           int maxit) {
   const int g = blockIdx.x, c = threadIdx.x;
+  double n2 = 0, dummy = 0;
+  block_sum_part(part1, g, nsp, nbx, n2, dummy);
   bool active = false;
   if (c < nsp) {
     const long long i = (long long)g * nsp + c;
-    double n2, dummy;
-    sum_part(part1, g, c, nsp, nbx, n2, dummy);
     const int col = ((g0 + g) % ncb) * nsp + c;
     if (sweep == 0) {
       o.bnorm[i] = sqrt(n2);
@@ -664,17 +756,19 @@ __global__ void k_rf_accum(RState o, BState in, int nsp) {
 // op 1: err = |s|/err0 -> max; op 2: om = (Gt.Gs)/(Gt.Gt); op 3: err =
 // |r|/err0 -> max; op 4: rho = rhat0.r, beta = rho/rho0 / (alph om), rho0 =
 // rho; op 5: init err0 = |r|, rho0 = rhat0.r.
-__global__ void k_plain_scal(BState s, int op, const double* __restrict__ part1,
-                             const double* __restrict__ part2, int nsp, int nbx, int ncb,
-                             int ncols, int g0, unsigned long long* __restrict__ maxbits) {
+__global__ __launch_bounds__(kScalThreads) void k_plain_scal(BState s, int op,
+                                                             const double* __restrict__ part1,
+                                                             const double* __restrict__ part2, int nsp,
+                                                             int nbx, int ncb, int ncols, int g0,
+                                                             unsigned long long* __restrict__ maxbits) {
   const int g = blockIdx.x, c = threadIdx.x;
+  double ar = 0, ai = 0, br = 0, bi = 0;
+  block_sum_part(part1, g, nsp, nbx, ar, ai);
+  if (part2) block_sum_part(part2, g, nsp, nbx, br, bi);
   if (c >= nsp) return;
   const long long i = (long long)g * nsp + c;
   const int col = ((g0 + g) % ncb) * nsp + c;
   const bool valid = col < ncols && s.bnorm[i] > 0.0;
-  double ar, ai, br = 0.0, bi = 0.0;
-  sum_part(part1, g, c, nsp, nbx, ar, ai);
-  if (part2) sum_part(part2, g, c, nsp, nbx, br, bi);
   switch (op) {
     case 5:
       s.bnorm[i] = sqrt(ar);
@@ -762,10 +856,16 @@ struct Buf {
 struct dogpup_solver {
   int nn = 0, nf = 0, ncols = 0, nsp = 0, ncb = 0, ngroups = 0, mode = 0;
   int nnzA = 0, nnzG = 0;
+  // caller's patterns in the reference node order (FSAI setup runs on these)
+  const int *rA0 = nullptr, *cA0 = nullptr, *rG0 = nullptr, *cG0 = nullptr;
+  // patterns in solver node order (owned): A, G, G^T and the value maps
+  // slot -> slot of the reference-order arrays
   const int *rA = nullptr, *cA = nullptr, *rG = nullptr, *cG = nullptr;
-  const int *rGT = nullptr, *cGT = nullptr, *permGT = nullptr;
-  dogpup::Buf a64re, a64im, a32re, a32im, fscale, g64, gt64, g32, gt32;
-  dogpup::Buf ws, part1, part2, state, rstate, scratch;
+  const int *rGT = nullptr, *cGT = nullptr;
+  dogpup::Buf pAr, pAc, pGr, pGc, pGTr, pGTc, amap, gmap, gtmap, order;
+  // values: a64re/a64im and g64 in reference order; the rest in solver order
+  dogpup::Buf a64re, a64im, a64pre, a64pim, a32re, a32im, fscale, g64, gp64, gtp64, g32, gt32;
+  dogpup::Buf ws, part1, part2, state, rstate, bperm, xperm;
   int chunk = 0, nbx = 0;
   bool have_matrix = false;
   double stats[4] = {0, 0, 0, 0};
@@ -799,17 +899,42 @@ __global__ void k_cast_f32(const double* __restrict__ a, float* __restrict__ b, 
     b[i] = (float)a[i];
 }
 
-// GT values [f][p] = G values [f][perm[p]] (complex)
+// out[p] = in[map[p]] (real values, type conversion)
 template <typename TO>
-__global__ void k_gather_gt(const double* __restrict__ g, const int* __restrict__ perm,
-                            TO* __restrict__ gt, long long nnz, int nf) {
+__global__ void k_gather_real(const double* __restrict__ in, const int* __restrict__ map,
+                              TO* __restrict__ out, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = (TO)in[map[i]];
+}
+
+// out[f][p] = in[f][map[p]] (complex [f][nnz])
+template <typename TO>
+__global__ void k_gather_cplx(const double* __restrict__ in, const int* __restrict__ map,
+                              TO* __restrict__ out, long long nnz, int nf) {
   const long long total = nnz * nf;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (long long)gridDim.x * blockDim.x) {
     const long long f = i / nnz, p = i % nnz;
-    const long long s = f * nnz + perm[p];
-    gt[2 * i] = (TO)g[2 * s];
-    gt[2 * i + 1] = (TO)g[2 * s + 1];
+    const long long s = f * nnz + map[p];
+    out[2 * i] = (TO)in[2 * s];
+    out[2 * i + 1] = (TO)in[2 * s + 1];
+  }
+}
+
+// Row permutation of a complex128 block [g][node][nsp]: dir 0 gathers into
+// solver order (dst[g][i] = src[g][order[i]]), dir 1 scatters back.
+__global__ void k_permute_rows(const double* __restrict__ src, double* __restrict__ dst,
+                               const int* __restrict__ order, int nn, int nsp, int gc, int dir) {
+  const long long per = (long long)nn * nsp;
+  const long long total = per * gc;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long g = i / per, r = i % per;
+    const int node = (int)(r / nsp), c = (int)(r % nsp);
+    const long long o = g * per + (long long)order[node] * nsp + c;
+    const double2 v = reinterpret_cast<const double2*>(src)[dir == 0 ? o : i];
+    reinterpret_cast<double2*>(dst)[dir == 0 ? i : o] = v;
   }
 }
 
@@ -836,11 +961,11 @@ struct Chunk {
   double *p1, *p2;
 
   dim3 vgrid() const { return dim3(S->nbx, gcount); }
-  const T* a0() const { return sizeof(T) == 4 ? (const T*)S->a32re.p : (const T*)S->a64re.p; }
-  const T* a1() const { return sizeof(T) == 4 ? (const T*)S->a32im.p : (const T*)S->a64im.p; }
+  const T* a0() const { return sizeof(T) == 4 ? (const T*)S->a32re.p : (const T*)S->a64pre.p; }
+  const T* a1() const { return sizeof(T) == 4 ? (const T*)S->a32im.p : (const T*)S->a64pim.p; }
   const T* gvals(bool tr) const {
     if (sizeof(T) == 4) return tr ? (const T*)S->gt32.p : (const T*)S->g32.p;
-    return tr ? (const T*)S->gt64.p : (const T*)S->g64.p;
+    return tr ? (const T*)S->gtp64.p : (const T*)S->gp64.p;
   }
 
   template <int EPI>
@@ -906,7 +1031,7 @@ struct Chunk {
       k_init<T, NSP><<<vgrid(), 256, 0, st>>>(b, V.x, V.r, V.rhat, V.p, p1, p2, S->nn, S->nbx, nullptr);
       DP_LAUNCH_CHECK();
       DP_TRY(reset_count());
-      k_sc_init<<<gcount, 64, 0, st>>>(bs, p1, p2, mask, nsp, S->nbx, S->ncb, S->ncols, g0);
+      k_sc_init<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, mask, nsp, S->nbx, S->ncb, S->ncols, g0);
       DP_LAUNCH_CHECK();
     }
     int active = 0;
@@ -916,19 +1041,19 @@ struct Chunk {
       DP_TRY(spmvG<EPI_STORE>(false, V.p, V.u, nullptr, ga));
       DP_TRY(spmvG<EPI_STORE>(true, V.u, V.h, nullptr, ga));
       DP_TRY(spmvA<EPI_STORE_DOT>(V.h, V.v, V.rhat, ga));
-      k_sc_alpha<<<gcount, 64, 0, st>>>(bs, p1, nsp, S->nbx);
+      k_sc_alpha<<<gcount, kScalThreads, 0, st>>>(bs, p1, nsp, S->nbx);
       DP_LAUNCH_CHECK();
       pbegin();
       k_axpy_s<T, NSP><<<vgrid(), 256, 0, st>>>(V.r, V.v, V.s, bs.alpha, p1, S->nn, S->nbx, ga);
       DP_LAUNCH_CHECK();
       pend(2, groups(true) * vbytes() * 3);
-      k_sc_s<<<gcount, 64, 0, st>>>(bs, p1, nsp, S->nbx);
+      k_sc_s<<<gcount, kScalThreads, 0, st>>>(bs, p1, nsp, S->nbx);
       DP_LAUNCH_CHECK();
       DP_TRY(spmvG<EPI_STORE>(false, V.s, V.u, nullptr, ga));     // u = Gs
       DP_TRY(spmvG<EPI_STORE>(true, V.u, V.z, nullptr, ga));      // z = G^T G s
       DP_TRY(spmvA<EPI_STORE>(V.z, V.t, nullptr, ga));            // t = A z
       DP_TRY(spmvG<EPI_DOT2>(false, V.t, nullptr, V.u, ga));      // (Gt.Gs), (Gt.Gt)
-      k_sc_omega<<<gcount, 64, 0, st>>>(bs, p1, p2, nsp, S->nbx);
+      k_sc_omega<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, nsp, S->nbx);
       DP_LAUNCH_CHECK();
       pbegin();
       k_update_xr<T, NSP><<<vgrid(), 256, 0, st>>>(V.x, V.h, V.z, V.r, V.s, V.t, V.rhat, bs.alpha,
@@ -936,7 +1061,7 @@ struct Chunk {
       DP_LAUNCH_CHECK();
       pend(2, groups(true) * vbytes() * 8);
       DP_TRY(reset_count());
-      k_sc_beta<<<gcount, 64, 0, st>>>(bs, p1, p2, nsp, S->nbx, maxit);
+      k_sc_beta<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, nsp, S->nbx, maxit);
       DP_LAUNCH_CHECK();
       const int ag = S->active_groups;  // groups still active before this update
       pbegin();
@@ -1032,6 +1157,15 @@ static int export_results(dogpup_solver* S, hipStream_t st, int g0, int gcount, 
   return DOGPUP_OK;
 }
 
+// Row permutation between the caller's node order and the solver's.
+static int permute_block(dogpup_solver* S, hipStream_t st, const double* src, double* dst, int gc,
+                         int dir) {
+  const long long n = (long long)gc * S->nn * S->nsp;
+  k_permute_rows<<<grid1(n), 256, 0, st>>>(src, dst, S->order.as<int>(), S->nn, S->nsp, gc, dir);
+  DP_LAUNCH_CHECK();
+  return DOGPUP_OK;
+}
+
 template <int NSP>
 static int solve_c128(dogpup_solver* S, hipStream_t st, const double* b, double* x, double tol,
                       int maxit, int* h_iters, double* h_relres, int* h_status) {
@@ -1041,8 +1175,10 @@ static int solve_c128(dogpup_solver* S, hipStream_t st, const double* b, double*
     Chunk<double, NSP> C{S, st, gc, g0, carve_state(S->state, gc * NSP, gc), S->part1.as<double>(),
                          S->part2.as<double>()};
     Vecs<double> V = carve<double>(S->ws, gc * per_group, false, false);
-    V.x = x + (size_t)g0 * S->nn * NSP * 2;
-    const double* bc = b + (size_t)g0 * S->nn * NSP * 2;
+    const size_t goff = (size_t)g0 * S->nn * NSP * 2;
+    DP_TRY(permute_block(S, st, b + goff, S->bperm.as<double>(), gc, 0));
+    V.x = S->xperm.as<double>();
+    const double* bc = S->bperm.as<double>();
     k_fill<<<div_up(gc * NSP, 256), 256, 0, st>>>(C.bs.tol, tol, gc * NSP);
     DP_LAUNCH_CHECK();
     int its = 0;
@@ -1050,7 +1186,7 @@ static int solve_c128(dogpup_solver* S, hipStream_t st, const double* b, double*
     S->stats[0] = std::max(S->stats[0], (double)its);
     // true residual for reporting
     k_residual64<NSP><<<dim3(S->nbx, gc), 256, 0, st>>>(
-        S->rA, S->cA, S->a64re.as<double>(), S->a64im.as<double>(), S->fscale.as<double>(), bc, V.x,
+        S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), S->fscale.as<double>(), bc, V.x,
         nullptr, nullptr, nullptr, nullptr, nullptr, 0, C.p1, C.p2, S->nn, S->ncb, g0, S->nbx);
     DP_LAUNCH_CHECK();
     // relres = |b - Ax| / |b|: reuse k_rf_check's arithmetic through a small pass
@@ -1058,6 +1194,7 @@ static int solve_c128(dogpup_solver* S, hipStream_t st, const double* b, double*
     DP_HIP(hipMemcpyAsync(part.data(), C.p1, part.size() * 8, hipMemcpyDeviceToHost, st));
     std::vector<double> bn(gc * NSP);
     DP_HIP(hipMemcpyAsync(bn.data(), C.bs.bnorm, bn.size() * 8, hipMemcpyDeviceToHost, st));
+    DP_TRY(permute_block(S, st, V.x, x + goff, gc, 1));
     DP_TRY(export_results(S, st, g0, gc, C.bs.iters, C.bs.relres, C.bs.status, h_iters, h_relres,
                           h_status));
     for (int g = 0; g < gc; ++g) {
@@ -1091,28 +1228,31 @@ template <int NSP>
 static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, double* x, double tol,
                         int maxit, int* h_iters, double* h_relres, int* h_status) {
   const size_t per_group32 = (size_t)S->nn * NSP * 2 * sizeof(float);
-  const double tol_in = 1e-4;
+  const char* env_tol = getenv("DOGPUP_INNER_TOL");
+  const double tol_in = env_tol ? atof(env_tol) : 1e-4;
   for (int g0 = 0; g0 < S->ngroups; g0 += S->chunk) {
     const int gc = std::min(S->chunk, S->ngroups - g0);
     Chunk<float, NSP> C{S, st, gc, g0, carve_state(S->state, gc * NSP, gc), S->part1.as<double>(),
                         S->part2.as<double>()};
     RState R = carve_rstate(S->rstate, gc * NSP);
     Vecs<float> V = carve<float>(S->ws, gc * per_group32, true, false);
-    double* xc = x + (size_t)g0 * S->nn * NSP * 2;
-    const double* bc = b + (size_t)g0 * S->nn * NSP * 2;
+    const size_t goff = (size_t)g0 * S->nn * NSP * 2;
+    DP_TRY(permute_block(S, st, b + goff, S->bperm.as<double>(), gc, 0));
+    double* xc = S->xperm.as<double>();
+    const double* bc = S->bperm.as<double>();
     DP_HIP(hipMemsetAsync(xc, 0, (size_t)gc * S->nn * NSP * 16, st));
     const double e64 = (double)S->nn * NSP;  // complex elements per group
     for (int sweep = 0;; ++sweep) {
       C.pbegin();
       k_residual64<NSP><<<dim3(S->nbx, gc), 256, 0, st>>>(
-          S->rA, S->cA, S->a64re.as<double>(), S->a64im.as<double>(), S->fscale.as<double>(), bc, xc,
+          S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), S->fscale.as<double>(), bc, xc,
           V.r, V.rhat, V.p, V.x, sweep == 0 ? nullptr : R.status, 1, C.p1, C.p2, S->nn, S->ncb, g0,
           S->nbx);
       DP_LAUNCH_CHECK();
       C.pend(3, (sweep == 0 ? gc : S->active_groups) * e64 * (16 + 16 + 4 * 8) +
                     (double)S->nnzA * 20 + 4.0 * (S->nn + 1));
       DP_TRY(C.reset_count());
-      k_rf_check<<<gc, 64, 0, st>>>(R, C.bs, C.p1, NSP, S->nbx, S->ncb, S->ncols, g0, sweep, tol,
+      k_rf_check<<<gc, kScalThreads, 0, st>>>(R, C.bs, C.p1, NSP, S->nbx, S->ncb, S->ncols, g0, sweep, tol,
                                     tol_in, maxit);
       DP_LAUNCH_CHECK();
       int active = 0;
@@ -1120,7 +1260,7 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
       if (active == 0) break;
       const int active_groups_at_sweep = S->active_groups;
       // inner state from the residual pass: bnorm = |r64|, rho = r32.r32
-      k_sc_init<<<gc, 64, 0, st>>>(C.bs, C.p1, C.p2, R.status, NSP, S->nbx, S->ncb, S->ncols, g0);
+      k_sc_init<<<gc, kScalThreads, 0, st>>>(C.bs, C.p1, C.p2, R.status, NSP, S->nbx, S->ncb, S->ncols, g0);
       DP_LAUNCH_CHECK();
       int its = 0;
       DP_TRY(C.run_fused(V, nullptr, maxit, R.status, &its));
@@ -1130,7 +1270,7 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
           xc, V.x, C.bs.status, S->nn);
       DP_LAUNCH_CHECK();
       C.pend(3, sweep_groups * e64 * (32 + 8));
-      k_rf_accum<<<gc, 64, 0, st>>>(R, C.bs, NSP);
+      k_rf_accum<<<gc, kScalThreads, 0, st>>>(R, C.bs, NSP);
       DP_LAUNCH_CHECK();
       S->stats[0] = std::max(S->stats[0], (double)its);
       S->stats[2] += 1;
@@ -1138,6 +1278,7 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
       S->stats[3] += its;
       if (sweep > 64) break;
     }
+    DP_TRY(permute_block(S, st, xc, x + goff, gc, 1));
     DP_TRY(export_results(S, st, g0, gc, R.iters, R.relres, R.status, h_iters, h_relres, h_status));
   }
   return DOGPUP_OK;
@@ -1152,14 +1293,16 @@ static int solve_plain(dogpup_solver* S, hipStream_t st, const double* b, double
     Chunk<double, NSP> C{S, st, gc, g0, carve_state(S->state, gc * NSP, gc), S->part1.as<double>(),
                          S->part2.as<double>()};
     Vecs<double> V = carve<double>(S->ws, gc * per_group, false, true);
-    V.x = x + (size_t)g0 * S->nn * NSP * 2;
-    const double* bc = b + (size_t)g0 * S->nn * NSP * 2;
+    const size_t goff = (size_t)g0 * S->nn * NSP * 2;
+    DP_TRY(permute_block(S, st, b + goff, S->bperm.as<double>(), gc, 0));
+    V.x = S->xperm.as<double>();
+    const double* bc = S->bperm.as<double>();
     unsigned long long* dmax = reinterpret_cast<unsigned long long*>(C.bs.counter);
     const dim3 vg = C.vgrid();
     const int nn = S->nn, nbx = S->nbx;
     auto maxerr = [&](int op, const double* pa, double* out) -> int {
       DP_HIP(hipMemsetAsync(dmax, 0, 8, st));
-      k_plain_scal<<<gc, 64, 0, st>>>(C.bs, op, pa, nullptr, NSP, nbx, S->ncb, S->ncols, g0, dmax);
+      k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, op, pa, nullptr, NSP, nbx, S->ncb, S->ncols, g0, dmax);
       DP_LAUNCH_CHECK();
       DP_HIP(hipMemcpyAsync(S->h_max, dmax, 8, hipMemcpyDeviceToHost, st));
       DP_HIP(hipStreamSynchronize(st));
@@ -1172,7 +1315,7 @@ static int solve_plain(dogpup_solver* S, hipStream_t st, const double* b, double
     // x = 0, r = b, rhat0 = r, p = r
     k_init<double, NSP><<<vg, 256, 0, st>>>(bc, V.x, V.r, V.rhat, V.p, C.p1, C.p2, nn, nbx, nullptr);
     DP_LAUNCH_CHECK();
-    k_plain_scal<<<gc, 64, 0, st>>>(C.bs, 5, C.p1, C.p2, NSP, nbx, S->ncb, S->ncols, g0, dmax);
+    k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 5, C.p1, C.p2, NSP, nbx, S->ncb, S->ncols, g0, dmax);
     DP_LAUNCH_CHECK();
     int it = 0;
     for (it = 1; it <= maxit; ++it) {
@@ -1181,7 +1324,7 @@ static int solve_plain(dogpup_solver* S, hipStream_t st, const double* b, double
       DP_TRY(C.template spmvA<EPI_STORE>(V.h, V.v, nullptr, nullptr));          // v = A h
       k_dot<double, NSP><<<vg, 256, 0, st>>>(V.v, V.rhat, 0, C.p1, nn, nbx, nullptr);
       DP_LAUNCH_CHECK();
-      k_plain_scal<<<gc, 64, 0, st>>>(C.bs, 0, C.p1, nullptr, NSP, nbx, S->ncb, S->ncols, g0, dmax);
+      k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 0, C.p1, nullptr, NSP, nbx, S->ncb, S->ncols, g0, dmax);
       DP_LAUNCH_CHECK();
       k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.u, V.x, V.h, C.bs.alpha, 0, nn, nbx, nullptr);  // hx = x + h/alph
       DP_LAUNCH_CHECK();
@@ -1205,7 +1348,7 @@ static int solve_plain(dogpup_solver* S, hipStream_t st, const double* b, double
       DP_LAUNCH_CHECK();
       k_dot<double, NSP><<<vg, 256, 0, st>>>(V.w, V.w, 0, C.p2, nn, nbx, nullptr);
       DP_LAUNCH_CHECK();
-      k_plain_scal<<<gc, 64, 0, st>>>(C.bs, 2, C.p1, C.p2, NSP, nbx, S->ncb, S->ncols, g0, dmax);
+      k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 2, C.p1, C.p2, NSP, nbx, S->ncb, S->ncols, g0, dmax);
       DP_LAUNCH_CHECK();
       k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.x, V.h, V.z, C.bs.omega, 2, nn, nbx, nullptr);  // x = h + om z
       DP_LAUNCH_CHECK();
@@ -1219,11 +1362,12 @@ static int solve_plain(dogpup_solver* S, hipStream_t st, const double* b, double
       if (err < tol) break;
       k_dot<double, NSP><<<vg, 256, 0, st>>>(V.rhat, V.r, 0, C.p1, nn, nbx, nullptr);
       DP_LAUNCH_CHECK();
-      k_plain_scal<<<gc, 64, 0, st>>>(C.bs, 4, C.p1, nullptr, NSP, nbx, S->ncb, S->ncols, g0, dmax);
+      k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 4, C.p1, nullptr, NSP, nbx, S->ncb, S->ncols, g0, dmax);
       DP_LAUNCH_CHECK();
       k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.p, V.r, V.p, C.bs.beta, 2, nn, nbx, nullptr);  // p = r + beta p
       DP_LAUNCH_CHECK();
     }
+    DP_TRY(permute_block(S, st, V.x, x + goff, gc, 1));
     S->stats[0] = std::max(S->stats[0], (double)std::min(it, maxit));
     S->stats[1] += (double)std::min(it, maxit) * gc;
     // status: converged if the lockstep loop met tol
@@ -1355,32 +1499,99 @@ extern "C" {
 
 int dogpup_solver_create(dogpup_solver_t* out, int nn, int nfreq, int ncols, int nsp, int mode,
                          int nnzA, const int* d_rptrA, const int* d_cptrA, int nnzG,
-                         const int* d_rptrG, const int* d_cptrG, const int* d_rptrGT,
-                         const int* d_cptrGT, const int* d_permGT) {
+                         const int* d_rptrG, const int* d_cptrG, const int* h_order) {
   DP_ARG(out != nullptr, "out is NULL");
   *out = nullptr;
   DP_ARG(nn > 0 && nfreq > 0 && ncols > 0 && nnzA > 0 && nnzG > 0, "bad dimensions");
   DP_ARG(check_nsp(nsp), "group width must be 8, 16, 32 or 64 (got %d)", nsp);
   DP_ARG(mode == DOGPUP_MODE_C128 || mode == DOGPUP_MODE_C64_REFINE || mode == DOGPUP_MODE_C128_PLAIN,
          "unknown mode %d", mode);
-  DP_ARG(d_rptrA && d_cptrA && d_rptrG && d_cptrG && d_rptrGT && d_cptrGT && d_permGT, "NULL pattern");
+  DP_ARG(d_rptrA && d_cptrA && d_rptrG && d_cptrG, "NULL pattern");
   std::unique_ptr<dogpup_solver> S(new dogpup_solver());
   S->nn = nn; S->nf = nfreq; S->ncols = ncols; S->nsp = nsp; S->mode = mode;
   S->ncb = (ncols + nsp - 1) / nsp;
   S->ngroups = nfreq * S->ncb;
   S->nnzA = nnzA; S->nnzG = nnzG;
-  S->rA = d_rptrA; S->cA = d_cptrA; S->rG = d_rptrG; S->cG = d_cptrG;
-  S->rGT = d_rptrGT; S->cGT = d_cptrGT; S->permGT = d_permGT;
-  S->nbx = (int)div_up(nn, kTile);
+  S->rA0 = d_rptrA; S->cA0 = d_cptrA; S->rG0 = d_rptrG; S->cG0 = d_cptrG;
+  // gridDim.x of the row kernels is a multiple of 8 (one contiguous tile range per XCD)
+  S->nbx = (int)(div_up(div_up(nn, kTile), 8) * 8);
   DP_ARG(S->ngroups <= 65535, "too many groups (%d)", S->ngroups);
+
+  // solver node order and the permuted patterns (host-side, once per handle)
+  std::vector<int> order(nn), inv(nn, -1);
+  for (int i = 0; i < nn; ++i) order[i] = h_order ? h_order[i] : i;
+  for (int i = 0; i < nn; ++i) {
+    DP_ARG(order[i] >= 0 && order[i] < nn && inv[order[i]] < 0, "node order is not a permutation");
+    inv[order[i]] = i;
+  }
+  std::vector<int> rA(nn + 1), cA(nnzA), rG(nn + 1), cG(nnzG);
+  DP_HIP(hipMemcpy(rA.data(), d_rptrA, (nn + 1) * 4, hipMemcpyDeviceToHost));
+  DP_HIP(hipMemcpy(cA.data(), d_cptrA, (size_t)nnzA * 4, hipMemcpyDeviceToHost));
+  DP_HIP(hipMemcpy(rG.data(), d_rptrG, (nn + 1) * 4, hipMemcpyDeviceToHost));
+  DP_HIP(hipMemcpy(cG.data(), d_cptrG, (size_t)nnzG * 4, hipMemcpyDeviceToHost));
+  DP_ARG(rA[0] == 0 && rA[nn] == nnzA && rG[0] == 0 && rG[nn] == nnzG, "CSR row pointers inconsistent with nnz");
+  for (int i = 0; i < nn; ++i) DP_ARG(rA[i] <= rA[i + 1] && rG[i] <= rG[i + 1], "CSR row pointers decrease");
+  for (int v : cA) DP_ARG(v >= 0 && v < nn, "A column index out of range");
+  for (int v : cG) DP_ARG(v >= 0 && v < nn, "G column index out of range");
+  auto permute = [&](const std::vector<int>& rp, const std::vector<int>& cp, std::vector<int>& rq,
+                     std::vector<int>& cq, std::vector<int>& map) {
+    rq.assign(nn + 1, 0);
+    cq.resize(cp.size());
+    map.resize(cp.size());
+    for (int i = 0; i < nn; ++i) rq[i + 1] = rq[i] + (rp[order[i] + 1] - rp[order[i]]);
+    std::vector<std::pair<int, int>> row;
+    for (int i = 0; i < nn; ++i) {
+      const int o = order[i];
+      row.clear();
+      for (int p = rp[o]; p < rp[o + 1]; ++p) row.push_back({inv[cp[p]], p});
+      std::sort(row.begin(), row.end());
+      for (size_t k = 0; k < row.size(); ++k) {
+        cq[rq[i] + k] = row[k].first;
+        map[rq[i] + k] = row[k].second;
+      }
+    }
+  };
+  std::vector<int> rAp, cAp, amap, rGp, cGp, gmap;
+  permute(rA, cA, rAp, cAp, amap);
+  permute(rG, cG, rGp, cGp, gmap);
+  // G^T in solver order: counting sort of the permuted G by column
+  std::vector<int> rGT(nn + 1, 0), cGT(nnzG), gtmap(nnzG);
+  for (int v : cGp) rGT[v + 1]++;
+  for (int i = 0; i < nn; ++i) rGT[i + 1] += rGT[i];
+  {
+    std::vector<int> fill(rGT.begin(), rGT.end() - 1);
+    for (int i = 0; i < nn; ++i)
+      for (int p = rGp[i]; p < rGp[i + 1]; ++p) {
+        const int q = fill[cGp[p]]++;
+        cGT[q] = i;
+        gtmap[q] = gmap[p];
+      }
+  }
+  auto up = [&](dogpup::Buf& b, const std::vector<int>& v) -> int {
+    DP_TRY(b.ensure(v.size() * 4));
+    DP_HIP(hipMemcpy(b.p, v.data(), v.size() * 4, hipMemcpyHostToDevice));
+    return DOGPUP_OK;
+  };
+  DP_TRY(up(S->pAr, rAp)); DP_TRY(up(S->pAc, cAp)); DP_TRY(up(S->amap, amap));
+  DP_TRY(up(S->pGr, rGp)); DP_TRY(up(S->pGc, cGp)); DP_TRY(up(S->gmap, gmap));
+  DP_TRY(up(S->pGTr, rGT)); DP_TRY(up(S->pGTc, cGT)); DP_TRY(up(S->gtmap, gtmap));
+  DP_TRY(up(S->order, order));
+  S->rA = S->pAr.as<int>(); S->cA = S->pAc.as<int>();
+  S->rG = S->pGr.as<int>(); S->cG = S->pGc.as<int>();
+  S->rGT = S->pGTr.as<int>(); S->cGT = S->pGTc.as<int>();
+
   const size_t csize = mode == DOGPUP_MODE_C64_REFINE ? 8 : 16;
   const size_t per_group = align256((size_t)nn * nsp * csize);
+  const size_t per_group64 = align256((size_t)nn * nsp * 16);
   const int nvec = nvec_for(mode);
-  long long chunk = (long long)(ws_budget_bytes() / ((double)per_group * nvec));
+  // vector workspace + permuted b and x (complex128) per group
+  long long chunk = (long long)(ws_budget_bytes() / ((double)per_group * nvec + 2.0 * per_group64));
   if (chunk < 1) chunk = 1;
   if (chunk > S->ngroups) chunk = S->ngroups;
   S->chunk = (int)chunk;
   DP_TRY(S->ws.ensure((size_t)nvec * align256(per_group * S->chunk)));
+  DP_TRY(S->bperm.ensure(per_group64 * S->chunk));
+  DP_TRY(S->xperm.ensure(per_group64 * S->chunk));
   const size_t pbytes = (size_t)S->chunk * S->nbx * nsp * 16;
   DP_TRY(S->part1.ensure(pbytes));
   DP_TRY(S->part2.ensure(pbytes));
@@ -1411,26 +1622,33 @@ int dogpup_solver_set_matrix(dogpup_solver_t S, const double* d_valA_re, const d
   DP_HIP(hipMemcpyAsync(S->a64re.p, d_valA_re, nA * 8, hipMemcpyDeviceToDevice, st));
   DP_HIP(hipMemcpyAsync(S->a64im.p, d_valA_im, nA * 8, hipMemcpyDeviceToDevice, st));
   DP_HIP(hipMemcpyAsync(S->fscale.p, d_fscale, (size_t)S->nf * 8, hipMemcpyDeviceToDevice, st));
+  // FSAI factors in the reference node order (pattern = the caller's tril(A))
   const size_t nG = (size_t)S->nnzG * S->nf;
   DP_TRY(S->g64.ensure(nG * 16));
   FsaiOut fo{S->g64.as<double>(), nullptr, S->nnzG, FSAI_OUT_SOLVER};
-  DP_TRY(fsai_launch(S->a64re.as<double>(), S->a64im.as<double>(), 0, S->rA, S->cA, S->rG, S->cG, S->nn,
-                     S->nf, S->fscale.as<double>(), fo, st));
+  DP_TRY(fsai_launch(S->a64re.as<double>(), S->a64im.as<double>(), 0, S->rA0, S->cA0, S->rG0, S->cG0,
+                     S->nn, S->nf, S->fscale.as<double>(), fo, st));
+  // values in solver node order
+  DP_TRY(S->a64pre.ensure(nA * 8));
+  DP_TRY(S->a64pim.ensure(nA * 8));
+  k_gather_real<double><<<grid1(nA), 256, 0, st>>>(S->a64re.as<double>(), S->amap.as<int>(), S->a64pre.as<double>(), nA);
+  k_gather_real<double><<<grid1(nA), 256, 0, st>>>(S->a64im.as<double>(), S->amap.as<int>(), S->a64pim.as<double>(), nA);
+  DP_LAUNCH_CHECK();
   if (S->mode == DOGPUP_MODE_C64_REFINE) {
     DP_TRY(S->a32re.ensure(nA * 4));
     DP_TRY(S->a32im.ensure(nA * 4));
-    k_cast_f32<<<grid1(nA), 256, 0, st>>>(S->a64re.as<double>(), S->a32re.as<float>(), nA);
-    k_cast_f32<<<grid1(nA), 256, 0, st>>>(S->a64im.as<double>(), S->a32im.as<float>(), nA);
+    k_gather_real<float><<<grid1(nA), 256, 0, st>>>(S->a64re.as<double>(), S->amap.as<int>(), S->a32re.as<float>(), nA);
+    k_gather_real<float><<<grid1(nA), 256, 0, st>>>(S->a64im.as<double>(), S->amap.as<int>(), S->a32im.as<float>(), nA);
     DP_TRY(S->g32.ensure(nG * 8));
-    k_cast_f32<<<grid1(2 * nG), 256, 0, st>>>(S->g64.as<double>(), S->g32.as<float>(), 2 * nG);
     DP_TRY(S->gt32.ensure(nG * 8));
-    k_gather_gt<float><<<grid1(nG), 256, 0, st>>>(S->g64.as<double>(), S->permGT, S->gt32.as<float>(),
-                                                   S->nnzG, S->nf);
+    k_gather_cplx<float><<<grid1(nG), 256, 0, st>>>(S->g64.as<double>(), S->gmap.as<int>(), S->g32.as<float>(), S->nnzG, S->nf);
+    k_gather_cplx<float><<<grid1(nG), 256, 0, st>>>(S->g64.as<double>(), S->gtmap.as<int>(), S->gt32.as<float>(), S->nnzG, S->nf);
     DP_LAUNCH_CHECK();
   } else {
-    DP_TRY(S->gt64.ensure(nG * 16));
-    k_gather_gt<double><<<grid1(nG), 256, 0, st>>>(S->g64.as<double>(), S->permGT,
-                                                    S->gt64.as<double>(), S->nnzG, S->nf);
+    DP_TRY(S->gp64.ensure(nG * 16));
+    DP_TRY(S->gtp64.ensure(nG * 16));
+    k_gather_cplx<double><<<grid1(nG), 256, 0, st>>>(S->g64.as<double>(), S->gmap.as<int>(), S->gp64.as<double>(), S->nnzG, S->nf);
+    k_gather_cplx<double><<<grid1(nG), 256, 0, st>>>(S->g64.as<double>(), S->gtmap.as<int>(), S->gtp64.as<double>(), S->nnzG, S->nf);
     DP_LAUNCH_CHECK();
   }
   S->have_matrix = true;

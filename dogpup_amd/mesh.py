@@ -142,7 +142,6 @@ class DMesh:
         self.bface = topology.robin_faces(self.elem, self.bnd, self.reference_faces, self.face)
         self.pattern = topology.system_pattern(nn, self.elem, self.bface)
         rG, cG = topology.fsai_pattern(self.pattern.rptr, self.pattern.cptr)
-        rGT, cGT, perm = topology.transpose_pattern(rG, cG)
         self.g_pattern = (rG, cG)
         d = self._dev = {}
         d["node"] = _f64(self.node)
@@ -152,9 +151,9 @@ class DMesh:
         d["rA"], d["cA"] = _i32(self.pattern.rptr), _i32(self.pattern.cptr)
         d["slot_ptr"], d["slot_src"] = _i32(self.pattern.slot_ptr), _i32(self.pattern.slot_src)
         d["rG"], d["cG"] = _i32(rG), _i32(cG)
-        d["rGT"], d["cGT"], d["permGT"] = _i32(rGT), _i32(cGT), _i32(perm)
         self.nnzA = int(self.pattern.cptr.size)
         self.nnzG = int(cG.size)
+        self.solver_order = topology.morton_order(self.node, self.elem)
 
     @property
     def nn(self):
@@ -253,7 +252,7 @@ class FluenceSolver:
     """Device-resident batched FSAI-BiCGSTAB for one mesh, column count and
     mode (wraps dogpup_solver_*)."""
 
-    def __init__(self, mesh: DMesh, ncols: int, mode: int):
+    def __init__(self, mesh: DMesh, ncols: int, mode: int, order=None):
         if mesh.optode is None:
             raise ValueError("add an optode first (the frequency axis comes from it)")
         self.mesh = mesh
@@ -263,10 +262,12 @@ class FluenceSolver:
         self.ncb = -(-ncols // self.nsp)
         self.mode = mode
         d = mesh._dev
+        if order is None:
+            order = mesh.solver_order
         h = C.c_void_p()
         _lib.call("dogpup_solver_create", C.byref(h), mesh.nn, self.nf, ncols, self.nsp, mode,
                   mesh.nnzA, _lib.ptr(d["rA"]), _lib.ptr(d["cA"]), mesh.nnzG, _lib.ptr(d["rG"]),
-                  _lib.ptr(d["cG"]), _lib.ptr(d["rGT"]), _lib.ptr(d["cGT"]), _lib.ptr(d["permGT"]))
+                  _lib.ptr(d["cG"]), _lib.ptr(np.ascontiguousarray(order, np.int32)))
         self.h = h
         self._version = None
 

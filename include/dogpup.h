@@ -128,14 +128,17 @@ int dogpup_assemble(int nn, int ne, int nbf, int nnz, const double* d_node, cons
 typedef struct dogpup_solver* dogpup_solver_t;
 
 /* d_rptrA/d_cptrA: full symmetric CSR of A (sorted columns).  d_rptrG/d_cptrG:
- * FSAI pattern (lower, ascending, diagonal last).  d_rptrGT/d_cptrGT/d_permGT:
- * CSR of G^T and the map from G^T slots to G slots.  ncols = columns per
- * frequency (sources); nsp = group width (8/16/32/64, >= columns per block).
- * The handle keeps the pattern pointers (caller keeps them alive). */
+ * FSAI pattern (lower, ascending, diagonal last), both in the caller's node
+ * order; the FSAI factors are computed on these, as the reference does.
+ * h_order (host, nn entries, or NULL for the identity) is the node order the
+ * iterations run in: solver row i holds caller node h_order[i] (a
+ * space-filling-curve order keeps each row tile's stencil in cache).  The
+ * solve takes and returns vectors in the caller's order.  ncols = columns
+ * per frequency (sources); nsp = group width (8/16/32/64).  The handle keeps
+ * the caller's pattern pointers (caller keeps them alive). */
 int dogpup_solver_create(dogpup_solver_t* out, int nn, int nfreq, int ncols, int nsp, int mode,
                          int nnzA, const int* d_rptrA, const int* d_cptrA, int nnzG,
-                         const int* d_rptrG, const int* d_cptrG, const int* d_rptrGT,
-                         const int* d_cptrGT, const int* d_permGT);
+                         const int* d_rptrG, const int* d_cptrG, const int* h_order);
 int dogpup_solver_destroy(dogpup_solver_t s);
 
 /* Set A (FP64 values on the handle's pattern; frequency k uses

@@ -40,6 +40,5 @@ def test_argument_errors_without_gpu():
     assert e.value.code == _lib.ERR_ARG
     h = C.c_void_p()
     with pytest.raises(_lib.DogpupError) as e:
-        _lib.call("dogpup_solver_create", C.byref(h), 10, 2, 4, 12, 0, 1, None, None, 1, None, None, None,
-                  None, None)
+        _lib.call("dogpup_solver_create", C.byref(h), 10, 2, 4, 12, 0, 1, None, None, 1, None, None, None)
     assert e.value.code == _lib.ERR_ARG and "group width" in str(e.value)
