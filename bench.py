@@ -240,6 +240,8 @@ def main(argv=None):
     release(D)
 
     names = ["G/G^T SpMV", "A SpMV", "BiCGSTAB vector passes", "FP64 refinement passes"]
+    pmc_kernel = {0: "dogpup::k_spmv<float, 32, 1, 0>", 1: "dogpup::k_spmv<float, 32, 0, 0>",
+                  2: "dogpup::k_update_xr<float, 32>"}
     cls = int(np.argmax(prof[:, 0]))
     ms, byts, nl = prof[cls]
     achieved = byts / (ms * 1e-3) / 1e9
@@ -250,6 +252,17 @@ def main(argv=None):
     extra["all_kernels_GBps"] = all_gbs
     extra["copy_GBps"] = copy_gbs
 
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_bicgstab_c64_slab269k_2048rhs.json")
+    if not a.small and os.path.exists(tfile) and cls in pmc_kernel:
+        with open(tfile) as fh:
+            tr = json.load(fh)["kernels"].get(pmc_kernel[cls])
+        if tr:
+            # traffic/algorithmic ratio of the full-activity launch (rocprofv3 PMC passes
+            # FETCH_SIZE and WRITE_SIZE, calibrated on a plain copy), applied to this
+            # run's average launch
+            traffic = {"bytes_per_launch": tr["ratio"] * byts / max(nl, 1), "ratio": tr["ratio"],
+                       "source": os.path.relpath(tfile, ROOT)}
     if not a.no_fp64:
         log("FP64 fused solve")
         dt64, phi64, _ = timed_solves(D, "s", "fp64", 1e-12, 1, 0)
@@ -301,7 +314,10 @@ def main(argv=None):
                    if not a.small else "slab 60x60x30 mm 2 mm grid, 16 sources x 8 frequencies",
                    "nodes": D.nn, "rhs": nrhs, "tol": a.tol},
         "roofline": {"bound": "hbm", "kernel": names[cls], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_ratio": traffic["ratio"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
                      "bytes_per_launch": byts / max(nl, 1), "avg_launch_ms": ms / max(nl, 1)},
         "cpu_baseline": cpu,
         "extra": extra,

@@ -78,3 +78,69 @@ def jacobian_tg(mesh, gate_fc, mode="mixed", tol=1e-10, dxyz=None, scale=None, f
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     return out, {"solve_s": t1 - t0, "kernel_s": t2 - t1, "field": field, "coef": coef, "link": link}
+
+
+def _links_by_source(link, ns):
+    order = np.argsort(link[:, 0], kind="stable").astype(np.int32)
+    ptr = np.zeros(ns + 1, np.int64)
+    np.add.at(ptr, link[:, 0] + 1, 1)
+    return np.cumsum(ptr).astype(np.int32), order
+
+
+def tg_data(mesh, field, gate_fc):
+    """Time-gated boundary data y = fc2tg(meas_flu(phi)) (NM x NG, FP64)."""
+    from .transforms import fc2tg
+    data = mesh.meas_flu(field)                           # NM x NF complex
+    return fc2tg(data, gate_fc, mesh.optode.fAxis, mesh.optode.tAxis.size, 1)
+
+
+def vjp(mesh, field, gate_fc, ybar, exact=False, dxyz=None):
+    """g = J_TG^T ybar for nodal mua (kappa fixed), J not formed.  exact=False
+    is the reference's nodal-product Jacobian, exact=True the exact FEM
+    derivative."""
+    opt = mesh.optode
+    ns = len(opt.s_bary[0])
+    nf, nt = opt.Nf, opt.tAxis.size
+    g = gate_fc.cpu().numpy() if torch.is_tensor(gate_fc) else np.asarray(gate_fc)
+    w = np.where(np.arange(nf) == 0, 0.5, 1.0)
+    cp = 2.0 * nt * w[None, :] * np.conj(g) / opt.s_fpsf[None, :]          # NG x NF
+    yb = ybar.detach().cpu().numpy() if torch.is_tensor(ybar) else np.asarray(ybar)
+    D = np.ascontiguousarray((yb @ cp).T)                                 # NF x NM
+    Dd = torch.as_tensor(D, dtype=torch.complex128, device=DEV)
+    link = np.stack([opt.link[:, 0], opt.link[:, 1] + ns], 1).astype(np.int32)
+    lptr, lidx = _links_by_source(opt.link, ns)
+    link_d, lptr_d, lidx_d = (torch.as_tensor(a, device=DEV) for a in (link, lptr, lidx))
+    E = torch.empty((mesh.nn, nf, ns), dtype=torch.complex128, device=DEV)
+    out = torch.empty(mesh.nn, dtype=torch.float64, device=DEV)
+    if dxyz is None:
+        dxyz = getattr(mesh, "dxyz", 1.0)
+    x = torch.view_as_real(field.data)
+    _lib.call("dogpup_jacobian_vjp", _lib.ptr(x), _lib.ptr(torch.view_as_real(Dd)), _lib.ptr(link_d),
+              _lib.ptr(lptr_d), _lib.ptr(lidx_d), _lib.ptr(mesh._dev["node"]), _lib.ptr(mesh._dev["elem"]),
+              _lib.ptr(torch.view_as_real(E)), _lib.ptr(out), mesh.nn, mesh.elem.shape[0], nf, field.ncols,
+              field.nsp, ns, opt.link.shape[0], int(exact), float(dxyz) ** 3, _lib.stream_ptr())
+    return out
+
+
+class AbsorptionTG(torch.autograd.Function):
+    """mua (NN, float64) -> time-gated boundary data (NM x NG) with kappa held
+    at the mesh's current values; backward = J_TG^T ybar through
+    dogpup_jacobian_vjp (nodal-product Jacobian of dMesh.m:206-247, or the
+    exact derivative with exact=True)."""
+
+    @staticmethod
+    def forward(ctx, mua, mesh, gate_fc, exact=False, mode="fp64", tol=1e-12):
+        mesh.update_properties(np.stack([mua.detach().cpu().numpy(), mesh.musp], 1), kappa=mesh.kappa.copy())
+        field = mesh.solve_point_sources("sd", mode=mode, tol=tol)
+        ctx.mesh, ctx.field, ctx.gate_fc, ctx.exact = mesh, field, gate_fc, exact
+        ctx.in_device, ctx.in_dtype = mua.device, mua.dtype
+        return tg_data(mesh, field, gate_fc)
+
+    @staticmethod
+    def backward(ctx, ybar):
+        g = vjp(ctx.mesh, ctx.field, ctx.gate_fc, ybar, exact=ctx.exact)
+        return g.to(ctx.in_device, ctx.in_dtype), None, None, None, None, None
+
+
+def absorption_tg(mua, mesh, gate_fc, exact=False, mode="fp64", tol=1e-12):
+    return AbsorptionTG.apply(mua, mesh, gate_fc, exact, mode, tol)

@@ -160,12 +160,17 @@ class DMesh:
         return self.node.shape[0]
 
     # -- assembly (dMesh.m:80-88, 1033-1194) -----------------------------------
-    def update_properties(self, props=None):
+    def update_properties(self, props=None, kappa=None):
+        """Re-assemble K and M (dMesh.m:80-88).  ``kappa`` (optional, NN)
+        overrides kappa = 1/(3(mua+musp)); the Jacobian and its finite
+        differences hold kappa fixed."""
         if props is not None:
             props = np.asarray(props, float)
             self.mua[:] = props[:, 0]
             self.musp[:] = props[:, 1]
             self.kappa[:] = 1.0 / (3.0 * (self.mua + self.musp))
+        if kappa is not None:
+            self.kappa[:] = np.asarray(kappa, float)
         d = self._dev
         ne = self.elem.shape[0]
         self.K = torch.empty(self.nnzA, dtype=torch.float64, device=DEV)   # Re A (full symmetric)

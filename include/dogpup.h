@@ -231,6 +231,51 @@ int dogpup_jacobian_tg(const double* d_field, const double* d_coef, const int* d
                        const float* d_scale, float* d_out, int nn, int nf, int ng, int ncols, int nsp,
                        int nm, int nv, void* stream);
 
+/* Vector-Jacobian product of the time-gated absorption Jacobian, without
+ * forming J (the backward of mua -> time-gated data, dMesh.m:206-247 with
+ * fc2tg): g = J_TG^T ybar.
+ * d_field: forward + adjoint fluence of one solve (complex128 solver layout,
+ * ncols columns, sources in columns 0..ns-1); d_D: NF x NM complex128,
+ * D[k][m] = sum_g ybar[m][g] * 2 Nt w_k conj(gate[g][k]) / s_fpsf[k];
+ * link: NM x 2 (source column, detector column); link_ptr/link_idx: links
+ * grouped by source (CSR over sources).  exact = 0: the reference's
+ * nodal-product Jacobian (scale dxyz3 = dxyz^3); exact = 1: the exact FEM
+ * derivative at fixed kappa, element-local int N_v N_i N_j (needs node,
+ * elem).  d_E: NN*NF*NS complex128 workspace; d_g: NN doubles (output). */
+int dogpup_jacobian_vjp(const double* d_field, const double* d_D, const int* d_link,
+                        const int* d_link_ptr, const int* d_link_idx, const double* d_node,
+                        const int* d_elem, double* d_E, double* d_g, int nn, int ne, int nf, int ncols,
+                        int nsp, int ns, int nm, int exact, double dxyz3, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Dense algebra of the weighted LM step (reconstruction/tik_inv.m:11-19,
+ * reshape_J.m:13-15): normal matrices on FP32 MFMA with FP32 accumulation,
+ * FP64 Cholesky of the damped system.  Row-major, device pointers.
+ * ------------------------------------------------------------------------- */
+
+/* C = alpha * A * op(B): trans_b = 1 -> op(B) = B^T with B N x K (ldb);
+ * trans_b = 0 -> B is K x N.  A is M x K (lda); C is M x N (ldc), FP64 when
+ * out_f64 else FP32; lower_only writes only C[i][j], j <= i (for J J^T). */
+int dogpup_gemm_f32(const float* d_A, const float* d_B, void* d_C, int M, int N, int K, long long lda,
+                    long long ldb, long long ldc, float alpha, int trans_b, int out_f64, int lower_only,
+                    void* stream);
+
+/* out[j] = sum_i A[i][j]^2 (FP64), A rows x cols FP32 (tik_inv's alpha). */
+int dogpup_col_sumsq(const float* d_A, int rows, long long cols, double* d_out, void* stream);
+
+/* out[j] = sum_i A[i][j] y[i] (A^T y, FP64 accumulation). */
+int dogpup_gemv_t(const float* d_A, const double* d_y, int rows, long long cols, double* d_out,
+                  void* stream);
+
+/* In-place lower Cholesky of (A + shift*I), A n x n FP64 row-major (lower
+ * triangle used).  *h_info = first non-positive pivot (1-based) or 0;
+ * returns DOGPUP_ERR_ARG when the matrix is not positive definite.
+ * Synchronous. */
+int dogpup_chol_factor(double* d_A, int n, double shift, int* h_info, void* stream);
+
+/* B <- (L L^T)^-1 B, B n x nrhs FP64 row-major, L from dogpup_chol_factor. */
+int dogpup_chol_solve(const double* d_L, double* d_B, int n, int nrhs, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -204,3 +204,47 @@ def test_jacobian_tg(setup):
           np.abs(info["link"].cpu().numpy() - link_all).max(), "J diff", (J2 - out).abs().max().item(),
           "shapes", tuple(J2.shape), tuple(out.shape))
     assert rel(np.transpose(J2.cpu().numpy(), (2, 1, 0)), ref) < 1e-5
+
+
+def test_autograd_vjp_vs_fd_and_reference_jacobian(setup):
+    """AbsorptionTG: exact-mode gradient vs central finite differences in
+    nodal mua (kappa fixed); nodal-mode gradient vs the contraction of the
+    oracle's reference Jacobian."""
+    from dogpup_amd import jacobian as jac
+    from oracle import jacobian as ojac, optode as oopt, solver as osol, transforms as otr
+    D, om, opt = setup["D"], setup["om"], setup["opt"]
+    t, f, nf = setup["t"], setup["f"], setup["nf"]
+    gates = np.stack([((t >= a) & (t < a + 500e-12)).astype(float) for a in (0.0, 200e-12, 400e-12)])
+    gfc = otr.td2fc(gates, f, t, 1)
+    kappa0 = D.kappa.copy()
+    mua0 = torch.tensor(D.mua.copy(), dtype=torch.float64, requires_grad=True)
+    rng = np.random.default_rng(5)
+    ybar = torch.as_tensor(rng.normal(size=(opt.link.shape[0], gfc.shape[0])), device="cuda")
+    y = jac.absorption_tg(mua0, D, gfc, True, "fp64", 1e-13)
+    (y * ybar).sum().backward()
+    g_exact = mua0.grad.numpy().copy()
+    nodes = [int(np.argmin(np.linalg.norm(D.node - np.array(p), axis=1))) for p in [(10, 10, 6), (7, 12, 8), (14, 9, 3)]]
+    for v in nodes:
+        d = 1e-5
+        vals = []
+        for sgn in (1, -1):
+            mu = D.mua.copy() * 0 + mua0.detach().numpy()
+            mu[v] += sgn * d
+            with torch.no_grad():
+                yy = jac.absorption_tg(torch.as_tensor(mu), D, gfc, True, "fp64", 1e-13)
+            vals.append(float((yy * ybar).sum()))
+        fd = (vals[0] - vals[1]) / (2 * d)
+        assert abs(fd - g_exact[v]) <= 1e-5 * abs(g_exact).max(), (v, fd, g_exact[v])
+    # restore the mesh and check the nodal mode against the oracle's J
+    D.update_properties(np.stack([mua0.detach().numpy(), D.musp], 1), kappa=kappa0)
+    mu1 = torch.tensor(D.mua.copy(), dtype=torch.float64, requires_grad=True)
+    y = jac.absorption_tg(mu1, D, gfc, False, "fp64", 1e-13)
+    (y * ybar).sum().backward()
+    g_nodal = mu1.grad.numpy()
+    sid, sw = opt.s_bary
+    did, dw = opt.d_bary
+    phi = osol.direct(setup["A"], oopt.source_matrix(om, sid, sw, opt.s_fpsf))
+    phiA = osol.direct(setup["A"], oopt.source_matrix(om, did, dw, opt.s_fpsf))
+    Jref = ojac.j_tg(phi, phiA, setup["link"], opt.s_fpsf, 1.0, gfc, f, t.size)   # NM x NG x NV, dxyz = 1
+    gref = np.einsum("mg,mgv->v", ybar.cpu().numpy(), Jref)
+    assert rel(g_nodal, gref) < 1e-8
