@@ -20,6 +20,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <algorithm>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(256) void k_spmv(
     const int* __restrict__ rptr, const int* __restrict__ cptr, const T* __restrict__ v0,
     const T* __restrict__ v1, long long nnz, const double* __restrict__ fscale,
     const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ w, double* __restrict__ part1,
-    double* __restrict__ part2, int nn, int ncb, int g0, int nbx,
+    double* __restrict__ part2, int nn, int ncb, int g0, int nbx, int tile_rows,
     const int* __restrict__ grp_active) {
   using G = Geo<T, NSP>;
   constexpr int CPL = G::CPL;
@@ -81,8 +82,8 @@ __global__ __launch_bounds__(256) void k_spmv(
   Acc<CPL> d1, d2;
   d1.zero();
   d2.zero();
-  const int tile0 = xcd_tile(bx) * kTile;
-  const int tile1 = min(tile0 + kTile, nn);
+  const int tile0 = xcd_tile(bx) * tile_rows;
+  const int tile1 = min(tile0 + tile_rows, nn);
   for (int r0 = tile0; r0 < tile1; r0 += kChunkRows) {
     const int r1 = min(r0 + kChunkRows, nn);
     __syncthreads();  // the previous chunk's LDS reads are done
@@ -208,8 +209,8 @@ __device__ inline void load_coef(const double* __restrict__ c, long long rhs0, T
   const long long rhs0 = (long long)g * NSP + c0;                              \
   const size_t gbase = (size_t)g * nn;                                         \
   (void)sh; (void)rhs0;                                                        \
-  for (int st = 0; st < G::STEPS; ++st) {                                      \
-    const int row = xcd_tile(bx) * kTile + st * G::RPB + wv * G::RPW + rsub;   \
+  for (int st = 0; st < tile_rows / G::RPB; ++st) {                            \
+    const int row = xcd_tile(bx) * tile_rows + st * G::RPB + wv * G::RPW + rsub; \
     if (row >= nn) break;                                                      \
     const size_t off = ((gbase + row) * NSP + c0) * 2;
 
@@ -219,7 +220,7 @@ __device__ inline void load_coef(const double* __restrict__ c, long long rhs0, T
 template <typename T, int NSP>
 __global__ __launch_bounds__(256) void k_axpy_s(const T* __restrict__ r, const T* __restrict__ v,
                                                 T* __restrict__ s, const double* __restrict__ alpha,
-                                                double* __restrict__ part1, int nn, int nbx,
+                                                double* __restrict__ part1, int nn, int nbx, int tile_rows,
                                                 const int* __restrict__ grp_active) {
   Acc<Geo<T, NSP>::CPL> d1;
   d1.zero();
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(256) void k_update_xr(
     const T* __restrict__ s, const T* __restrict__ t, const T* __restrict__ rhat,
     const double* __restrict__ alpha, const double* __restrict__ omega,
     const int* __restrict__ status, double* __restrict__ part1, double* __restrict__ part2, int nn,
-    int nbx, const int* __restrict__ grp_active) {
+    int nbx, int tile_rows, const int* __restrict__ grp_active) {
   Acc<Geo<T, NSP>::CPL> d1, d2;
   d1.zero();
   d2.zero();
@@ -294,7 +295,7 @@ template <typename T, int NSP>
 __global__ __launch_bounds__(256) void k_update_p(T* __restrict__ p, const T* __restrict__ r,
                                                   const T* __restrict__ v,
                                                   const double* __restrict__ beta,
-                                                  const double* __restrict__ omega, int nn, int nbx,
+                                                  const double* __restrict__ omega, int nn, int nbx, int tile_rows,
                                                   const int* __restrict__ grp_active) {
   T br[Geo<T, NSP>::CPL], bi[Geo<T, NSP>::CPL], omr[Geo<T, NSP>::CPL], omi[Geo<T, NSP>::CPL];
   ROW_LOOP_BEGIN
@@ -323,7 +324,7 @@ template <typename T, int NSP>
 __global__ __launch_bounds__(256) void k_init(const T* __restrict__ b, T* __restrict__ x,
                                               T* __restrict__ r, T* __restrict__ rhat,
                                               T* __restrict__ p, double* __restrict__ part1,
-                                              double* __restrict__ part2, int nn, int nbx,
+                                              double* __restrict__ part2, int nn, int nbx, int tile_rows,
                                               const int* __restrict__ grp_active) {
   Acc<Geo<T, NSP>::CPL> d1, d2;
   d1.zero();
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(256) void k_residual64(
     const double* __restrict__ b, const double* __restrict__ x64, float* __restrict__ r32,
     float* __restrict__ rhat32, float* __restrict__ p32, float* __restrict__ x32,
     const int* __restrict__ ostatus, int write, double* __restrict__ part1,
-    double* __restrict__ part2, int nn, int ncb, int g0, int nbx) {
+    double* __restrict__ part2, int nn, int ncb, int g0, int nbx, int tile_rows) {
   using G = Geo<double, NSP>;  // one complex128 per lane
   using GF = Geo<float, NSP>;
   (void)sizeof(GF);
@@ -378,8 +379,8 @@ __global__ __launch_bounds__(256) void k_residual64(
   Acc<1> d1, d2;
   d1.zero();
   d2.zero();
-  for (int st = 0; st < G::STEPS; ++st) {
-    const int row = xcd_tile(bx) * kTile + st * G::RPB + wv * G::RPW + rsub;
+  for (int st = 0; st < tile_rows / G::RPB; ++st) {
+    const int row = xcd_tile(bx) * tile_rows + st * G::RPB + wv * G::RPW + rsub;
     if (row >= nn) break;
     const size_t off = ((gbase + row) * NSP + c);
     double yr = 0.0, yi = 0.0;
@@ -437,7 +438,7 @@ __global__ __launch_bounds__(256) void k_add_correction(double* __restrict__ x64
 // part1 = sum a b (mode 0) or sum |a|^2 (mode 1)
 template <typename T, int NSP>
 __global__ __launch_bounds__(256) void k_dot(const T* __restrict__ a, const T* __restrict__ bb,
-                                             int mode, double* __restrict__ part1, int nn, int nbx,
+                                             int mode, double* __restrict__ part1, int nn, int nbx, int tile_rows,
                                              const int* __restrict__ grp_active) {
   Acc<Geo<T, NSP>::CPL> d1;
   d1.zero();
@@ -463,7 +464,7 @@ template <typename T, int NSP>
 __global__ __launch_bounds__(256) void k_axpby(T* __restrict__ out, const T* __restrict__ a,
                                                const T* __restrict__ bb,
                                                const double* __restrict__ coef, int op, int nn,
-                                               int nbx, const int* __restrict__ grp_active) {
+                                               int nbx, int tile_rows, const int* __restrict__ grp_active) {
   T cr[Geo<T, NSP>::CPL], ci[Geo<T, NSP>::CPL];
   ROW_LOOP_BEGIN
   if (st == 0) {
@@ -866,7 +867,7 @@ struct dogpup_solver {
   // values: a64re/a64im and g64 in reference order; the rest in solver order
   dogpup::Buf a64re, a64im, a64pre, a64pim, a32re, a32im, fscale, g64, gp64, gtp64, g32, gt32;
   dogpup::Buf ws, part1, part2, state, rstate, bperm, xperm;
-  int chunk = 0, nbx = 0;
+  int chunk = 0, nbx = 0, tile_rows = 512, lds_pad = 0;
   bool have_matrix = false;
   double stats[4] = {0, 0, 0, 0};
   int* h_counter = nullptr;  // pinned: [0] active RHS, [1] active groups
@@ -880,8 +881,22 @@ struct dogpup_solver {
   };
   bool profile = false;
   std::vector<Rec> recs;
+  // captured BiCGSTAB iterations, keyed by (chunk, maxit, precision)
+  bool use_graph = true;
+  int graph_k = 4;
+  std::map<long long, hipGraphExec_t> graphs;
+  hipStream_t own = nullptr;   // the solve's own (capturable) stream
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  void clear_graphs() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+  }
   double prof_ms[4] = {0, 0, 0, 0}, prof_bytes[4] = {0, 0, 0, 0}, prof_n[4] = {0, 0, 0, 0};
   ~dogpup_solver() {
+    clear_graphs();
+    if (own) (void)hipStreamDestroy(own);
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
     for (auto& r : recs) {
       (void)hipEventDestroy(r.a);
       (void)hipEventDestroy(r.b);
@@ -946,6 +961,11 @@ static unsigned grid1(long long n) {
   return (unsigned)b;
 }
 
+// Clears n ints (a kernel, so that captured graphs hold no memset node).
+__global__ void k_zero_ints(int* __restrict__ p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
 // Everything one chunk of groups needs on the device.
 template <typename T>
 struct Vecs {
@@ -968,12 +988,20 @@ struct Chunk {
     return tr ? (const T*)S->gtp64.p : (const T*)S->gp64.p;
   }
 
+  template <int MAT, int EPI>
+  int pad_lds() {
+    if (S->lds_pad > 0)
+      DP_HIP(hipFuncSetAttribute((const void*)k_spmv<T, NSP, MAT, EPI>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, S->lds_pad));
+    return DOGPUP_OK;
+  }
   template <int EPI>
   int spmvA(const T* x, T* y, const T* w, const int* ga) {
+    DP_TRY((pad_lds<MAT_A, EPI>()));
     pbegin();
-    k_spmv<T, NSP, MAT_A, EPI><<<vgrid(), 256, 0, st>>>(S->rA, S->cA, a0(), a1(), 0,
+    k_spmv<T, NSP, MAT_A, EPI><<<vgrid(), 256, S->lds_pad, st>>>(S->rA, S->cA, a0(), a1(), 0,
                                                         S->fscale.template as<double>(), x, y, w, p1,
-                                                        p2, S->nn, S->ncb, g0, S->nbx, ga);
+                                                        p2, S->nn, S->ncb, g0, S->nbx, S->tile_rows, ga);
     DP_LAUNCH_CHECK();
     pend(1, groups(ga != nullptr) * vbytes() * (EPI == EPI_STORE ? 2 : 3) +
                 (double)S->nnzA * (2 * sizeof(T) + 4) + 4.0 * (S->nn + 1));
@@ -983,9 +1011,10 @@ struct Chunk {
   int spmvG(bool tr, const T* x, T* y, const T* w, const int* ga) {
     const int* rp = tr ? S->rGT : S->rG;
     const int* cp = tr ? S->cGT : S->cG;
+    DP_TRY((pad_lds<MAT_G, EPI>()));
     pbegin();
-    k_spmv<T, NSP, MAT_G, EPI><<<vgrid(), 256, 0, st>>>(rp, cp, gvals(tr), nullptr, S->nnzG, nullptr,
-                                                        x, y, w, p1, p2, S->nn, S->ncb, g0, S->nbx, ga);
+    k_spmv<T, NSP, MAT_G, EPI><<<vgrid(), 256, S->lds_pad, st>>>(rp, cp, gvals(tr), nullptr, S->nnzG, nullptr,
+                                                        x, y, w, p1, p2, S->nn, S->ncb, g0, S->nbx, S->tile_rows, ga);
     DP_LAUNCH_CHECK();
     pend(0, groups(ga != nullptr) * (vbytes() * (EPI == EPI_STORE ? 2 : (EPI == EPI_DOT2 ? 2 : 3)) +
                                     (double)S->nnzG * (2 * sizeof(T) + 4) + 4.0 * (S->nn + 1)));
@@ -1000,7 +1029,8 @@ struct Chunk {
     return DOGPUP_OK;
   }
   int reset_count() {
-    DP_HIP(hipMemsetAsync(bs.counter, 0, 2 * sizeof(int), st));
+    k_zero_ints<<<1, 64, 0, st>>>(bs.counter, 2);
+    DP_LAUNCH_CHECK();
     return DOGPUP_OK;
   }
   // profiling brackets: class 0 = G/G^T SpMV, 1 = A SpMV, 2 = vector passes,
@@ -1024,11 +1054,51 @@ struct Chunk {
   // Fused BiCGSTAB.  If b != nullptr the init pass runs from b (x = 0);
   // otherwise r, rhat, p, x and the state are already initialised (the
   // refinement residual pass did it).  Returns iterations run.
-  int run_fused(Vecs<T> V, const T* b, int maxit, const int* mask, int* iters_run) {
+  // One BiCGSTAB iteration over every active group (no host synchronisation,
+  // no memset nodes: it is captured into a HIP graph when profiling is off).
+  int body(Vecs<T> V, int maxit) {
     const int* ga = bs.grp_active;
     const int nsp = NSP;
+    DP_TRY(spmvG<EPI_STORE>(false, V.p, V.u, nullptr, ga));      // u = G p
+    DP_TRY(spmvG<EPI_STORE>(true, V.u, V.h, nullptr, ga));       // h = G^T u
+    DP_TRY(spmvA<EPI_STORE_DOT>(V.h, V.v, V.rhat, ga));          // v = A h, rhat.v
+    k_sc_alpha<<<gcount, kScalThreads, 0, st>>>(bs, p1, nsp, S->nbx);
+    DP_LAUNCH_CHECK();
+    pbegin();
+    k_axpy_s<T, NSP><<<vgrid(), 256, 0, st>>>(V.r, V.v, V.s, bs.alpha, p1, S->nn, S->nbx, S->tile_rows, ga);
+    DP_LAUNCH_CHECK();
+    pend(2, groups(true) * vbytes() * 3);
+    k_sc_s<<<gcount, kScalThreads, 0, st>>>(bs, p1, nsp, S->nbx);
+    DP_LAUNCH_CHECK();
+    DP_TRY(spmvG<EPI_STORE>(false, V.s, V.u, nullptr, ga));      // u = G s
+    DP_TRY(spmvG<EPI_STORE>(true, V.u, V.z, nullptr, ga));       // z = G^T G s
+    DP_TRY(spmvA<EPI_STORE>(V.z, V.t, nullptr, ga));             // t = A z
+    DP_TRY(spmvG<EPI_DOT2>(false, V.t, nullptr, V.u, ga));       // (Gt.Gs), (Gt.Gt)
+    k_sc_omega<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, nsp, S->nbx);
+    DP_LAUNCH_CHECK();
+    pbegin();
+    k_update_xr<T, NSP><<<vgrid(), 256, 0, st>>>(V.x, V.h, V.z, V.r, V.s, V.t, V.rhat, bs.alpha,
+                                                bs.omega, bs.status, p1, p2, S->nn, S->nbx, S->tile_rows, ga);
+    DP_LAUNCH_CHECK();
+    pend(2, groups(true) * vbytes() * 8);
+    DP_TRY(reset_count());
+    k_sc_beta<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, nsp, S->nbx, maxit);
+    DP_LAUNCH_CHECK();
+    const int ag = S->active_groups;  // groups still active before this update
+    pbegin();
+    k_update_p<T, NSP><<<vgrid(), 256, 0, st>>>(V.p, V.r, V.v, bs.beta, bs.omega, S->nn, S->nbx, S->tile_rows, ga);
+    DP_LAUNCH_CHECK();
+    pend(2, (double)ag * vbytes() * 4);
+    return DOGPUP_OK;
+  }
+
+  // Fused BiCGSTAB.  If b != nullptr the init pass runs from b (x = 0);
+  // otherwise r, rhat, p, x and the state are already initialised (the
+  // refinement residual pass did it).  iters_run = loop iterations issued.
+  int run_fused(Vecs<T> V, const T* b, int maxit, const int* mask, int* iters_run) {
+    const int nsp = NSP;
     if (b) {
-      k_init<T, NSP><<<vgrid(), 256, 0, st>>>(b, V.x, V.r, V.rhat, V.p, p1, p2, S->nn, S->nbx, nullptr);
+      k_init<T, NSP><<<vgrid(), 256, 0, st>>>(b, V.x, V.r, V.rhat, V.p, p1, p2, S->nn, S->nbx, S->tile_rows, nullptr);
       DP_LAUNCH_CHECK();
       DP_TRY(reset_count());
       k_sc_init<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, mask, nsp, S->nbx, S->ncb, S->ncols, g0);
@@ -1037,39 +1107,42 @@ struct Chunk {
     int active = 0;
     DP_TRY(wait_count(&active));
     int it = 0;
-    while (active > 0 && it < maxit) {
-      DP_TRY(spmvG<EPI_STORE>(false, V.p, V.u, nullptr, ga));
-      DP_TRY(spmvG<EPI_STORE>(true, V.u, V.h, nullptr, ga));
-      DP_TRY(spmvA<EPI_STORE_DOT>(V.h, V.v, V.rhat, ga));
-      k_sc_alpha<<<gcount, kScalThreads, 0, st>>>(bs, p1, nsp, S->nbx);
-      DP_LAUNCH_CHECK();
-      pbegin();
-      k_axpy_s<T, NSP><<<vgrid(), 256, 0, st>>>(V.r, V.v, V.s, bs.alpha, p1, S->nn, S->nbx, ga);
-      DP_LAUNCH_CHECK();
-      pend(2, groups(true) * vbytes() * 3);
-      k_sc_s<<<gcount, kScalThreads, 0, st>>>(bs, p1, nsp, S->nbx);
-      DP_LAUNCH_CHECK();
-      DP_TRY(spmvG<EPI_STORE>(false, V.s, V.u, nullptr, ga));     // u = Gs
-      DP_TRY(spmvG<EPI_STORE>(true, V.u, V.z, nullptr, ga));      // z = G^T G s
-      DP_TRY(spmvA<EPI_STORE>(V.z, V.t, nullptr, ga));            // t = A z
-      DP_TRY(spmvG<EPI_DOT2>(false, V.t, nullptr, V.u, ga));      // (Gt.Gs), (Gt.Gt)
-      k_sc_omega<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, nsp, S->nbx);
-      DP_LAUNCH_CHECK();
-      pbegin();
-      k_update_xr<T, NSP><<<vgrid(), 256, 0, st>>>(V.x, V.h, V.z, V.r, V.s, V.t, V.rhat, bs.alpha,
-                                                  bs.omega, bs.status, p1, p2, S->nn, S->nbx, ga);
-      DP_LAUNCH_CHECK();
-      pend(2, groups(true) * vbytes() * 8);
-      DP_TRY(reset_count());
-      k_sc_beta<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, nsp, S->nbx, maxit);
-      DP_LAUNCH_CHECK();
-      const int ag = S->active_groups;  // groups still active before this update
-      pbegin();
-      k_update_p<T, NSP><<<vgrid(), 256, 0, st>>>(V.p, V.r, V.v, bs.beta, bs.omega, S->nn, S->nbx, ga);
-      DP_LAUNCH_CHECK();
-      pend(2, (double)ag * vbytes() * 4);
-      ++it;
-      DP_TRY(wait_count(&active));
+    if (S->profile || !S->use_graph) {
+      while (active > 0 && it < maxit) {
+        DP_TRY(body(V, maxit));
+        ++it;
+        DP_TRY(wait_count(&active));
+      }
+    } else {
+      // Replay a captured iteration K times between convergence polls;
+      // converged groups make every kernel exit early, so replays past
+      // convergence cost only launch latency.
+      const long long key = ((long long)g0 << 32) ^ ((long long)maxit << 2) ^ (sizeof(T) == 4 ? 1 : 2);
+      hipGraphExec_t ex = nullptr;
+      auto found = S->graphs.find(key);
+      if (found != S->graphs.end()) {
+        ex = found->second;
+      } else {
+        DP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = body(V, maxit);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(st, &g);
+        if (rc != DOGPUP_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+        if (ec != hipSuccess) return fail(DOGPUP_ERR_HIP, "graph capture: %s", hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) return fail(DOGPUP_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ei));
+        S->graphs[key] = ex;
+      }
+      const int K = S->graph_k;
+      while (active > 0 && it < maxit) {
+        for (int q = 0; q < K; ++q) DP_HIP(hipGraphLaunch(ex, st));
+        it += K;
+        DP_TRY(wait_count(&active));
+      }
     }
     *iters_run = it;
     return DOGPUP_OK;
@@ -1187,7 +1260,7 @@ static int solve_c128(dogpup_solver* S, hipStream_t st, const double* b, double*
     // true residual for reporting
     k_residual64<NSP><<<dim3(S->nbx, gc), 256, 0, st>>>(
         S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), S->fscale.as<double>(), bc, V.x,
-        nullptr, nullptr, nullptr, nullptr, nullptr, 0, C.p1, C.p2, S->nn, S->ncb, g0, S->nbx);
+        nullptr, nullptr, nullptr, nullptr, nullptr, 0, C.p1, C.p2, S->nn, S->ncb, g0, S->nbx, S->tile_rows);
     DP_LAUNCH_CHECK();
     // relres = |b - Ax| / |b|: reuse k_rf_check's arithmetic through a small pass
     std::vector<double> part((size_t)gc * S->nbx * NSP * 2);
@@ -1247,7 +1320,7 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
       k_residual64<NSP><<<dim3(S->nbx, gc), 256, 0, st>>>(
           S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), S->fscale.as<double>(), bc, xc,
           V.r, V.rhat, V.p, V.x, sweep == 0 ? nullptr : R.status, 1, C.p1, C.p2, S->nn, S->ncb, g0,
-          S->nbx);
+          S->nbx, S->tile_rows);
       DP_LAUNCH_CHECK();
       C.pend(3, (sweep == 0 ? gc : S->active_groups) * e64 * (16 + 16 + 4 * 8) +
                     (double)S->nnzA * 20 + 4.0 * (S->nn + 1));
@@ -1313,7 +1386,7 @@ static int solve_plain(dogpup_solver* S, hipStream_t st, const double* b, double
       return DOGPUP_OK;
     };
     // x = 0, r = b, rhat0 = r, p = r
-    k_init<double, NSP><<<vg, 256, 0, st>>>(bc, V.x, V.r, V.rhat, V.p, C.p1, C.p2, nn, nbx, nullptr);
+    k_init<double, NSP><<<vg, 256, 0, st>>>(bc, V.x, V.r, V.rhat, V.p, C.p1, C.p2, nn, nbx, S->tile_rows, nullptr);
     DP_LAUNCH_CHECK();
     k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 5, C.p1, C.p2, NSP, nbx, S->ncb, S->ncols, g0, dmax);
     DP_LAUNCH_CHECK();
@@ -1322,15 +1395,15 @@ static int solve_plain(dogpup_solver* S, hipStream_t st, const double* b, double
       DP_TRY(C.template spmvG<EPI_STORE>(false, V.p, V.u, nullptr, nullptr));   // h = G p
       DP_TRY(C.template spmvG<EPI_STORE>(true, V.u, V.h, nullptr, nullptr));    // h = G^T h
       DP_TRY(C.template spmvA<EPI_STORE>(V.h, V.v, nullptr, nullptr));          // v = A h
-      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.v, V.rhat, 0, C.p1, nn, nbx, nullptr);
+      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.v, V.rhat, 0, C.p1, nn, nbx, S->tile_rows, nullptr);
       DP_LAUNCH_CHECK();
       k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 0, C.p1, nullptr, NSP, nbx, S->ncb, S->ncols, g0, dmax);
       DP_LAUNCH_CHECK();
-      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.u, V.x, V.h, C.bs.alpha, 0, nn, nbx, nullptr);  // hx = x + h/alph
+      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.u, V.x, V.h, C.bs.alpha, 0, nn, nbx, S->tile_rows, nullptr);  // hx = x + h/alph
       DP_LAUNCH_CHECK();
-      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.s, V.r, V.v, C.bs.alpha, 1, nn, nbx, nullptr);  // s = r - v/alph
+      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.s, V.r, V.v, C.bs.alpha, 1, nn, nbx, S->tile_rows, nullptr);  // s = r - v/alph
       DP_LAUNCH_CHECK();
-      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.s, nullptr, 1, C.p1, nn, nbx, nullptr);
+      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.s, nullptr, 1, C.p1, nn, nbx, S->tile_rows, nullptr);
       DP_LAUNCH_CHECK();
       double err = 0;
       DP_TRY(maxerr(1, C.p1, &err));
@@ -1344,27 +1417,27 @@ static int solve_plain(dogpup_solver* S, hipStream_t st, const double* b, double
       DP_TRY(C.template spmvA<EPI_STORE>(V.z, V.t, nullptr, nullptr));          // t
       // Gtt = G t is materialised, as the reference does
       DP_TRY(C.template spmvG<EPI_STORE>(false, V.t, V.w, nullptr, nullptr));
-      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.w, V.u, 0, C.p1, nn, nbx, nullptr);
+      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.w, V.u, 0, C.p1, nn, nbx, S->tile_rows, nullptr);
       DP_LAUNCH_CHECK();
-      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.w, V.w, 0, C.p2, nn, nbx, nullptr);
+      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.w, V.w, 0, C.p2, nn, nbx, S->tile_rows, nullptr);
       DP_LAUNCH_CHECK();
       k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 2, C.p1, C.p2, NSP, nbx, S->ncb, S->ncols, g0, dmax);
       DP_LAUNCH_CHECK();
-      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.x, V.h, V.z, C.bs.omega, 2, nn, nbx, nullptr);  // x = h + om z
+      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.x, V.h, V.z, C.bs.omega, 2, nn, nbx, S->tile_rows, nullptr);  // x = h + om z
       DP_LAUNCH_CHECK();
-      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.r, V.s, V.t, C.bs.omega, 3, nn, nbx, nullptr);  // r = s - om t
+      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.r, V.s, V.t, C.bs.omega, 3, nn, nbx, S->tile_rows, nullptr);  // r = s - om t
       DP_LAUNCH_CHECK();
-      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.p, V.p, V.v, C.bs.omega, 3, nn, nbx, nullptr);  // p = p - om v
+      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.p, V.p, V.v, C.bs.omega, 3, nn, nbx, S->tile_rows, nullptr);  // p = p - om v
       DP_LAUNCH_CHECK();
-      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.r, nullptr, 1, C.p1, nn, nbx, nullptr);
+      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.r, nullptr, 1, C.p1, nn, nbx, S->tile_rows, nullptr);
       DP_LAUNCH_CHECK();
       DP_TRY(maxerr(3, C.p1, &err));
       if (err < tol) break;
-      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.rhat, V.r, 0, C.p1, nn, nbx, nullptr);
+      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.rhat, V.r, 0, C.p1, nn, nbx, S->tile_rows, nullptr);
       DP_LAUNCH_CHECK();
       k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 4, C.p1, nullptr, NSP, nbx, S->ncb, S->ncols, g0, dmax);
       DP_LAUNCH_CHECK();
-      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.p, V.r, V.p, C.bs.beta, 2, nn, nbx, nullptr);  // p = r + beta p
+      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.p, V.r, V.p, C.bs.beta, 2, nn, nbx, S->tile_rows, nullptr);  // p = r + beta p
       DP_LAUNCH_CHECK();
     }
     DP_TRY(permute_block(S, st, V.x, x + goff, gc, 1));
@@ -1513,8 +1586,12 @@ int dogpup_solver_create(dogpup_solver_t* out, int nn, int nfreq, int ncols, int
   S->ngroups = nfreq * S->ncb;
   S->nnzA = nnzA; S->nnzG = nnzG;
   S->rA0 = d_rptrA; S->cA0 = d_cptrA; S->rG0 = d_rptrG; S->cG0 = d_cptrG;
-  // gridDim.x of the row kernels is a multiple of 8 (one contiguous tile range per XCD)
-  S->nbx = (int)(div_up(div_up(nn, kTile), 8) * 8);
+  // Row tile of the vector/SpMV kernels (a multiple of 64 rows); gridDim.x
+  // is a multiple of 8 (one contiguous tile range per XCD).
+  if (const char* e = getenv("DOGPUP_TILE")) S->tile_rows = atoi(e);
+  if (S->tile_rows < 64 || S->tile_rows % 64) S->tile_rows = 512;
+  if (const char* e = getenv("DOGPUP_SPMV_LDS_PAD")) S->lds_pad = atoi(e);
+  S->nbx = (int)(div_up(div_up(nn, S->tile_rows), 8) * 8);
   DP_ARG(S->ngroups <= 65535, "too many groups (%d)", S->ngroups);
 
   // solver node order and the permuted patterns (host-side, once per handle)
@@ -1600,6 +1677,11 @@ int dogpup_solver_create(dogpup_solver_t* out, int nn, int nfreq, int ncols, int
   DP_TRY(S->fscale.ensure((size_t)nfreq * 8));
   DP_HIP(hipHostMalloc((void**)&S->h_counter, 16, hipHostMallocDefault));
   DP_HIP(hipHostMalloc((void**)&S->h_max, 16, hipHostMallocDefault));
+  DP_HIP(hipStreamCreateWithFlags(&S->own, hipStreamNonBlocking));
+  DP_HIP(hipEventCreateWithFlags(&S->ev_in, hipEventDisableTiming));
+  DP_HIP(hipEventCreateWithFlags(&S->ev_out, hipEventDisableTiming));
+  if (const char* e = getenv("DOGPUP_GRAPH")) S->use_graph = atoi(e) != 0;
+  if (const char* e = getenv("DOGPUP_GRAPH_K")) S->graph_k = std::max(1, atoi(e));
   *out = S.release();
   return DOGPUP_OK;
 }
@@ -1616,6 +1698,7 @@ int dogpup_solver_set_matrix(dogpup_solver_t S, const double* d_valA_re, const d
                              const double* d_fscale, void* stream) {
   DP_ARG(S && d_valA_re && d_valA_im && d_fscale, "NULL argument");
   hipStream_t st = as_stream(stream);
+  S->clear_graphs();
   const size_t nA = S->nnzA;
   DP_TRY(S->a64re.ensure(nA * 8));
   DP_TRY(S->a64im.ensure(nA * 8));
@@ -1660,7 +1743,11 @@ int dogpup_solver_solve(dogpup_solver_t S, const double* d_b, double* d_x, doubl
   DP_ARG(S && d_b && d_x, "NULL argument");
   DP_ARG(S->have_matrix, "dogpup_solver_set_matrix has not been called");
   DP_ARG(tol > 0 && maxit > 0, "tol and maxit must be positive");
-  hipStream_t st = as_stream(stream);
+  // run on the handle's own stream (capturable), ordered after the caller's
+  hipStream_t caller = as_stream(stream);
+  DP_HIP(hipEventRecord(S->ev_in, caller));
+  DP_HIP(hipStreamWaitEvent(S->own, S->ev_in, 0));
+  hipStream_t st = S->own;
   for (double& v : S->stats) v = 0.0;
   int rc;
   switch (S->mode) {
@@ -1674,6 +1761,8 @@ int dogpup_solver_solve(dogpup_solver_t S, const double* d_b, double* d_x, doubl
       rc = by_nsp<SolvePlain>(S->nsp, S, st, d_b, d_x, tol, maxit, h_iters, h_relres, h_status);
   }
   if (rc != DOGPUP_OK) return rc;
+  DP_HIP(hipEventRecord(S->ev_out, st));
+  DP_HIP(hipStreamWaitEvent(caller, S->ev_out, 0));
   DP_HIP(hipStreamSynchronize(st));
   for (int c = 0; c < 4; ++c) S->prof_ms[c] = S->prof_bytes[c] = S->prof_n[c] = 0.0;
   for (auto& r : S->recs) {

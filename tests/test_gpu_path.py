@@ -248,3 +248,30 @@ def test_autograd_vjp_vs_fd_and_reference_jacobian(setup):
     Jref = ojac.j_tg(phi, phiA, setup["link"], opt.s_fpsf, 1.0, gfc, f, t.size)   # NM x NG x NV, dxyz = 1
     gref = np.einsum("mg,mgv->v", ybar.cpu().numpy(), Jref)
     assert rel(g_nodal, gref) < 1e-8
+
+
+def test_graph_replay_matches_eager(setup, monkeypatch):
+    """The captured-iteration path (HIP graph replayed between convergence
+    polls, several solves on one handle) gives exactly the eager result."""
+    from dogpup_amd import mesh as dm
+    D = setup["D"]
+    res = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("DOGPUP_GRAPH", flag)
+        S = dm.FluenceSolver(D, 4, dm.MODES["mixed"])
+        nodes, w = D._bary_dev("s")
+        spec = torch.as_tensor(D.optode.s_fpsf, dtype=torch.complex128, device="cuda")
+        b = S.empty_block()
+        from dogpup_amd import _lib
+        _lib.call("dogpup_point_sources", _lib.ptr(torch.view_as_real(b)), D.nn, S.nf, 4, S.nsp,
+                  _lib.ptr(nodes), _lib.ptr(w), _lib.ptr(torch.view_as_real(spec)), _lib.stream_ptr())
+        outs = []
+        for rep in range(3):
+            f = S.solve(b, tol=1e-10, maxit=500)
+            torch.cuda.synchronize()
+            outs.append(f.data.clone())
+            assert (f.status == 0).all()
+        for o in outs[1:]:
+            assert torch.equal(o, outs[0])
+        res[flag] = outs[0]
+    assert torch.equal(res["0"], res["1"])
