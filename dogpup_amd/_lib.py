@@ -50,7 +50,7 @@ SIGNATURES = {
     "dogpup_jacobian_tg": [vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32, vp],
     "dogpup_meas": [vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp],
     "dogpup_jacobian_vjp": [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32, i32, f64, vp],
-    "dogpup_gemm_f32": [vp, vp, vp, i32, i32, i32, C.c_longlong, C.c_longlong, C.c_longlong, C.c_float, i32, i32, i32, vp],
+    "dogpup_gemm_f32": [vp, vp, vp, i32, i32, i32, C.c_longlong, C.c_longlong, C.c_longlong, C.c_float, i32, i32, i32, i32, vp],
     "dogpup_col_sumsq": [vp, i32, C.c_longlong, vp, vp],
     "dogpup_gemv_t": [vp, vp, i32, C.c_longlong, vp, vp],
     "dogpup_chol_factor": [vp, i32, f64, vp, vp],

@@ -1,13 +1,17 @@
 // Dense algebra of the LM step (reconstruction/tik_inv.m, reshape_J.m):
-// normal matrices on FP32 MFMA with FP32 accumulation, and an FP64
-// Cholesky factorisation and triangular solves for the damped system.
+// normal matrices on MFMA (FP32 with FP32 accumulation, or exact FP64
+// products of FP32 operands with FP64 accumulation), and an FP64 Cholesky
+// factorisation and triangular solves for the damped system.
 //
 // GEMM tiles: a 256-thread block (4 wavefronts) computes a 64 x 64 tile of
 // C; each wavefront owns a 32 x 32 quarter as 2 x 2 tiles of
-// v_mfma_f32_16x16x4_f32.  Operand tiles of 64 x 16 are staged through LDS
-// per K step.  MFMA 16x16x4 lane layout: lane l supplies A[l%16][l/16] and
-// B[l/16][l%16] (K index l/16) and receives D[4*(l/16)+j][l%16] in its j-th
-// accumulator register.
+// v_mfma_f32_16x16x4_f32 (or v_mfma_f64_16x16x4_f64).  Operand tiles of
+// 64 x 16 are staged through LDS per K step.  MFMA 16x16x4 lane layout (both
+// precisions): lane l supplies A[l%16][l/16] and B[l/16][l%16] (K index
+// l/16); its accumulator register r holds D[4*(l/16)+r][l%16] for FP32 and
+// D[l/16+4*r][l%16] for FP64 (measured on gfx950 by an exact-integer test).
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace dogpup {
@@ -19,32 +23,38 @@ constexpr int GK = 16;   // K step
 
 // C[m][n] = alpha * sum_k A[m*lda + k] * B[n*ldb + k]   (TRANS_B = true, "NT")
 // C[m][n] = alpha * sum_k A[m*lda + k] * B[k*ldb + n]   (TRANS_B = false, "NN")
-// Output FP64 (OUT_D) or FP32.
-template <bool TRANS_B, bool OUT_D>
+// Inputs FP32.  F64: operands converted exactly to FP64 in LDS and
+// multiplied on v_mfma_f64_16x16x4f64 with FP64 accumulation (for damped
+// normal equations whose shift is below FP32 rounding); otherwise
+// v_mfma_f32_16x16x4f32 with FP32 accumulation.  Output FP64 (OUT_D) or FP32.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <bool TRANS_B, bool OUT_D, bool F64>
 __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, const float* __restrict__ B,
                                                   void* __restrict__ Cv, int M, int N, int K, long long lda,
                                                   long long ldb, long long ldc, float alpha,
                                                   int lower_only) {
-  __shared__ float As[GT][GK + 1];
-  __shared__ float Bs[GT][GK + 1];
+  using E = typename std::conditional<F64, double, float>::type;
+  using V4 = typename std::conditional<F64, f64x4, f32x4>::type;
+  __shared__ E As[GT][GK + 1];
+  __shared__ E Bs[GT][GK + 1];
   const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
   if (lower_only && n0 > m0 + GT - 1) return;  // tile entirely above the diagonal
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
-  f32x4 acc[2][2];
+  V4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < 2; ++j) acc[i][j] = V4{0, 0, 0, 0};
   for (int k0 = 0; k0 < K; k0 += GK) {
-    // stage A tile 64 x 16: thread t loads row t/4, k-quad (t%4)*4
     {
       const int r = t >> 2, kq = (t & 3) * 4;
       const int gm = m0 + r;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int gk = k0 + kq + q;
-        As[r][kq + q] = (gm < M && gk < K) ? A[(long long)gm * lda + gk] : 0.f;
+        As[r][kq + q] = (gm < M && gk < K) ? (E)A[(long long)gm * lda + gk] : (E)0;
       }
     }
     if (TRANS_B) {
@@ -53,16 +63,15 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, c
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int gk = k0 + kq + q;
-        Bs[r][kq + q] = (gn < N && gk < K) ? B[(long long)gn * ldb + gk] : 0.f;
+        Bs[r][kq + q] = (gn < N && gk < K) ? (E)B[(long long)gn * ldb + gk] : (E)0;
       }
     } else {
-      // B is K x N row-major: thread t loads k = t/16, n = (t%16)*4 .. +3
       const int kk = t >> 4, nq = (t & 15) * 4;
       const int gk = k0 + kk;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int gn = n0 + nq + q;
-        Bs[nq + q][kk] = (gn < N && gk < K) ? B[(long long)gk * ldb + gn] : 0.f;
+        Bs[nq + q][kk] = (gn < N && gk < K) ? (E)B[(long long)gk * ldb + gn] : (E)0;
       }
     }
     __syncthreads();
@@ -71,11 +80,12 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, c
       const int ka = kk + (lane >> 4);
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        const float a = As[wm + 16 * i + (lane & 15)][ka];
+        const E a = As[wm + 16 * i + (lane & 15)][ka];
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          const float b = Bs[wn + 16 * j + (lane & 15)][ka];
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i][j], 0, 0, 0);
+          const E b = Bs[wn + 16 * j + (lane & 15)][ka];
+          if constexpr (F64) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i][j], 0, 0, 0);
+          else acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i][j], 0, 0, 0);
         }
       }
     }
@@ -87,12 +97,14 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, c
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gm = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        // accumulator row: FP32 MFMA 4*(lane/16) + r, FP64 MFMA lane/16 + 4*r
+        const int rr = F64 ? ((lane >> 4) + 4 * r) : (4 * (lane >> 4) + r);
+        const int gm = m0 + wm + 16 * i + rr;
         const int gn = n0 + wn + 16 * j + (lane & 15);
         if (gm < M && gn < N && (!lower_only || gn <= gm)) {
-          const float v = alpha * acc[i][j][r];
+          const E v = (E)alpha * acc[i][j][r];
           if (OUT_D) reinterpret_cast<double*>(Cv)[(long long)gm * ldc + gn] = (double)v;
-          else reinterpret_cast<float*>(Cv)[(long long)gm * ldc + gn] = v;
+          else reinterpret_cast<float*>(Cv)[(long long)gm * ldc + gn] = (float)v;
         }
       }
 }
@@ -275,19 +287,22 @@ extern "C" {
 
 int dogpup_gemm_f32(const float* d_A, const float* d_B, void* d_C, int M, int N, int K, long long lda,
                     long long ldb, long long ldc, float alpha, int trans_b, int out_f64, int lower_only,
-                    void* stream) {
+                    int compute_f64, void* stream) {
   DP_ARG(d_A && d_B && d_C && M >= 0 && N >= 0 && K >= 0, "bad arguments");
   if (M == 0 || N == 0) return DOGPUP_OK;
   dim3 grid(div_up(N, GT), div_up(M, GT));
   DP_ARG(grid.y <= 65535, "M too large");
   hipStream_t st = as_stream(stream);
-  if (trans_b) {
-    if (out_f64) k_gemm_f32<true, true><<<grid, 256, 0, st>>>(d_A, d_B, d_C, M, N, K, lda, ldb, ldc, alpha, lower_only);
-    else k_gemm_f32<true, false><<<grid, 256, 0, st>>>(d_A, d_B, d_C, M, N, K, lda, ldb, ldc, alpha, lower_only);
+#define DP_GEMM(TB, OD, F6) \
+  k_gemm_f32<TB, OD, F6><<<grid, 256, 0, st>>>(d_A, d_B, d_C, M, N, K, lda, ldb, ldc, alpha, lower_only)
+  if (compute_f64) {
+    if (trans_b) { if (out_f64) DP_GEMM(true, true, true); else DP_GEMM(true, false, true); }
+    else { if (out_f64) DP_GEMM(false, true, true); else DP_GEMM(false, false, true); }
   } else {
-    if (out_f64) k_gemm_f32<false, true><<<grid, 256, 0, st>>>(d_A, d_B, d_C, M, N, K, lda, ldb, ldc, alpha, lower_only);
-    else k_gemm_f32<false, false><<<grid, 256, 0, st>>>(d_A, d_B, d_C, M, N, K, lda, ldb, ldc, alpha, lower_only);
+    if (trans_b) { if (out_f64) DP_GEMM(true, true, false); else DP_GEMM(true, false, false); }
+    else { if (out_f64) DP_GEMM(false, true, false); else DP_GEMM(false, false, false); }
   }
+#undef DP_GEMM
   DP_LAUNCH_CHECK();
   return DOGPUP_OK;
 }

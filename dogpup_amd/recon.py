@@ -2,7 +2,9 @@
 
 ``tik_inv`` and ``reshape_J`` mirror ``DOGPUP/reconstruction/tik_inv.m`` and
 ``reshape_J.m``.  J is rows x NV (float32 on the device); the normal matrix
-J J^T (rows x rows) is formed on FP32 MFMA with FP32 accumulation and
+J J^T (rows x rows) is formed on MFMA (by default exact FP64 products of the
+FP32 entries with FP64 accumulation, because the reference's damping of
+1e-6 * max(diag) is below FP32 rounding; FP32 MFMA on request) and
 factorised in FP64.  tik_inv is solved in the data space,
     (J^T J + mu I)^-1 J^T y = J^T (J J^T + mu I)^-1 y,   mu = lam * max_j sum_i J_ij^2,
 which is the same update without the NV x NV matrix (580 GB at 269k voxels).
@@ -27,27 +29,28 @@ def _f64(a):
     return t.to(DEV, torch.float64).contiguous()
 
 
-def gemm_nt(A, B, out_f64=True, lower_only=False, alpha=1.0):
-    """C = alpha * A B^T (FP32 MFMA, FP32 accumulation)."""
+def gemm_nt(A, B, out_f64=True, lower_only=False, alpha=1.0, exact=True):
+    """C = alpha * A B^T from FP32 operands; exact=True: FP64 MFMA with FP64
+    accumulation, exact=False: FP32 MFMA with FP32 accumulation."""
     A, B = _f32(A), _f32(B)
     M, K = A.shape
     N = B.shape[0]
     assert B.shape[1] == K
     C = torch.zeros((M, N), dtype=torch.float64 if out_f64 else torch.float32, device=DEV)
     _lib.call("dogpup_gemm_f32", _lib.ptr(A), _lib.ptr(B), _lib.ptr(C), M, N, K, K, K, N, float(alpha), 1,
-              int(out_f64), int(lower_only), _lib.stream_ptr())
+              int(out_f64), int(lower_only), int(exact), _lib.stream_ptr())
     return C
 
 
-def gemm_nn(A, B, alpha=1.0):
-    """C = alpha * A B (FP32 MFMA, FP32 output)."""
+def gemm_nn(A, B, alpha=1.0, exact=False):
+    """C = alpha * A B (FP32 output; FP32 MFMA, or FP64 MFMA when exact)."""
     A, B = _f32(A), _f32(B)
     M, K = A.shape
     N = B.shape[1]
     assert B.shape[0] == K
     C = torch.empty((M, N), dtype=torch.float32, device=DEV)
     _lib.call("dogpup_gemm_f32", _lib.ptr(A), _lib.ptr(B), _lib.ptr(C), M, N, K, K, N, N, float(alpha), 0, 0,
-              0, _lib.stream_ptr())
+              0, int(exact), _lib.stream_ptr())
     return C
 
 

@@ -253,12 +253,14 @@ int dogpup_jacobian_vjp(const double* d_field, const double* d_D, const int* d_l
  * FP64 Cholesky of the damped system.  Row-major, device pointers.
  * ------------------------------------------------------------------------- */
 
-/* C = alpha * A * op(B): trans_b = 1 -> op(B) = B^T with B N x K (ldb);
- * trans_b = 0 -> B is K x N.  A is M x K (lda); C is M x N (ldc), FP64 when
- * out_f64 else FP32; lower_only writes only C[i][j], j <= i (for J J^T). */
+/* C = alpha * A * op(B) from FP32 operands: trans_b = 1 -> op(B) = B^T with
+ * B N x K (ldb); trans_b = 0 -> B is K x N.  A is M x K (lda); C is M x N
+ * (ldc), FP64 when out_f64 else FP32; lower_only writes only C[i][j], j <= i
+ * (for J J^T).  compute_f64 = 0: FP32 MFMA, FP32 accumulation; 1: exact FP64
+ * products on FP64 MFMA with FP64 accumulation. */
 int dogpup_gemm_f32(const float* d_A, const float* d_B, void* d_C, int M, int N, int K, long long lda,
                     long long ldb, long long ldc, float alpha, int trans_b, int out_f64, int lower_only,
-                    void* stream);
+                    int compute_f64, void* stream);
 
 /* out[j] = sum_i A[i][j]^2 (FP64), A rows x cols FP32 (tik_inv's alpha). */
 int dogpup_col_sumsq(const float* d_A, int rows, long long cols, double* d_out, void* stream);

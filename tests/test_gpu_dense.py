@@ -23,8 +23,9 @@ def test_mfma_gemm_layout_exact():
     for M, N, K in [(16, 16, 4), (70, 33, 29), (130, 65, 100)]:
         A = rng.integers(-4, 5, size=(M, K)).astype(np.float32)
         B = rng.integers(-4, 5, size=(N, K)).astype(np.float32)
-        C = recon.gemm_nt(A, B).cpu().numpy()
-        assert np.array_equal(C, (A.astype(np.float64) @ B.T.astype(np.float64)))
+        for exact in (False, True):
+            C = recon.gemm_nt(A, B, exact=exact).cpu().numpy()
+            assert np.array_equal(C, (A.astype(np.float64) @ B.T.astype(np.float64))), exact
         Bn = rng.integers(-4, 5, size=(K, N)).astype(np.float32)
         C2 = recon.gemm_nn(A, Bn).cpu().numpy()
         assert np.array_equal(C2, (A @ Bn))
