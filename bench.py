@@ -62,24 +62,24 @@ def build_workload(small=False):
     return D, opt, t
 
 
-def timed_solves(D, which, mode, tol, steps, warmup, profile=False):
+def timed_solves(D, which, mode, tol, steps, warmup, profile=False, method="bicgstab"):
     """Run warmup + timed steps of (FSAI setup + solve); returns (seconds per
     step, field of the last step, per-class profile or None)."""
     import torch
     from dogpup_amd import _lib
     ncols = sum(len((D.optode.s_bary if w == "s" else D.optode.d_bary)[0]) for w in which)
-    S = D.solver(ncols, mode)
+    S = D.solver(ncols, mode, method)
     phi = None
     for _ in range(warmup):
         S._version = None
-        phi = D.solve_point_sources(which, mode=mode, tol=tol)
+        phi = D.solve_point_sources(which, mode=mode, tol=tol, method=method)
     if profile:
         _lib.call("dogpup_solver_set_profile", S.h, 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         S._version = None                       # FSAI setup is part of every step
-        phi = D.solve_point_sources(which, mode=mode, tol=tol)
+        phi = D.solve_point_sources(which, mode=mode, tol=tol, method=method)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     prof = None
@@ -211,6 +211,9 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-jacobian", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--method", default="bicgstab", choices=["bicgstab", "cocg"],
+                    help="Krylov method of the headline mixed solve")
+    ap.add_argument("--no-alt", action="store_true", help="skip the mixed solve with the other method")
     ap.add_argument("--tol", type=float, default=1e-8,
                     help="relative FP64 residual of the mixed solve (chosen to meet 1e-6 fluence error)")
     a = ap.parse_args(argv)
@@ -229,7 +232,8 @@ def main(argv=None):
 
     # headline: mixed precision, timed with per-launch events
     log("mixed solve (timed)")
-    dt_mixed, phi_mixed, prof = timed_solves(D, "s", "mixed", a.tol, a.steps, a.warmup, profile=True)
+    dt_mixed, phi_mixed, prof = timed_solves(D, "s", "mixed", a.tol, a.steps, a.warmup, profile=True,
+                                             method=a.method)
     value = nrhs / dt_mixed
     extra["mixed_s_per_solve"] = dt_mixed
     extra["mixed_max_iters"] = int(phi_mixed.iters.max())
@@ -238,10 +242,29 @@ def main(argv=None):
     extra["mixed_relres_max"] = float(phi_mixed.relres.max())
     x_mixed = phi_mixed.data.clone()
     release(D)
+    if not a.no_alt:
+        alt = "cocg" if a.method == "bicgstab" else "bicgstab"
+        log(f"mixed solve, {alt}")
+        dta, pha, _ = timed_solves(D, "s", "mixed", a.tol, 1, 1, method=alt)
+        err = rel_err_per_rhs(pha.data.permute(0, 2, 1).reshape(-1, D.nn),
+                              x_mixed.permute(0, 2, 1).reshape(-1, D.nn))
+        extra[f"mixed_{alt}_s_per_solve"] = dta
+        extra[f"mixed_{alt}_max_iters"] = int(pha.iters.max())
+        extra[f"mixed_{alt}_iters_per_freq"] = pha.iters.max(axis=1).tolist()
+        extra[f"mixed_{alt}_status"] = np.unique(pha.status).tolist()
+        extra[f"mixed_{alt}_vs_{a.method}_rel_diff_max"] = float(err.max())
+        del pha
+        release(D)
 
-    names = ["G/G^T SpMV", "A SpMV", "BiCGSTAB vector passes", "FP64 refinement passes"]
-    pmc_kernel = {0: "dogpup::k_spmv<float, 32, 1, 0>", 1: "dogpup::k_spmv<float, 32, 0, 0>",
-                  2: "dogpup::k_update_xr<float, 32>"}
+    names = ["G/G^T SpMV", "A SpMV", "Krylov vector passes", "FP64 refinement passes"]
+    # kernel whose PMC traffic ratio stands for each class (the full-activity
+    # launch of the class's most frequent kernel)
+    if a.method == "bicgstab":
+        pmc_kernel = {0: "dogpup::k_spmv<float, 32, 1, 0>", 1: "dogpup::k_spmv<float, 32, 0, 0>",
+                      2: "dogpup::k_update_xr<float, 32>"}
+    else:
+        pmc_kernel = {0: "dogpup::k_spmv<float, 32, 1, 4>", 1: "dogpup::k_spmv<float, 32, 0, 1>",
+                      2: "dogpup::k_cg_update<float, 32>"}
     cls = int(np.argmax(prof[:, 0]))
     ms, byts, nl = prof[cls]
     achieved = byts / (ms * 1e-3) / 1e9
@@ -253,7 +276,7 @@ def main(argv=None):
     extra["copy_GBps"] = copy_gbs
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_bicgstab_c64_slab269k_2048rhs.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{a.method}_c64_slab269k_2048rhs.json")
     if not a.small and os.path.exists(tfile) and cls in pmc_kernel:
         with open(tfile) as fh:
             tr = json.load(fh)["kernels"].get(pmc_kernel[cls])
@@ -289,7 +312,7 @@ def main(argv=None):
         gfc = boxcar_gates_fc(opt, t, 8, 500e-12, 200e-12)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        J, info = jacobian_tg(D, gfc, mode="mixed", tol=a.tol)
+        J, info = jacobian_tg(D, gfc, mode="mixed", tol=a.tol, method=a.method)
         torch.cuda.synchronize()
         extra["jacobian_s"] = time.perf_counter() - t0
         extra["jacobian_solve_s"] = info["solve_s"]
@@ -310,9 +333,10 @@ def main(argv=None):
         "ms_per_step": dt_mixed * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64+f64-refine", "data": "synthetic",
         "config": {"workload": "slab 80x80x40 mm, 1 mm Kuhn grid (269,001 nodes), 32 sources x 64 "
-                               "frequencies = 2048 RHS, FSAI-BiCGSTAB to a 1e-8 FP64 relative residual"
+                               f"frequencies = 2048 RHS, FSAI-preconditioned {a.method.upper()} in complex64 "
+                               f"with FP64 refinement to a {a.tol:g} FP64 relative residual"
                    if not a.small else "slab 60x60x30 mm 2 mm grid, 16 sources x 8 frequencies",
-                   "nodes": D.nn, "rhs": nrhs, "tol": a.tol},
+                   "nodes": D.nn, "rhs": nrhs, "tol": a.tol, "method": a.method},
         "roofline": {"bound": "hbm", "kernel": names[cls], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,

@@ -16,6 +16,7 @@ OK = 0
 ERR_ARG, ERR_HIP, ERR_UNSUPPORTED, ERR_NOMEM = -1, -2, -3, -4
 RHS_CONVERGED, RHS_MAXIT, RHS_BREAKDOWN, RHS_ZERO = 0, 1, 2, 3
 MODE_C128, MODE_C64_REFINE, MODE_C128_PLAIN = 0, 1, 2
+METHOD_BICGSTAB, METHOD_COCG = 0, 1
 
 _lock = threading.Lock()
 _lib = None
@@ -56,6 +57,7 @@ SIGNATURES = {
     "dogpup_chol_factor": [vp, i32, f64, vp, vp],
     "dogpup_chol_solve": [vp, vp, i32, i32, vp],
     "dogpup_solver_set_profile": [vp, i32],
+    "dogpup_solver_set_method": [vp, i32],
     "dogpup_solver_profile": [vp, vp, i32],
 }
 

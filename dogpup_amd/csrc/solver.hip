@@ -30,7 +30,11 @@
 namespace dogpup {
 
 enum { MAT_A = 0, MAT_G = 1 };
-enum { EPI_STORE = 0, EPI_STORE_DOT = 1, EPI_DOT2 = 2 };
+// SpMV epilogues: EPI_STORE y; EPI_STORE_DOT y and w.y; EPI_DOT2 only w.y
+// and y.y (y not stored); EPI_STORE_SELF y and y.y; EPI_AXPBY y <- y_new +
+// coef*y_old (in place, one complex coefficient per column).  All dots are
+// unconjugated.
+enum { EPI_STORE = 0, EPI_STORE_DOT = 1, EPI_DOT2 = 2, EPI_STORE_SELF = 3, EPI_AXPBY = 4 };
 enum { ST_ACTIVE = -1, ST_SKIP = -2 };
 
 // Row tiles are assigned so that each XCD sweeps one contiguous range of
@@ -51,6 +55,16 @@ __device__ inline int xcd_tile(int bx) { return (bx & 7) * (gridDim.x >> 3) + (b
 // per wavefront from global memory costs as many address cycles as the
 // gather itself).  A chunk with more than kChunkCap nonzeros falls back to
 // reading the CSR arrays directly.
+// Per-lane column coefficients (complex double per RHS) as T.
+template <typename T, int CPL>
+__device__ inline void load_coef(const double* __restrict__ c, long long rhs0, T re[CPL], T im[CPL]) {
+#pragma unroll
+  for (int e = 0; e < CPL; ++e) {
+    re[e] = (T)c[2 * (rhs0 + e)];
+    im[e] = (T)c[2 * (rhs0 + e) + 1];
+  }
+}
+
 constexpr int kChunkRows = 64;
 constexpr int kChunkCap = 2048;
 
@@ -58,8 +72,8 @@ template <typename T, int NSP, int MAT, int EPI>
 __global__ __launch_bounds__(256) void k_spmv(
     const int* __restrict__ rptr, const int* __restrict__ cptr, const T* __restrict__ v0,
     const T* __restrict__ v1, long long nnz, const double* __restrict__ fscale,
-    const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ w, double* __restrict__ part1,
-    double* __restrict__ part2, int nn, int ncb, int g0, int nbx, int tile_rows,
+    const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ w, const double* __restrict__ coef,
+    double* __restrict__ part1, double* __restrict__ part2, int nn, int ncb, int g0, int nbx, int tile_rows,
     const int* __restrict__ grp_active) {
   using G = Geo<T, NSP>;
   constexpr int CPL = G::CPL;
@@ -82,6 +96,8 @@ __global__ __launch_bounds__(256) void k_spmv(
   Acc<CPL> d1, d2;
   d1.zero();
   d2.zero();
+  T cr[CPL], ci[CPL];
+  if (EPI == EPI_AXPBY) load_coef<T, CPL>(coef, (long long)g * NSP + c0, cr, ci);
   const int tile0 = xcd_tile(bx) * tile_rows;
   const int tile1 = min(tile0 + tile_rows, nn);
   for (int r0 = tile0; r0 < tile1; r0 += kChunkRows) {
@@ -163,8 +179,24 @@ __global__ __launch_bounds__(256) void k_spmv(
         }
       }
       const size_t off = ((gbase + row) * NSP + c0) * 2;
+      if (EPI == EPI_AXPBY) {
+        T pr[CPL], pi[CPL];
+        Pack<T>::load(y + off, pr, pi);
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) {
+          ar[e] += cr[e] * pr[e] - ci[e] * pi[e];
+          ai[e] += cr[e] * pi[e] + ci[e] * pr[e];
+        }
+      }
       if (EPI != EPI_DOT2) Pack<T>::store(y + off, ar, ai);
-      if (EPI != EPI_STORE) {
+      if (EPI == EPI_STORE_SELF) {
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) {
+          d1.re[e] += (double)ar[e] * ar[e] - (double)ai[e] * ai[e];
+          d1.im[e] += 2.0 * (double)ar[e] * ai[e];
+        }
+      }
+      if (EPI == EPI_STORE_DOT || EPI == EPI_DOT2) {
         T wr[CPL], wi[CPL];
         Pack<T>::load(w + off, wr, wi);
 #pragma unroll
@@ -179,22 +211,12 @@ __global__ __launch_bounds__(256) void k_spmv(
       }
     }
   }
-  if (EPI != EPI_STORE) {
+  if (EPI != EPI_STORE && EPI != EPI_AXPBY) {
     block_reduce_store<T, NSP>(d1, sh, part1, (long long)g * nbx + bx);
     if (EPI == EPI_DOT2) {
       __syncthreads();
       block_reduce_store<T, NSP>(d2, sh, part2, (long long)g * nbx + bx);
     }
-  }
-}
-
-// Per-lane column coefficients (complex double per RHS) as T.
-template <typename T, int CPL>
-__device__ inline void load_coef(const double* __restrict__ c, long long rhs0, T re[CPL], T im[CPL]) {
-#pragma unroll
-  for (int e = 0; e < CPL; ++e) {
-    re[e] = (T)c[2 * (rhs0 + e)];
-    im[e] = (T)c[2 * (rhs0 + e) + 1];
   }
 }
 
@@ -317,6 +339,46 @@ __global__ __launch_bounds__(256) void k_update_p(T* __restrict__ p, const T* __
   Pack<T>::store(p + off, pr, pi);
   ROW_LOOP_END
   (void)bx;
+}
+
+// COCG step: x += alpha p (active columns only), r -= alpha q;
+// part2 = sum |r|^2
+template <typename T, int NSP>
+__global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, const T* __restrict__ p,
+                                                   T* __restrict__ r, const T* __restrict__ q,
+                                                   const double* __restrict__ alpha,
+                                                   const int* __restrict__ status, double* __restrict__ part2,
+                                                   int nn, int nbx, int tile_rows,
+                                                   const int* __restrict__ grp_active) {
+  Acc<Geo<T, NSP>::CPL> d2;
+  d2.zero();
+  T alr[Geo<T, NSP>::CPL], ali[Geo<T, NSP>::CPL];
+  bool act[Geo<T, NSP>::CPL];
+  ROW_LOOP_BEGIN
+  if (st == 0) {
+    load_coef<T, CPL>(alpha, rhs0, alr, ali);
+#pragma unroll
+    for (int e = 0; e < CPL; ++e) act[e] = status[rhs0 + e] == ST_ACTIVE;
+  }
+  T xr[CPL], xi[CPL], pr[CPL], pi[CPL], rr[CPL], ri[CPL], qr[CPL], qi[CPL];
+  Pack<T>::load(x + off, xr, xi);
+  Pack<T>::load(p + off, pr, pi);
+  Pack<T>::load(r + off, rr, ri);
+  Pack<T>::load(q + off, qr, qi);
+#pragma unroll
+  for (int e = 0; e < CPL; ++e) {
+    if (act[e]) {
+      xr[e] += alr[e] * pr[e] - ali[e] * pi[e];
+      xi[e] += alr[e] * pi[e] + ali[e] * pr[e];
+    }
+    rr[e] -= alr[e] * qr[e] - ali[e] * qi[e];
+    ri[e] -= alr[e] * qi[e] + ali[e] * qr[e];
+    d2.re[e] += (double)rr[e] * rr[e] + (double)ri[e] * ri[e];
+  }
+  Pack<T>::store(x + off, xr, xi);
+  Pack<T>::store(r + off, rr, ri);
+  ROW_LOOP_END
+  block_reduce_store<T, NSP>(d2, sh, part2, (long long)g * nbx + bx);
 }
 
 // r = b, rhat = b, p = b, x = 0; part1 = sum |b|^2, part2 = sum b b
@@ -696,6 +758,96 @@ __global__ __launch_bounds__(kScalThreads) void k_sc_beta(BState s, const double
   group_flag(s, g, active);
 }
 
+// ---- COCG scalars (conjugate orthogonal CG: CG with the unconjugated
+// bilinear form, for complex-symmetric A and the symmetric M^-1 = G^T G).
+// rho = (G r).(G r) = r.M^-1 r, alpha = rho / (p.A p), beta = rho_new / rho.
+// A column that is not active gets alpha = beta = 0, so its vectors stay
+// bounded while the rest of its group iterates.
+
+// After the start pass u = G r (part1 = u.u): rho = u.u, beta = 0.
+__global__ __launch_bounds__(kScalThreads) void k_cg_start(BState s, const double* __restrict__ part1,
+                                                           int nsp, int nbx) {
+  const int g = blockIdx.x, c = threadIdx.x;
+  double ur = 0, ui = 0;
+  block_sum_part(part1, g, nsp, nbx, ur, ui);
+  bool active = false;
+  if (c < nsp) {
+    const long long i = (long long)g * nsp + c;
+    if (s.status[i] == ST_ACTIVE) {
+      if ((ur == 0.0 && ui == 0.0) || !isfinite(ur) || !isfinite(ui)) {
+        s.status[i] = DOGPUP_RHS_BREAKDOWN;
+      } else {
+        s.rho[2 * i] = ur;
+        s.rho[2 * i + 1] = ui;
+        active = true;
+      }
+    }
+    s.alpha[2 * i] = s.alpha[2 * i + 1] = 0.0;
+    s.beta[2 * i] = s.beta[2 * i + 1] = 0.0;
+  }
+  group_flag(s, g, active);
+}
+
+// part1 = p.(A p)
+__global__ __launch_bounds__(kScalThreads) void k_cg_alpha(BState s, const double* __restrict__ part1,
+                                                           int nsp, int nbx) {
+  const int g = blockIdx.x, c = threadIdx.x;
+  if (!s.grp_active[g]) return;
+  double dr = 0, di = 0;
+  block_sum_part(part1, g, nsp, nbx, dr, di);
+  if (c >= nsp) return;
+  const long long i = (long long)g * nsp + c;
+  double ar = 0.0, ai = 0.0;
+  if (s.status[i] == ST_ACTIVE) {
+    if ((dr == 0.0 && di == 0.0) || !isfinite(dr) || !isfinite(di)) {
+      s.status[i] = DOGPUP_RHS_BREAKDOWN;
+    } else {
+      const cpx<double> a = cdiv(cpx<double>{s.rho[2 * i], s.rho[2 * i + 1]}, cpx<double>{dr, di});
+      ar = a.re;
+      ai = a.im;
+    }
+  }
+  s.alpha[2 * i] = ar;
+  s.alpha[2 * i + 1] = ai;
+}
+
+// part2 = |r|^2 (after the update), part1 = u.u with u = G r: convergence,
+// then beta and rho for the columns that go on.
+__global__ __launch_bounds__(kScalThreads) void k_cg_beta(BState s, const double* __restrict__ part1,
+                                                          const double* __restrict__ part2, int nsp,
+                                                          int nbx, int maxit) {
+  const int g = blockIdx.x, c = threadIdx.x;
+  if (!s.grp_active[g]) return;
+  double n2 = 0, dummy = 0, ur = 0, ui = 0;
+  block_sum_part(part2, g, nsp, nbx, n2, dummy);
+  block_sum_part(part1, g, nsp, nbx, ur, ui);
+  bool active = false;
+  if (c < nsp) {
+    const long long i = (long long)g * nsp + c;
+    double br = 0.0, bi = 0.0;
+    if (s.status[i] == ST_ACTIVE) {
+      const int it = ++s.iters[i];
+      const double rn = sqrt(n2) / s.bnorm[i];
+      s.relres[i] = rn;
+      if (!isfinite(rn)) s.status[i] = DOGPUP_RHS_BREAKDOWN;
+      else if (rn < s.tol[i]) s.status[i] = DOGPUP_RHS_CONVERGED;
+      else if (it >= maxit) s.status[i] = DOGPUP_RHS_MAXIT;
+      else if ((ur == 0.0 && ui == 0.0) || !isfinite(ur) || !isfinite(ui)) s.status[i] = DOGPUP_RHS_BREAKDOWN;
+      else {
+        const cpx<double> b = cdiv(cpx<double>{ur, ui}, cpx<double>{s.rho[2 * i], s.rho[2 * i + 1]});
+        br = b.re;
+        bi = b.im;
+        s.rho[2 * i] = ur;
+        s.rho[2 * i + 1] = ui;
+        active = true;
+      }
+    }
+    s.beta[2 * i] = br;
+    s.beta[2 * i + 1] = bi;
+  }
+  group_flag(s, g, active);
+}
+
 // Outer (FP64) refinement state
 struct RState {
   double* bnorm;    // nrhs
@@ -857,6 +1009,7 @@ struct Buf {
 struct dogpup_solver {
   int nn = 0, nf = 0, ncols = 0, nsp = 0, ncb = 0, ngroups = 0, mode = 0;
   int nnzA = 0, nnzG = 0;
+  int method = DOGPUP_METHOD_BICGSTAB;  // Krylov method of the fused forms
   // caller's patterns in the reference node order (FSAI setup runs on these)
   const int *rA0 = nullptr, *cA0 = nullptr, *rG0 = nullptr, *cG0 = nullptr;
   // patterns in solver node order (owned): A, G, G^T and the value maps
@@ -1000,24 +1153,26 @@ struct Chunk {
     DP_TRY((pad_lds<MAT_A, EPI>()));
     pbegin();
     k_spmv<T, NSP, MAT_A, EPI><<<vgrid(), 256, S->lds_pad, st>>>(S->rA, S->cA, a0(), a1(), 0,
-                                                        S->fscale.template as<double>(), x, y, w, p1,
-                                                        p2, S->nn, S->ncb, g0, S->nbx, S->tile_rows, ga);
+                                                        S->fscale.template as<double>(), x, y, w, nullptr,
+                                                        p1, p2, S->nn, S->ncb, g0, S->nbx, S->tile_rows, ga);
     DP_LAUNCH_CHECK();
-    pend(1, groups(ga != nullptr) * vbytes() * (EPI == EPI_STORE ? 2 : 3) +
+    pend(1, groups(ga != nullptr) * vbytes() * ((EPI == EPI_STORE || w == x) ? 2 : 3) +
                 (double)S->nnzA * (2 * sizeof(T) + 4) + 4.0 * (S->nn + 1));
     return DOGPUP_OK;
   }
   template <int EPI>
-  int spmvG(bool tr, const T* x, T* y, const T* w, const int* ga) {
+  int spmvG(bool tr, const T* x, T* y, const T* w, const int* ga, const double* coef = nullptr) {
     const int* rp = tr ? S->rGT : S->rG;
     const int* cp = tr ? S->cGT : S->cG;
     DP_TRY((pad_lds<MAT_G, EPI>()));
     pbegin();
     k_spmv<T, NSP, MAT_G, EPI><<<vgrid(), 256, S->lds_pad, st>>>(rp, cp, gvals(tr), nullptr, S->nnzG, nullptr,
-                                                        x, y, w, p1, p2, S->nn, S->ncb, g0, S->nbx, S->tile_rows, ga);
+                                                        x, y, w, coef, p1, p2, S->nn, S->ncb, g0, S->nbx,
+                                                        S->tile_rows, ga);
     DP_LAUNCH_CHECK();
-    pend(0, groups(ga != nullptr) * (vbytes() * (EPI == EPI_STORE ? 2 : (EPI == EPI_DOT2 ? 2 : 3)) +
-                                    (double)S->nnzG * (2 * sizeof(T) + 4) + 4.0 * (S->nn + 1)));
+    const int streams = (EPI == EPI_STORE || EPI == EPI_DOT2 || EPI == EPI_STORE_SELF) ? 2 : 3;
+    pend(0, groups(ga != nullptr) * (vbytes() * streams + (double)S->nnzG * (2 * sizeof(T) + 4) +
+                                    4.0 * (S->nn + 1)));
     return DOGPUP_OK;
   }
 
@@ -1092,9 +1247,47 @@ struct Chunk {
     return DOGPUP_OK;
   }
 
-  // Fused BiCGSTAB.  If b != nullptr the init pass runs from b (x = 0);
-  // otherwise r, rhat, p, x and the state are already initialised (the
-  // refinement residual pass did it).  iters_run = loop iterations issued.
+  // One COCG iteration over every active group: q = A p (p.q), alpha,
+  // x += alpha p and r -= alpha q (|r|^2), u = G r (u.u), convergence and
+  // beta, p = G^T u + beta p.  Three operator applications and 13 vector
+  // streams per iteration (BiCGSTAB: seven and 30).
+  int body_cocg(Vecs<T> V, int maxit) {
+    const int* ga = bs.grp_active;
+    const int nsp = NSP;
+    DP_TRY(spmvA<EPI_STORE_DOT>(V.p, V.v, V.p, ga));             // q = A p, p.q
+    k_cg_alpha<<<gcount, kScalThreads, 0, st>>>(bs, p1, nsp, S->nbx);
+    DP_LAUNCH_CHECK();
+    pbegin();
+    k_cg_update<T, NSP><<<vgrid(), 256, 0, st>>>(V.x, V.p, V.r, V.v, bs.alpha, bs.status, p2, S->nn, S->nbx,
+                                                S->tile_rows, ga);
+    DP_LAUNCH_CHECK();
+    pend(2, groups(true) * vbytes() * 6);
+    DP_TRY(spmvG<EPI_STORE_SELF>(false, V.r, V.u, nullptr, ga));  // u = G r, u.u
+    DP_TRY(reset_count());
+    k_cg_beta<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, nsp, S->nbx, maxit);
+    DP_LAUNCH_CHECK();
+    DP_TRY(spmvG<EPI_AXPBY>(true, V.u, V.p, nullptr, ga, bs.beta));  // p = G^T u + beta p
+    return DOGPUP_OK;
+  }
+
+  // COCG start from r (state initialised): u = G r, rho = u.u, p = G^T u
+  // (p holds finite values and beta = 0).
+  int cocg_start(Vecs<T> V) {
+    const int* ga = bs.grp_active;
+    DP_TRY(spmvG<EPI_STORE_SELF>(false, V.r, V.u, nullptr, ga));
+    DP_TRY(reset_count());
+    k_cg_start<<<gcount, kScalThreads, 0, st>>>(bs, p1, NSP, S->nbx);
+    DP_LAUNCH_CHECK();
+    DP_TRY(spmvG<EPI_AXPBY>(true, V.u, V.p, nullptr, ga, bs.beta));
+    return DOGPUP_OK;
+  }
+
+  int step(Vecs<T> V, int maxit) { return S->method == DOGPUP_METHOD_COCG ? body_cocg(V, maxit) : body(V, maxit); }
+
+  // Fused BiCGSTAB or COCG.  If b != nullptr the init pass runs from b
+  // (x = 0); otherwise r, rhat, p, x and the state are already initialised
+  // (the refinement residual pass did it).  iters_run = loop iterations
+  // issued.
   int run_fused(Vecs<T> V, const T* b, int maxit, const int* mask, int* iters_run) {
     const int nsp = NSP;
     if (b) {
@@ -1104,12 +1297,13 @@ struct Chunk {
       k_sc_init<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, mask, nsp, S->nbx, S->ncb, S->ncols, g0);
       DP_LAUNCH_CHECK();
     }
+    if (S->method == DOGPUP_METHOD_COCG) DP_TRY(cocg_start(V));
     int active = 0;
     DP_TRY(wait_count(&active));
     int it = 0;
     if (S->profile || !S->use_graph) {
       while (active > 0 && it < maxit) {
-        DP_TRY(body(V, maxit));
+        DP_TRY(step(V, maxit));
         ++it;
         DP_TRY(wait_count(&active));
       }
@@ -1117,14 +1311,15 @@ struct Chunk {
       // Replay a captured iteration K times between convergence polls;
       // converged groups make every kernel exit early, so replays past
       // convergence cost only launch latency.
-      const long long key = ((long long)g0 << 32) ^ ((long long)maxit << 2) ^ (sizeof(T) == 4 ? 1 : 2);
+      const long long key = ((long long)g0 << 32) ^ ((long long)maxit << 3) ^ ((long long)S->method << 2) ^
+                            (sizeof(T) == 4 ? 1 : 2);
       hipGraphExec_t ex = nullptr;
       auto found = S->graphs.find(key);
       if (found != S->graphs.end()) {
         ex = found->second;
       } else {
         DP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        const int rc = body(V, maxit);
+        const int rc = step(V, maxit);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(st, &g);
         if (rc != DOGPUP_OK) {
@@ -1776,6 +1971,16 @@ int dogpup_solver_solve(dogpup_solver_t S, const double* d_b, double* d_x, doubl
     (void)hipEventDestroy(r.b);
   }
   S->recs.clear();
+  return DOGPUP_OK;
+}
+
+int dogpup_solver_set_method(dogpup_solver_t S, int method) {
+  DP_ARG(S != nullptr, "NULL solver");
+  DP_ARG(method == DOGPUP_METHOD_BICGSTAB || method == DOGPUP_METHOD_COCG, "unknown method %d", method);
+  DP_ARG(method == DOGPUP_METHOD_BICGSTAB || S->mode != DOGPUP_MODE_C128_PLAIN,
+         "the reference-structure form is BiCGSTAB only");
+  if (method != S->method) S->clear_graphs();
+  S->method = method;
   return DOGPUP_OK;
 }
 

@@ -48,7 +48,8 @@ def tg_coefficients(optode, gate_fc, dxyz):
     return torch.as_tensor(np.ascontiguousarray(coef), dtype=torch.complex128, device=DEV)
 
 
-def jacobian_tg(mesh, gate_fc, mode="mixed", tol=1e-10, dxyz=None, scale=None, field=None):
+def jacobian_tg(mesh, gate_fc, mode="mixed", tol=1e-10, dxyz=None, scale=None, field=None,
+                method="bicgstab"):
     """J_TG as float32 [NV][NG][NM] (= the reference's NM x NG x NV,
     column-major).  ``field``: a forward+adjoint FluenceField from
     ``mesh.solve_point_sources('sd')`` to reuse; ``scale``: optional NM x NG
@@ -61,7 +62,7 @@ def jacobian_tg(mesh, gate_fc, mode="mixed", tol=1e-10, dxyz=None, scale=None, f
             raise ValueError("voxel side dxyz is needed (set mesh.dxyz or pass dxyz)")
     t0 = time.perf_counter()
     if field is None:
-        field = mesh.solve_point_sources("sd", mode=mode, tol=tol)
+        field = mesh.solve_point_sources("sd", mode=mode, tol=tol, method=method)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     coef = tg_coefficients(opt, gate_fc, dxyz)
@@ -131,7 +132,7 @@ class AbsorptionTG(torch.autograd.Function):
     @staticmethod
     def forward(ctx, mua, mesh, gate_fc, exact=False, mode="fp64", tol=1e-12):
         mesh.update_properties(np.stack([mua.detach().cpu().numpy(), mesh.musp], 1), kappa=mesh.kappa.copy())
-        field = mesh.solve_point_sources("sd", mode=mode, tol=tol)
+        field = mesh.solve_point_sources("sd", mode=mode, tol=tol, method=method)
         ctx.mesh, ctx.field, ctx.gate_fc, ctx.exact = mesh, field, gate_fc, exact
         ctx.in_device, ctx.in_dtype = mua.device, mua.dtype
         return tg_data(mesh, field, gate_fc)

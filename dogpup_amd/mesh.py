@@ -21,6 +21,7 @@ from . import _lib, topology
 from .transforms import td2fc
 
 DEV = "cuda"
+METHODS = {"bicgstab": _lib.METHOD_BICGSTAB, "cocg": _lib.METHOD_COCG}
 MODES = {"c128": _lib.MODE_C128, "c64": _lib.MODE_C64_REFINE, "mixed": _lib.MODE_C64_REFINE,
          "plain": _lib.MODE_C128_PLAIN, "fp64": _lib.MODE_C128}
 
@@ -207,23 +208,26 @@ class DMesh:
         return _i32(nodes), _f64(w)
 
     # -- solver -----------------------------------------------------------------
-    def solver(self, ncols, mode="mixed"):
+    def solver(self, ncols, mode="mixed", method="bicgstab"):
         key = (ncols, mode)
         s = self._solvers.get(key)
         if s is None:
             s = FluenceSolver(self, ncols, MODES[mode])
             self._solvers[key] = s
+        s.set_method(method)
         return s
 
-    def solve_point_sources(self, which="s", mode="mixed", tol=None, maxit=1000, extra=None):
+    def solve_point_sources(self, which="s", mode="mixed", tol=None, maxit=1000, extra=None,
+                            method="bicgstab"):
         """Solve for the sources ('s'), detectors ('d') or both ('sd', forward
         and adjoint batched in one launch: columns 0..NS-1 sources,
-        NS..NS+ND-1 detectors)."""
+        NS..NS+ND-1 detectors).  method: 'bicgstab' (the reference's) or
+        'cocg' (conjugate orthogonal CG, same solution)."""
         parts = [self._bary_dev(w) for w in which]
         nodes = torch.cat([p[0] for p in parts])
         w = torch.cat([p[1] for p in parts])
         ncols = nodes.shape[0]
-        S = self.solver(ncols, mode)
+        S = self.solver(ncols, mode, method)
         spec = torch.as_tensor(self.optode.s_fpsf, dtype=torch.complex128, device=DEV)
         b = S.empty_block()
         _lib.call("dogpup_point_sources", _lib.ptr(torch.view_as_real(b)), self.nn, S.nf, ncols, S.nsp,
@@ -275,6 +279,14 @@ class FluenceSolver:
                   _lib.ptr(d["cG"]), _lib.ptr(np.ascontiguousarray(order, np.int32)))
         self.h = h
         self._version = None
+        self.method = "bicgstab"
+
+    def set_method(self, method: str):
+        """Krylov method of the fused modes: 'bicgstab' or 'cocg'."""
+        if method not in METHODS:
+            raise ValueError(f"unknown method {method!r}; expected one of {sorted(METHODS)}")
+        _lib.call("dogpup_solver_set_method", self.h, METHODS[method])
+        self.method = method
 
     def __del__(self):
         try:

@@ -44,6 +44,10 @@ extern "C" {
 #define DOGPUP_MODE_C64_REFINE 1    /* complex64 fused BiCGSTAB inside FP64 iterative refinement */
 #define DOGPUP_MODE_C128_PLAIN 2    /* complex128, reference structure: unfused passes, lockstep */
 
+/* Krylov method of the fused forms (DOGPUP_MODE_C128, DOGPUP_MODE_C64_REFINE) */
+#define DOGPUP_METHOD_BICGSTAB 0    /* the reference's BiCGSTAB (dMesh.m:1247-1348), the default */
+#define DOGPUP_METHOD_COCG 1        /* conjugate orthogonal CG: CG with the unconjugated form */
+
 const char* dogpup_last_error(void);
 int dogpup_version(void);
 int dogpup_device_count(int* count);
@@ -155,6 +159,14 @@ int dogpup_solver_set_matrix(dogpup_solver_t s, const double* d_valA_re, const d
  * iterations, final relative residual, status.  Synchronous. */
 int dogpup_solver_solve(dogpup_solver_t s, const double* d_b, double* d_x, double tol, int maxit,
                         int* h_iters, double* h_relres, int* h_status, void* stream);
+
+/* Krylov method for the following solves (DOGPUP_METHOD_*).  Both converge
+ * to the same x on the same tolerance; A_k is complex symmetric and
+ * M^-1 = G_k^T G_k is symmetric, which is what COCG needs.  COCG applies three
+ * operators per iteration (A, G, G^T) and moves 13 vectors, BiCGSTAB seven
+ * and 30.  No counterpart in the reference, which hard-codes BiCGSTAB
+ * (dMesh.m:1247-1348); the reference-structure mode is BiCGSTAB only. */
+int dogpup_solver_set_method(dogpup_solver_t s, int method);
 
 /* Counters of the last solve: number of vector-kernel passes weighted by the
  * columns they moved, for roofline accounting.  h_out[0] = BiCGSTAB

@@ -114,21 +114,26 @@ def test_compat_fsaip_gen(setup):
     assert rel(got, valG) < 1e-13
 
 
-@pytest.mark.parametrize("mode,tol,bound", [("fp64", 1e-12, 1e-9), ("mixed", 1e-10, 1e-7),
-                                            ("plain", 1e-12, 1e-9)])
-def test_solver_vs_direct(setup, mode, tol, bound):
+@pytest.mark.parametrize("mode,tol,bound,method", [("fp64", 1e-12, 1e-9, "bicgstab"),
+                                                   ("mixed", 1e-10, 1e-7, "bicgstab"),
+                                                   ("plain", 1e-12, 1e-9, "bicgstab"),
+                                                   ("fp64", 1e-12, 1e-9, "cocg"),
+                                                   ("mixed", 1e-10, 1e-7, "cocg")])
+def test_solver_vs_direct(setup, mode, tol, bound, method):
     from oracle import optode as oopt, solver as osol
     D, A, om = setup["D"], setup["A"], setup["om"]
     opt = setup["opt"]
     sid, sw = opt.s_bary
     Q = oopt.source_matrix(om, sid, sw, opt.s_fpsf)
     ref = osol.direct(A, Q)
-    phi = D.solve_point_sources("s", mode=mode, tol=tol)
+    print(f"[case] solver {mode} {method} tol {tol}", flush=True)
+    phi = D.solve_point_sources("s", mode=mode, tol=tol, method=method)
     got = phi.to_reference_layout().cpu().numpy()
     assert (phi.status == 0).all(), phi.status
+    assert (phi.relres < tol).all(), phi.relres
     for k in range(setup["nf"]):
         for s in range(Q.shape[2]):
-            assert rel(got[:, k, s], ref[:, k, s]) < bound, (mode, k, s)
+            assert rel(got[:, k, s], ref[:, k, s]) < bound, (mode, method, k, s)
 
 
 def test_solver_fsai_factors(setup):
@@ -250,7 +255,8 @@ def test_autograd_vjp_vs_fd_and_reference_jacobian(setup):
     assert rel(g_nodal, gref) < 1e-8
 
 
-def test_graph_replay_matches_eager(setup, monkeypatch):
+@pytest.mark.parametrize("method", ["bicgstab", "cocg"])
+def test_graph_replay_matches_eager(setup, monkeypatch, method):
     """The captured-iteration path (HIP graph replayed between convergence
     polls, several solves on one handle) gives exactly the eager result."""
     from dogpup_amd import mesh as dm
@@ -258,7 +264,9 @@ def test_graph_replay_matches_eager(setup, monkeypatch):
     res = {}
     for flag in ("0", "1"):
         monkeypatch.setenv("DOGPUP_GRAPH", flag)
+        print(f"[case] graph {flag} {method}", flush=True)
         S = dm.FluenceSolver(D, 4, dm.MODES["mixed"])
+        S.set_method(method)
         nodes, w = D._bary_dev("s")
         spec = torch.as_tensor(D.optode.s_fpsf, dtype=torch.complex128, device="cuda")
         b = S.empty_block()
