@@ -32,9 +32,9 @@ namespace dogpup {
 enum { MAT_A = 0, MAT_G = 1 };
 // SpMV epilogues: EPI_STORE y; EPI_STORE_DOT y and w.y; EPI_DOT2 only w.y
 // and y.y (y not stored); EPI_STORE_SELF y and y.y; EPI_AXPBY y <- y_new +
-// coef*y_old (in place, one complex coefficient per column).  All dots are
-// unconjugated.
-enum { EPI_STORE = 0, EPI_STORE_DOT = 1, EPI_DOT2 = 2, EPI_STORE_SELF = 3, EPI_AXPBY = 4 };
+// coef*y_old (in place, one complex coefficient per column); EPI_AXPBY_X
+// also xacc += coef2*y_old.  All dots are unconjugated.
+enum { EPI_STORE = 0, EPI_STORE_DOT = 1, EPI_DOT2 = 2, EPI_STORE_SELF = 3, EPI_AXPBY = 4, EPI_AXPBY_X = 5 };
 enum { ST_ACTIVE = -1, ST_SKIP = -2 };
 
 // Row tiles are assigned so that each XCD sweeps one contiguous range of
@@ -48,13 +48,16 @@ __device__ inline int xcd_tile(int bx) { return (bx & 7) * (gridDim.x >> 3) + (b
 // all frequencies); MAT_G: M = per-frequency complex values v0[f][nnz].
 //
 // The block walks its row tile in chunks of kChunkRows rows.  Each chunk's
-// CSR column indices and values are first copied into LDS with coalesced
-// loads; the row loop then reads them from LDS, so the only vector-memory
-// instruction per nonzero is the 16-byte-per-lane gather of x (the matrix
-// entry of a nonzero is shared by the LPR lanes of its row, and fetching it
-// per wavefront from global memory costs as many address cycles as the
-// gather itself).  A chunk with more than kChunkCap nonzeros falls back to
-// reading the CSR arrays directly.
+// CSR row pointers, column indices and values are first copied into LDS with
+// coalesced loads (fetching the matrix entry per wavefront from global
+// memory would cost as many address cycles as the x gather itself); the
+// only vector-memory instruction per nonzero is then the 16-byte-per-lane
+// gather of x.  A row issues U gathers before it consumes any (U = 8 by
+// default: 6 waves per SIMD with 8 loads in flight each measured faster than
+// 4 or 16 on the 15-point A and 8-point G rows); the entries past the row's
+// end gather the row's last column again with weight zero, which keeps the
+// loads unconditional.  A chunk with more
+// than kChunkCap nonzeros reads the CSR arrays from global memory instead.
 // Per-lane column coefficients (complex double per RHS) as T.
 template <typename T, int CPL>
 __device__ inline void load_coef(const double* __restrict__ c, long long rhs0, T re[CPL], T im[CPL]) {
@@ -66,14 +69,15 @@ __device__ inline void load_coef(const double* __restrict__ c, long long rhs0, T
 }
 
 constexpr int kChunkRows = 64;
-constexpr int kChunkCap = 2048;
+constexpr int kChunkCap = 1024;
 
-template <typename T, int NSP, int MAT, int EPI>
+template <typename T, int NSP, int MAT, int EPI, int U>
 __global__ __launch_bounds__(256) void k_spmv(
     const int* __restrict__ rptr, const int* __restrict__ cptr, const T* __restrict__ v0,
     const T* __restrict__ v1, long long nnz, const double* __restrict__ fscale,
     const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ w, const double* __restrict__ coef,
-    double* __restrict__ part1, double* __restrict__ part2, int nn, int ncb, int g0, int nbx, int tile_rows,
+    const double* __restrict__ coef2, T* __restrict__ xacc, double* __restrict__ part1,
+    double* __restrict__ part2, int nn, int ncb, int g0, int nbx, int tile_rows,
     const int* __restrict__ grp_active) {
   using G = Geo<T, NSP>;
   constexpr int CPL = G::CPL;
@@ -96,8 +100,10 @@ __global__ __launch_bounds__(256) void k_spmv(
   Acc<CPL> d1, d2;
   d1.zero();
   d2.zero();
-  T cr[CPL], ci[CPL];
-  if (EPI == EPI_AXPBY) load_coef<T, CPL>(coef, (long long)g * NSP + c0, cr, ci);
+  constexpr bool AXPBY = EPI == EPI_AXPBY || EPI == EPI_AXPBY_X;
+  T cr[CPL], ci[CPL], c2r[CPL], c2i[CPL];
+  if (AXPBY) load_coef<T, CPL>(coef, (long long)g * NSP + c0, cr, ci);
+  if (EPI == EPI_AXPBY_X) load_coef<T, CPL>(coef2, (long long)g * NSP + c0, c2r, c2i);
   const int tile0 = xcd_tile(bx) * tile_rows;
   const int tile1 = min(tile0 + tile_rows, nn);
   for (int r0 = tile0; r0 < tile1; r0 += kChunkRows) {
@@ -129,59 +135,49 @@ __global__ __launch_bounds__(256) void k_spmv(
 #pragma unroll
       for (int e = 0; e < CPL; ++e) { ar[e] = T(0); ai[e] = T(0); }
       const int s0 = srp[lr] - es, s1 = srp[lr + 1] - es;
-      int p = s0;
-      for (; p + 4 <= s1; p += 4) {
-        int cc[4];
-        T mr[4], mi[4];
+      for (int p = s0; p < s1; p += U) {
+        int cc[U];
+        T mr[U], mi[U];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < U; ++q) {
+          const int pq = min(p + q, s1 - 1);
           if (staged) {
-            cc[q] = scol[p + q];
-            mr[q] = sval[2 * (p + q)];
-            mi[q] = sval[2 * (p + q) + 1];
+            cc[q] = scol[pq];
+            mr[q] = sval[2 * pq];
+            mi[q] = sval[2 * pq + 1];
           } else {
-            const long long pg = es + p + q;
+            const long long pg = es + pq;
             cc[q] = cptr[pg];
             if (MAT == MAT_A) { mr[q] = v0[pg]; mi[q] = fk * v1[pg]; }
             else { mr[q] = gv[2 * pg]; mi[q] = gv[2 * pg + 1]; }
           }
+          if (p + q >= s1) { mr[q] = T(0); mi[q] = T(0); }
         }
-        T xr[4][CPL], xi[4][CPL];
+        T xr[U][CPL], xi[U][CPL];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) Pack<T>::load(xg + (size_t)cc[q] * NSP * 2, xr[q], xi[q]);
+        for (int q = 0; q < U; ++q) Pack<T>::load(xg + (size_t)cc[q] * NSP * 2, xr[q], xi[q]);
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < U; ++q)
 #pragma unroll
           for (int e = 0; e < CPL; ++e) {
             ar[e] += mr[q] * xr[q][e] - mi[q] * xi[q][e];
             ai[e] += mr[q] * xi[q][e] + mi[q] * xr[q][e];
           }
       }
-      for (; p < s1; ++p) {
-        int cc;
-        T mr, mi;
-        if (staged) {
-          cc = scol[p];
-          mr = sval[2 * p];
-          mi = sval[2 * p + 1];
-        } else {
-          const long long pg = es + p;
-          cc = cptr[pg];
-          if (MAT == MAT_A) { mr = v0[pg]; mi = fk * v1[pg]; }
-          else { mr = gv[2 * pg]; mi = gv[2 * pg + 1]; }
-        }
-        T xr[CPL], xi[CPL];
-        Pack<T>::load(xg + (size_t)cc * NSP * 2, xr, xi);
-#pragma unroll
-        for (int e = 0; e < CPL; ++e) {
-          ar[e] += mr * xr[e] - mi * xi[e];
-          ai[e] += mr * xi[e] + mi * xr[e];
-        }
-      }
       const size_t off = ((gbase + row) * NSP + c0) * 2;
-      if (EPI == EPI_AXPBY) {
+      if (AXPBY) {
         T pr[CPL], pi[CPL];
         Pack<T>::load(y + off, pr, pi);
+        if (EPI == EPI_AXPBY_X) {
+          T xr[CPL], xi[CPL];
+          Pack<T>::load(xacc + off, xr, xi);
+#pragma unroll
+          for (int e = 0; e < CPL; ++e) {
+            xr[e] += c2r[e] * pr[e] - c2i[e] * pi[e];
+            xi[e] += c2r[e] * pi[e] + c2i[e] * pr[e];
+          }
+          Pack<T>::store(xacc + off, xr, xi);
+        }
 #pragma unroll
         for (int e = 0; e < CPL; ++e) {
           ar[e] += cr[e] * pr[e] - ci[e] * pi[e];
@@ -211,7 +207,7 @@ __global__ __launch_bounds__(256) void k_spmv(
       }
     }
   }
-  if (EPI != EPI_STORE && EPI != EPI_AXPBY) {
+  if (EPI != EPI_STORE && !AXPBY) {
     block_reduce_store<T, NSP>(d1, sh, part1, (long long)g * nbx + bx);
     if (EPI == EPI_DOT2) {
       __syncthreads();
@@ -341,41 +337,27 @@ __global__ __launch_bounds__(256) void k_update_p(T* __restrict__ p, const T* __
   (void)bx;
 }
 
-// COCG step: x += alpha p (active columns only), r -= alpha q;
-// part2 = sum |r|^2
+// COCG residual step: r -= alpha q; part2 = sum |r|^2.  (x += alpha p is
+// applied in the G^T pass of the same iteration, which reads p anyway.)
 template <typename T, int NSP>
-__global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ x, const T* __restrict__ p,
-                                                   T* __restrict__ r, const T* __restrict__ q,
-                                                   const double* __restrict__ alpha,
-                                                   const int* __restrict__ status, double* __restrict__ part2,
+__global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ r, const T* __restrict__ q,
+                                                   const double* __restrict__ alpha, double* __restrict__ part2,
                                                    int nn, int nbx, int tile_rows,
                                                    const int* __restrict__ grp_active) {
   Acc<Geo<T, NSP>::CPL> d2;
   d2.zero();
   T alr[Geo<T, NSP>::CPL], ali[Geo<T, NSP>::CPL];
-  bool act[Geo<T, NSP>::CPL];
   ROW_LOOP_BEGIN
-  if (st == 0) {
-    load_coef<T, CPL>(alpha, rhs0, alr, ali);
-#pragma unroll
-    for (int e = 0; e < CPL; ++e) act[e] = status[rhs0 + e] == ST_ACTIVE;
-  }
-  T xr[CPL], xi[CPL], pr[CPL], pi[CPL], rr[CPL], ri[CPL], qr[CPL], qi[CPL];
-  Pack<T>::load(x + off, xr, xi);
-  Pack<T>::load(p + off, pr, pi);
+  if (st == 0) load_coef<T, CPL>(alpha, rhs0, alr, ali);
+  T rr[CPL], ri[CPL], qr[CPL], qi[CPL];
   Pack<T>::load(r + off, rr, ri);
   Pack<T>::load(q + off, qr, qi);
 #pragma unroll
   for (int e = 0; e < CPL; ++e) {
-    if (act[e]) {
-      xr[e] += alr[e] * pr[e] - ali[e] * pi[e];
-      xi[e] += alr[e] * pi[e] + ali[e] * pr[e];
-    }
     rr[e] -= alr[e] * qr[e] - ali[e] * qi[e];
     ri[e] -= alr[e] * qi[e] + ali[e] * qr[e];
     d2.re[e] += (double)rr[e] * rr[e] + (double)ri[e] * ri[e];
   }
-  Pack<T>::store(x + off, xr, xi);
   Pack<T>::store(r + off, rr, ri);
   ROW_LOOP_END
   block_reduce_store<T, NSP>(d2, sh, part2, (long long)g * nbx + bx);
@@ -570,6 +552,7 @@ struct BState {
   int* bd;         // nrhs
   int* iters;      // nrhs
   int* grp_active; // ngroups
+  int* grp_prev;   // ngroups: active at the start of the current COCG iteration
   int* counter;    // 4
 };
 
@@ -817,7 +800,10 @@ __global__ __launch_bounds__(kScalThreads) void k_cg_beta(BState s, const double
                                                           const double* __restrict__ part2, int nsp,
                                                           int nbx, int maxit) {
   const int g = blockIdx.x, c = threadIdx.x;
-  if (!s.grp_active[g]) return;
+  // grp_prev gates this iteration's G^T pass, which applies x += alpha p
+  const int was = s.grp_active[g];
+  if (c == 0) s.grp_prev[g] = was;
+  if (!was) return;
   double n2 = 0, dummy = 0, ur = 0, ui = 0;
   block_sum_part(part2, g, nsp, nbx, n2, dummy);
   block_sum_part(part1, g, nsp, nbx, ur, ui);
@@ -1017,6 +1003,7 @@ struct dogpup_solver {
   const int *rA = nullptr, *cA = nullptr, *rG = nullptr, *cG = nullptr;
   const int *rGT = nullptr, *cGT = nullptr;
   dogpup::Buf pAr, pAc, pGr, pGc, pGTr, pGTc, amap, gmap, gtmap, order;
+  int unroll_a = 8, unroll_g = 8;  // gathers in flight per row (k_spmv: 4, 8 or 16)
   // values: a64re/a64im and g64 in reference order; the rest in solver order
   dogpup::Buf a64re, a64im, a64pre, a64pim, a32re, a32im, fscale, g64, gp64, gtp64, g32, gt32;
   dogpup::Buf ws, part1, part2, state, rstate, bperm, xperm;
@@ -1141,36 +1128,49 @@ struct Chunk {
     return tr ? (const T*)S->gtp64.p : (const T*)S->gp64.p;
   }
 
-  template <int MAT, int EPI>
+  template <int MAT, int EPI, int U>
   int pad_lds() {
     if (S->lds_pad > 0)
-      DP_HIP(hipFuncSetAttribute((const void*)k_spmv<T, NSP, MAT, EPI>,
+      DP_HIP(hipFuncSetAttribute((const void*)k_spmv<T, NSP, MAT, EPI, U>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, S->lds_pad));
     return DOGPUP_OK;
   }
+  template <int MAT, int EPI, int U>
+  int launch(const int* rp, const int* cp, const T* m0, const T* m1, long long nz, const T* x, T* y,
+             const T* w, const int* ga, const double* coef, const double* coef2, T* xacc) {
+    DP_TRY((pad_lds<MAT, EPI, U>()));
+    k_spmv<T, NSP, MAT, EPI, U><<<vgrid(), 256, S->lds_pad, st>>>(
+        rp, cp, m0, m1, nz, S->fscale.template as<double>(), x, y, w, coef, coef2, xacc, p1, p2, S->nn, S->ncb,
+        g0, S->nbx, S->tile_rows, ga);
+    DP_LAUNCH_CHECK();
+    return DOGPUP_OK;
+  }
+  template <int MAT, int EPI>
+  int launch_u(int u, const int* rp, const int* cp, const T* m0, const T* m1, long long nz, const T* x, T* y,
+               const T* w, const int* ga, const double* coef, const double* coef2, T* xacc) {
+    switch (u) {
+      case 4: return launch<MAT, EPI, 4>(rp, cp, m0, m1, nz, x, y, w, ga, coef, coef2, xacc);
+      case 8: return launch<MAT, EPI, 8>(rp, cp, m0, m1, nz, x, y, w, ga, coef, coef2, xacc);
+      default: return launch<MAT, EPI, 16>(rp, cp, m0, m1, nz, x, y, w, ga, coef, coef2, xacc);
+    }
+  }
   template <int EPI>
   int spmvA(const T* x, T* y, const T* w, const int* ga) {
-    DP_TRY((pad_lds<MAT_A, EPI>()));
     pbegin();
-    k_spmv<T, NSP, MAT_A, EPI><<<vgrid(), 256, S->lds_pad, st>>>(S->rA, S->cA, a0(), a1(), 0,
-                                                        S->fscale.template as<double>(), x, y, w, nullptr,
-                                                        p1, p2, S->nn, S->ncb, g0, S->nbx, S->tile_rows, ga);
-    DP_LAUNCH_CHECK();
+    DP_TRY((launch_u<MAT_A, EPI>(S->unroll_a, S->rA, S->cA, a0(), a1(), 0, x, y, w, ga, nullptr, nullptr,
+                                 nullptr)));
     pend(1, groups(ga != nullptr) * vbytes() * ((EPI == EPI_STORE || w == x) ? 2 : 3) +
                 (double)S->nnzA * (2 * sizeof(T) + 4) + 4.0 * (S->nn + 1));
     return DOGPUP_OK;
   }
   template <int EPI>
-  int spmvG(bool tr, const T* x, T* y, const T* w, const int* ga, const double* coef = nullptr) {
-    const int* rp = tr ? S->rGT : S->rG;
-    const int* cp = tr ? S->cGT : S->cG;
-    DP_TRY((pad_lds<MAT_G, EPI>()));
+  int spmvG(bool tr, const T* x, T* y, const T* w, const int* ga, const double* coef = nullptr,
+            const double* coef2 = nullptr, T* xacc = nullptr) {
     pbegin();
-    k_spmv<T, NSP, MAT_G, EPI><<<vgrid(), 256, S->lds_pad, st>>>(rp, cp, gvals(tr), nullptr, S->nnzG, nullptr,
-                                                        x, y, w, coef, p1, p2, S->nn, S->ncb, g0, S->nbx,
-                                                        S->tile_rows, ga);
-    DP_LAUNCH_CHECK();
-    const int streams = (EPI == EPI_STORE || EPI == EPI_DOT2 || EPI == EPI_STORE_SELF) ? 2 : 3;
+    DP_TRY((launch_u<MAT_G, EPI>(S->unroll_g, tr ? S->rGT : S->rG, tr ? S->cGT : S->cG, gvals(tr), nullptr,
+                                 S->nnzG, x, y, w, ga, coef, coef2, xacc)));
+    const int streams = (EPI == EPI_STORE || EPI == EPI_DOT2 || EPI == EPI_STORE_SELF) ? 2
+                        : (EPI == EPI_AXPBY_X ? 5 : 3);
     pend(0, groups(ga != nullptr) * (vbytes() * streams + (double)S->nnzG * (2 * sizeof(T) + 4) +
                                     4.0 * (S->nn + 1)));
     return DOGPUP_OK;
@@ -1248,9 +1248,12 @@ struct Chunk {
   }
 
   // One COCG iteration over every active group: q = A p (p.q), alpha,
-  // x += alpha p and r -= alpha q (|r|^2), u = G r (u.u), convergence and
-  // beta, p = G^T u + beta p.  Three operator applications and 13 vector
-  // streams per iteration (BiCGSTAB: seven and 30).
+  // r -= alpha q (|r|^2), u = G r (u.u), convergence and beta, then
+  // p = G^T u + beta p with x += alpha p (old p) in the same pass.  Three
+  // operator applications and 12 vector streams per iteration (BiCGSTAB:
+  // seven and 30).  The G^T pass runs for the groups that were active when
+  // the iteration began (grp_prev), so a group that converges in this
+  // iteration still receives its last x update.
   int body_cocg(Vecs<T> V, int maxit) {
     const int* ga = bs.grp_active;
     const int nsp = NSP;
@@ -1258,20 +1261,20 @@ struct Chunk {
     k_cg_alpha<<<gcount, kScalThreads, 0, st>>>(bs, p1, nsp, S->nbx);
     DP_LAUNCH_CHECK();
     pbegin();
-    k_cg_update<T, NSP><<<vgrid(), 256, 0, st>>>(V.x, V.p, V.r, V.v, bs.alpha, bs.status, p2, S->nn, S->nbx,
-                                                S->tile_rows, ga);
+    k_cg_update<T, NSP><<<vgrid(), 256, 0, st>>>(V.r, V.v, bs.alpha, p2, S->nn, S->nbx, S->tile_rows, ga);
     DP_LAUNCH_CHECK();
-    pend(2, groups(true) * vbytes() * 6);
+    pend(2, groups(true) * vbytes() * 3);
     DP_TRY(spmvG<EPI_STORE_SELF>(false, V.r, V.u, nullptr, ga));  // u = G r, u.u
     DP_TRY(reset_count());
     k_cg_beta<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, nsp, S->nbx, maxit);
     DP_LAUNCH_CHECK();
-    DP_TRY(spmvG<EPI_AXPBY>(true, V.u, V.p, nullptr, ga, bs.beta));  // p = G^T u + beta p
+    // p = G^T u + beta p, x += alpha p
+    DP_TRY(spmvG<EPI_AXPBY_X>(true, V.u, V.p, nullptr, bs.grp_prev, bs.beta, bs.alpha, V.x));
     return DOGPUP_OK;
   }
 
-  // COCG start from r (state initialised): u = G r, rho = u.u, p = G^T u
-  // (p holds finite values and beta = 0).
+  // COCG start from r (state initialised, x = 0): u = G r, rho = u.u,
+  // p = G^T u (p holds finite values, beta = 0).
   int cocg_start(Vecs<T> V) {
     const int* ga = bs.grp_active;
     DP_TRY(spmvG<EPI_STORE_SELF>(false, V.r, V.u, nullptr, ga));
@@ -1377,11 +1380,11 @@ static BState carve_state(const Buf& b, int nrhs, int ngroups) {
   s.bnorm = d(nrhs); s.relres = d(nrhs); s.tol = d(nrhs);
   s.rho = d(2 * nrhs); s.alpha = d(2 * nrhs); s.omega = d(2 * nrhs); s.beta = d(2 * nrhs);
   s.status = i(nrhs); s.half = i(nrhs); s.bd = i(nrhs); s.iters = i(nrhs);
-  s.grp_active = i(ngroups); s.counter = i(4);
+  s.grp_active = i(ngroups); s.grp_prev = i(ngroups); s.counter = i(4);
   return s;
 }
 static size_t state_bytes(int nrhs, int ngroups) {
-  return 7 * align256((size_t)nrhs * 16) + 4 * align256((size_t)nrhs * 4) + align256((size_t)ngroups * 4) + 256;
+  return 7 * align256((size_t)nrhs * 16) + 4 * align256((size_t)nrhs * 4) + 2 * align256((size_t)ngroups * 4) + 256;
 }
 static RState carve_rstate(const Buf& b, int nrhs) {
   RState s;
@@ -1848,6 +1851,8 @@ int dogpup_solver_create(dogpup_solver_t* out, int nn, int nfreq, int ncols, int
   DP_TRY(up(S->pGr, rGp)); DP_TRY(up(S->pGc, cGp)); DP_TRY(up(S->gmap, gmap));
   DP_TRY(up(S->pGTr, rGT)); DP_TRY(up(S->pGTc, cGT)); DP_TRY(up(S->gtmap, gtmap));
   DP_TRY(up(S->order, order));
+  if (const char* e = getenv("DOGPUP_SPMV_UA")) S->unroll_a = atoi(e);
+  if (const char* e = getenv("DOGPUP_SPMV_UG")) S->unroll_g = atoi(e);
   S->rA = S->pAr.as<int>(); S->cA = S->pAc.as<int>();
   S->rG = S->pGr.as<int>(); S->cG = S->pGc.as<int>();
   S->rGT = S->pGTr.as<int>(); S->cGT = S->pGTc.as<int>();

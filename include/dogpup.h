@@ -163,7 +163,7 @@ int dogpup_solver_solve(dogpup_solver_t s, const double* d_b, double* d_x, doubl
 /* Krylov method for the following solves (DOGPUP_METHOD_*).  Both converge
  * to the same x on the same tolerance; A_k is complex symmetric and
  * M^-1 = G_k^T G_k is symmetric, which is what COCG needs.  COCG applies three
- * operators per iteration (A, G, G^T) and moves 13 vectors, BiCGSTAB seven
+ * operators per iteration (A, G, G^T) and moves 12 vectors, BiCGSTAB seven
  * and 30.  No counterpart in the reference, which hard-codes BiCGSTAB
  * (dMesh.m:1247-1348); the reference-structure mode is BiCGSTAB only. */
 int dogpup_solver_set_method(dogpup_solver_t s, int method);
