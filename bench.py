@@ -211,7 +211,7 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-jacobian", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--method", default="bicgstab", choices=["bicgstab", "cocg"],
+    ap.add_argument("--method", default="cocg", choices=["bicgstab", "cocg"],
                     help="Krylov method of the headline mixed solve")
     ap.add_argument("--no-alt", action="store_true", help="skip the mixed solve with the other method")
     ap.add_argument("--tol", type=float, default=1e-8,

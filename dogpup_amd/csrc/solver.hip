@@ -71,43 +71,22 @@ __device__ inline void load_coef(const double* __restrict__ c, long long rhs0, T
 constexpr int kChunkRows = 64;
 constexpr int kChunkCap = 1024;
 
-template <typename T, int NSP, int MAT, int EPI, int U>
-__global__ __launch_bounds__(256) void k_spmv(
-    const int* __restrict__ rptr, const int* __restrict__ cptr, const T* __restrict__ v0,
-    const T* __restrict__ v1, long long nnz, const double* __restrict__ fscale,
-    const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ w, const double* __restrict__ coef,
-    const double* __restrict__ coef2, T* __restrict__ xacc, double* __restrict__ part1,
-    double* __restrict__ part2, int nn, int ncb, int g0, int nbx, int tile_rows,
-    const int* __restrict__ grp_active) {
+// Row products of one row tile [tile0, tile1): for every row, (ar, ai) =
+// (M x)(row) over this lane's CPL columns, handed to epi(row, ar, ai).
+// srp/scol/sval are the caller's LDS chunk buffers.  All threads of the block
+// call it (it synchronises).
+template <typename T, int NSP, int MAT, int U, typename Epi>
+__device__ inline void spmv_rows(const int* __restrict__ rptr, const int* __restrict__ cptr,
+                                 const T* __restrict__ v0, const T* __restrict__ v1, T fk,
+                                 const T* __restrict__ xg, int tile0, int tile1, int* srp, int* scol, T* sval,
+                                 Epi&& epi) {
   using G = Geo<T, NSP>;
   constexpr int CPL = G::CPL;
   static_assert(kChunkRows % G::RPB == 0, "chunk");
-  const int g = blockIdx.y, bx = blockIdx.x;
-  if (grp_active && !grp_active[g]) return;
-  __shared__ double sh[2 * 4 * NSP];
-  __shared__ int srp[kChunkRows + 1];
-  __shared__ int scol[kChunkCap];
-  __shared__ T sval[2 * kChunkCap];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int rsub = lane / G::LPR, cl = lane % G::LPR, c0 = cl * CPL;
-  const int f = (g0 + g) / ncb;
-  T fk = T(0);
-  const T* gv = v0;
-  if (MAT == MAT_A) fk = (T)fscale[f];
-  else gv = v0 + 2 * (long long)f * nnz;
-  const size_t gbase = (size_t)g * nn;
-  const T* xg = x + (gbase * NSP + c0) * 2;
-  Acc<CPL> d1, d2;
-  d1.zero();
-  d2.zero();
-  constexpr bool AXPBY = EPI == EPI_AXPBY || EPI == EPI_AXPBY_X;
-  T cr[CPL], ci[CPL], c2r[CPL], c2i[CPL];
-  if (AXPBY) load_coef<T, CPL>(coef, (long long)g * NSP + c0, cr, ci);
-  if (EPI == EPI_AXPBY_X) load_coef<T, CPL>(coef2, (long long)g * NSP + c0, c2r, c2i);
-  const int tile0 = xcd_tile(bx) * tile_rows;
-  const int tile1 = min(tile0 + tile_rows, nn);
+  const int rsub = lane / G::LPR;
   for (int r0 = tile0; r0 < tile1; r0 += kChunkRows) {
-    const int r1 = min(r0 + kChunkRows, nn);
+    const int r1 = min(r0 + kChunkRows, tile1);
     __syncthreads();  // the previous chunk's LDS reads are done
     for (int q = threadIdx.x; q <= r1 - r0; q += 256) srp[q] = rptr[r0 + q];
     __syncthreads();
@@ -120,8 +99,8 @@ __global__ __launch_bounds__(256) void k_spmv(
           sval[2 * q] = v0[es + q];
           sval[2 * q + 1] = fk * v1[es + q];
         } else {
-          sval[2 * q] = gv[2 * (long long)(es + q)];
-          sval[2 * q + 1] = gv[2 * (long long)(es + q) + 1];
+          sval[2 * q] = v0[2 * (long long)(es + q)];
+          sval[2 * q + 1] = v0[2 * (long long)(es + q) + 1];
         }
       }
     }
@@ -149,7 +128,7 @@ __global__ __launch_bounds__(256) void k_spmv(
             const long long pg = es + pq;
             cc[q] = cptr[pg];
             if (MAT == MAT_A) { mr[q] = v0[pg]; mi[q] = fk * v1[pg]; }
-            else { mr[q] = gv[2 * pg]; mi[q] = gv[2 * pg + 1]; }
+            else { mr[q] = v0[2 * pg]; mi[q] = v0[2 * pg + 1]; }
           }
           if (p + q >= s1) { mr[q] = T(0); mi[q] = T(0); }
         }
@@ -164,49 +143,89 @@ __global__ __launch_bounds__(256) void k_spmv(
             ai[e] += mr[q] * xi[q][e] + mi[q] * xr[q][e];
           }
       }
-      const size_t off = ((gbase + row) * NSP + c0) * 2;
-      if (AXPBY) {
-        T pr[CPL], pi[CPL];
-        Pack<T>::load(y + off, pr, pi);
-        if (EPI == EPI_AXPBY_X) {
-          T xr[CPL], xi[CPL];
-          Pack<T>::load(xacc + off, xr, xi);
-#pragma unroll
-          for (int e = 0; e < CPL; ++e) {
-            xr[e] += c2r[e] * pr[e] - c2i[e] * pi[e];
-            xi[e] += c2r[e] * pi[e] + c2i[e] * pr[e];
-          }
-          Pack<T>::store(xacc + off, xr, xi);
-        }
+      epi(row, ar, ai);
+    }
+  }
+}
+
+// 6 waves per SIMD (at most 80 VGPRs) measured best for U = 8.
+template <typename T, int NSP, int MAT, int EPI, int U>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(U <= 8 ? 6 : 4))) void k_spmv(
+    const int* __restrict__ rptr, const int* __restrict__ cptr, const T* __restrict__ v0,
+    const T* __restrict__ v1, long long nnz, const double* __restrict__ fscale,
+    const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ w, const double* __restrict__ coef,
+    const double* __restrict__ coef2, T* __restrict__ xacc, double* __restrict__ part1,
+    double* __restrict__ part2, int nn, int ncb, int g0, int nbx, int tile_rows,
+    const int* __restrict__ grp_active) {
+  using G = Geo<T, NSP>;
+  constexpr int CPL = G::CPL;
+  const int g = blockIdx.y, bx = blockIdx.x;
+  if (grp_active && !grp_active[g]) return;
+  __shared__ double sh[2 * 4 * NSP];
+  __shared__ int srp[kChunkRows + 1];
+  __shared__ int scol[kChunkCap];
+  __shared__ T sval[2 * kChunkCap];
+  const int lane = threadIdx.x & 63;
+  const int c0 = (lane % G::LPR) * CPL;
+  const int f = (g0 + g) / ncb;
+  T fk = T(0);
+  const T* gv = v0;
+  if (MAT == MAT_A) fk = (T)fscale[f];
+  else gv = v0 + 2 * (long long)f * nnz;
+  const size_t gbase = (size_t)g * nn;
+  Acc<CPL> d1, d2;
+  d1.zero();
+  d2.zero();
+  constexpr bool AXPBY = EPI == EPI_AXPBY || EPI == EPI_AXPBY_X;
+  T cr[CPL], ci[CPL], c2r[CPL], c2i[CPL];
+  if (AXPBY) load_coef<T, CPL>(coef, (long long)g * NSP + c0, cr, ci);
+  if (EPI == EPI_AXPBY_X) load_coef<T, CPL>(coef2, (long long)g * NSP + c0, c2r, c2i);
+  const int tile0 = xcd_tile(bx) * tile_rows;
+  const int tile1 = min(tile0 + tile_rows, nn);
+  spmv_rows<T, NSP, MAT, U>(rptr, cptr, gv, v1, fk, x + (gbase * NSP + c0) * 2, tile0, tile1, srp, scol, sval,
+                            [&](int row, T* ar, T* ai) {
+    const size_t off = ((gbase + row) * NSP + c0) * 2;
+    if (AXPBY) {
+      T pr[CPL], pi[CPL];
+      Pack<T>::load(y + off, pr, pi);
+      if (EPI == EPI_AXPBY_X) {
+        T xr[CPL], xi[CPL];
+        Pack<T>::load(xacc + off, xr, xi);
 #pragma unroll
         for (int e = 0; e < CPL; ++e) {
-          ar[e] += cr[e] * pr[e] - ci[e] * pi[e];
-          ai[e] += cr[e] * pi[e] + ci[e] * pr[e];
+          xr[e] += c2r[e] * pr[e] - c2i[e] * pi[e];
+          xi[e] += c2r[e] * pi[e] + c2i[e] * pr[e];
         }
+        Pack<T>::store(xacc + off, xr, xi);
       }
-      if (EPI != EPI_DOT2) Pack<T>::store(y + off, ar, ai);
-      if (EPI == EPI_STORE_SELF) {
 #pragma unroll
-        for (int e = 0; e < CPL; ++e) {
-          d1.re[e] += (double)ar[e] * ar[e] - (double)ai[e] * ai[e];
-          d1.im[e] += 2.0 * (double)ar[e] * ai[e];
-        }
+      for (int e = 0; e < CPL; ++e) {
+        ar[e] += cr[e] * pr[e] - ci[e] * pi[e];
+        ai[e] += cr[e] * pi[e] + ci[e] * pr[e];
       }
-      if (EPI == EPI_STORE_DOT || EPI == EPI_DOT2) {
-        T wr[CPL], wi[CPL];
-        Pack<T>::load(w + off, wr, wi);
+    }
+    if (EPI != EPI_DOT2) Pack<T>::store(y + off, ar, ai);
+    if (EPI == EPI_STORE_SELF) {
 #pragma unroll
-        for (int e = 0; e < CPL; ++e) {
-          d1.re[e] += (double)wr[e] * ar[e] - (double)wi[e] * ai[e];
-          d1.im[e] += (double)wr[e] * ai[e] + (double)wi[e] * ar[e];
-          if (EPI == EPI_DOT2) {
-            d2.re[e] += (double)ar[e] * ar[e] - (double)ai[e] * ai[e];
-            d2.im[e] += 2.0 * (double)ar[e] * ai[e];
-          }
+      for (int e = 0; e < CPL; ++e) {
+        d1.re[e] += (double)ar[e] * ar[e] - (double)ai[e] * ai[e];
+        d1.im[e] += 2.0 * (double)ar[e] * ai[e];
+      }
+    }
+    if (EPI == EPI_STORE_DOT || EPI == EPI_DOT2) {
+      T wr[CPL], wi[CPL];
+      Pack<T>::load(w + off, wr, wi);
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) {
+        d1.re[e] += (double)wr[e] * ar[e] - (double)wi[e] * ai[e];
+        d1.im[e] += (double)wr[e] * ai[e] + (double)wi[e] * ar[e];
+        if (EPI == EPI_DOT2) {
+          d2.re[e] += (double)ar[e] * ar[e] - (double)ai[e] * ai[e];
+          d2.im[e] += 2.0 * (double)ar[e] * ai[e];
         }
       }
     }
-  }
+  });
   if (EPI != EPI_STORE && !AXPBY) {
     block_reduce_store<T, NSP>(d1, sh, part1, (long long)g * nbx + bx);
     if (EPI == EPI_DOT2) {
@@ -394,10 +413,11 @@ __global__ __launch_bounds__(256) void k_init(const T* __restrict__ b, T* __rest
   block_reduce_store<T, NSP>(d2, sh, part2, (long long)g * nbx + bx);
 }
 
-// FP64 refinement residual: r64 = b - A x64 (complex128).  When `write`,
-// stores complex64 r, rhat, p = r64 and x32 = 0 for columns whose outer
-// status is active (zeros elsewhere); part1 = sum |r64|^2 (FP64),
-// part2 = sum r32 r32.
+// FP64 refinement residual: r64 = b - A x64 (complex128), with A x64 from
+// the staged row products of k_spmv (x64 == nullptr: x64 = 0, r64 = b).
+// When `write`, stores complex64 r, p (and rhat, if given) = r64 and x32 = 0
+// for columns whose outer status is active (zeros elsewhere);
+// part1 = sum |r64|^2 (FP64), part2 = sum r32 r32.
 template <int NSP>
 __global__ __launch_bounds__(256) void k_residual64(
     const int* __restrict__ rptr, const int* __restrict__ cptr, const double* __restrict__ are,
@@ -407,37 +427,24 @@ __global__ __launch_bounds__(256) void k_residual64(
     const int* __restrict__ ostatus, int write, double* __restrict__ part1,
     double* __restrict__ part2, int nn, int ncb, int g0, int nbx, int tile_rows) {
   using G = Geo<double, NSP>;  // one complex128 per lane
-  using GF = Geo<float, NSP>;
-  (void)sizeof(GF);
   const int g = blockIdx.y, bx = blockIdx.x;
-  const int* grp_active = nullptr;
-  (void)grp_active;
   __shared__ double sh[2 * 4 * NSP];
+  __shared__ int srp[kChunkRows + 1];
+  __shared__ int scol[kChunkCap];
+  __shared__ double sval[2 * kChunkCap];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int rsub = lane / G::LPR, c = lane % G::LPR;
+  const int c = lane % G::LPR;
   const long long rhs = (long long)g * NSP + c;
   const bool act = ostatus ? (ostatus[rhs] == ST_ACTIVE) : true;
   const int f = (g0 + g) / ncb;
-  const double fk = fscale[f];
   const size_t gbase = (size_t)g * nn;
   Acc<1> d1, d2;
   d1.zero();
   d2.zero();
-  for (int st = 0; st < tile_rows / G::RPB; ++st) {
-    const int row = xcd_tile(bx) * tile_rows + st * G::RPB + wv * G::RPW + rsub;
-    if (row >= nn) break;
+  auto epi = [&](int row, const double* yr, const double* yi) {
     const size_t off = ((gbase + row) * NSP + c);
-    double yr = 0.0, yi = 0.0;
-    const int s0 = rptr[row], s1 = rptr[row + 1];
-    for (int p = s0; p < s1; ++p) {
-      const int cc = cptr[p];
-      const double mr = are[p], mi = fk * aim[p];
-      const double2 xv = *reinterpret_cast<const double2*>(x64 + 2 * ((gbase + cc) * NSP + c));
-      yr += mr * xv.x - mi * xv.y;
-      yi += mr * xv.y + mi * xv.x;
-    }
     const double2 bv = *reinterpret_cast<const double2*>(b + 2 * off);
-    double rr = bv.x - yr, ri = bv.y - yi;
+    double rr = bv.x - yr[0], ri = bv.y - yi[0];
     if (!act) { rr = 0.0; ri = 0.0; }
     d1.re[0] += rr * rr + ri * ri;
     if (write) {
@@ -446,10 +453,19 @@ __global__ __launch_bounds__(256) void k_residual64(
       d2.im[0] += 2.0 * (double)fr * fi;
       const float2 v = make_float2(fr, fi);
       reinterpret_cast<float2*>(r32)[off] = v;
-      reinterpret_cast<float2*>(rhat32)[off] = v;
+      if (rhat32) reinterpret_cast<float2*>(rhat32)[off] = v;
       reinterpret_cast<float2*>(p32)[off] = v;
       reinterpret_cast<float2*>(x32)[off] = make_float2(0.f, 0.f);
     }
+  };
+  const int tile0 = xcd_tile(bx) * tile_rows;
+  const int tile1 = min(tile0 + tile_rows, nn);
+  if (x64) {
+    spmv_rows<double, NSP, MAT_A, 8>(rptr, cptr, are, aim, fscale[f], x64 + (gbase * NSP + c) * 2, tile0, tile1,
+                                     srp, scol, sval, epi);
+  } else {
+    const double z[1] = {0.0};
+    for (int row = tile0 + wv * G::RPW + lane / G::LPR; row < tile1; row += G::RPB) epi(row, z, z);
   }
   block_reduce_store<double, NSP>(d1, sh, part1, (long long)g * nbx + bx);
   if (write) {
@@ -1516,9 +1532,9 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
     for (int sweep = 0;; ++sweep) {
       C.pbegin();
       k_residual64<NSP><<<dim3(S->nbx, gc), 256, 0, st>>>(
-          S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), S->fscale.as<double>(), bc, xc,
-          V.r, V.rhat, V.p, V.x, sweep == 0 ? nullptr : R.status, 1, C.p1, C.p2, S->nn, S->ncb, g0,
-          S->nbx, S->tile_rows);
+          S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), S->fscale.as<double>(), bc,
+          sweep == 0 ? nullptr : xc, V.r, S->method == DOGPUP_METHOD_COCG ? nullptr : V.rhat, V.p, V.x,
+          sweep == 0 ? nullptr : R.status, 1, C.p1, C.p2, S->nn, S->ncb, g0, S->nbx, S->tile_rows);
       DP_LAUNCH_CHECK();
       C.pend(3, (sweep == 0 ? gc : S->active_groups) * e64 * (16 + 16 + 4 * 8) +
                     (double)S->nnzA * 20 + 4.0 * (S->nn + 1));
