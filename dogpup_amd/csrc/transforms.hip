@@ -13,9 +13,24 @@
 // Jacobian (dMesh.m:206-247 followed by fc2tg, wghtd_LM_recon.m:58-61):
 //   J_TG(m, g, v) = scale(m,g) * sum_k Re( C_gk * phi(v,k,s_m) * phiA(v,k,d_m) )
 // with C_gk = -2 Nt w_k conj(Gate_gk) dxyz^3 / s_fpsf(k) supplied by the host.
-// One block per node stages that node's forward and adjoint fluence of every
-// frequency in LDS (complex64) and every lane produces whole (m, g) rows, so
-// the 282 GB complex Jacobian of the reference is never materialised.
+// The 282 GB complex Jacobian of the reference is never materialised.
+//
+// MFMA form (k_jtg_mfma, up to 32 distinct source and 32 distinct detector
+// columns): per node and gate the link block is a real GEMM
+//   J_g[d][s] = sum_kk A_g[d][kk] B[kk][s],  kk = (k, re/im),
+//   A_g[d][(k,0)] = Re(C_gk psi_dk), A_g[d][(k,1)] = -Im(C_gk psi_dk),
+//   B[(k,0)][s] = Re phi_sk,         B[(k,1)][s] = Im phi_sk,
+// i.e. (32 x 2NF) x (2NF x 32) per gate, on v_mfma_f32_16x16x4f32 with FP32
+// accumulation.  Two wavefronts own one node (16 detector rows each): they
+// stream the node's fluence two frequencies (one K step of 4) at a time
+// straight into the MFMA operand registers and keep all gates' accumulator
+// tiles in registers.  Links
+// are then picked out of the 32x32 block through an (d, s) -> m table.
+// Scalar form (k_jacobian_tg, any link list): one block per node stages that
+// node's fluence of every frequency in LDS (complex64) and every lane
+// produces whole (m, g) rows.
+#include <vector>
+
 #include "common.hpp"
 
 namespace dogpup {
@@ -198,6 +213,112 @@ __global__ __launch_bounds__(256) void k_jacobian_tg(
   }
 }
 
+// F[idx] if ok, else 0, as an unconditional global load (index 0 when not
+// ok) and a select: a conditional load would be compiled as a select between
+// the global address and a zero kept in scratch memory.
+__device__ inline double2 load_or_zero(const double2* __restrict__ F, long long idx, bool ok) {
+  const double2 v = F[ok ? idx : 0];
+  return make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+}
+
+// MFMA J-TG.  tab = [dcol(32) | scol(32) | mlink(32x32)]: field columns of
+// the rows (detectors) and columns (sources) of the link block (-1 =
+// padding) and mlink[i*32 + j] = m of the link (source scol[j], detector
+// dcol[i]) or -1; coef: complex128 NG x NF.  Gates [gb, gb + NGB) in this
+// launch.  Two wavefronts per node (detector rows 0-15 and 16-31), so a
+// wavefront keeps NGB x 2 accumulator tiles; 4 wavefronts per block.
+template <int NGB>
+__global__ __launch_bounds__(256) void k_jtg_mfma(const double* __restrict__ field,
+                                                  const double* __restrict__ coef,
+                                                  const int* __restrict__ tab, const float* __restrict__ scale,
+                                                  float* __restrict__ out, int nn, int nf, int ng, int gb,
+                                                  int ncols, int nsp, int nm, int nv) {
+  __shared__ float2 sc[NGB * 128];   // C_gk, k < 128
+  __shared__ int sd[32], ss[32];
+  __shared__ int sm[32 * 32];
+  const int nfp = (nf + 1) & ~1;
+  for (int i = threadIdx.x; i < NGB * nfp; i += 256) {
+    const int g = i / nfp, k = i % nfp;
+    float2 c = make_float2(0.f, 0.f);
+    if (gb + g < ng && k < nf)
+      c = make_float2((float)coef[2 * ((gb + g) * nf + k)], (float)coef[2 * ((gb + g) * nf + k) + 1]);
+    sc[g * nfp + k] = c;
+  }
+  if (threadIdx.x < 32) {
+    sd[threadIdx.x] = tab[threadIdx.x];
+    ss[threadIdx.x] = tab[32 + threadIdx.x];
+  }
+  for (int i = threadIdx.x; i < 1024; i += 256) sm[i] = tab[64 + i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int v = blockIdx.x * 2 + (wv >> 1), t = wv & 1;
+  if (v >= nv) return;
+  const int i16 = lane & 15, kk = lane >> 4, kf = kk >> 1, im = kk & 1;
+  const int ncb = (ncols + nsp - 1) / nsp;
+  // operand columns of this lane: detector t*16 + i16 (A row), sources
+  // u*16 + i16 (B columns)
+  const int cd = sd[t * 16 + i16];
+  const long long offd = cd >= 0 ? ((long long)(cd / nsp) * nn + v) * nsp + cd % nsp : -1;
+  long long offs[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int cs = ss[u * 16 + i16];
+    offs[u] = cs >= 0 ? ((long long)(cs / nsp) * nn + v) * nsp + cs % nsp : -1;
+  }
+  const long long kstride = (long long)ncb * nn * nsp;   // complex elements per frequency
+  const double2* F = reinterpret_cast<const double2*>(field);
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  f4 acc[NGB][2];
+#pragma unroll
+  for (int g = 0; g < NGB; ++g)
+#pragma unroll
+    for (int u = 0; u < 2; ++u) acc[g][u] = f4{0.f, 0.f, 0.f, 0.f};
+  double2 pd, ps0, ps1;
+  {
+    const bool in = kf < nf;
+    pd = load_or_zero(F, kf * kstride + offd, in && offd >= 0);
+    ps0 = load_or_zero(F, kf * kstride + offs[0], in && offs[0] >= 0);
+    ps1 = load_or_zero(F, kf * kstride + offs[1], in && offs[1] >= 0);
+  }
+  for (int k0 = 0; k0 < nfp; k0 += 2) {
+    // prefetch the next K step (two frequencies ahead of this lane's)
+    const int kn = k0 + 2 + kf;
+    const bool in = kn < nf;
+    const double2 npd = load_or_zero(F, kn * kstride + offd, in && offd >= 0);
+    const double2 nps0 = load_or_zero(F, kn * kstride + offs[0], in && offs[0] >= 0);
+    const double2 nps1 = load_or_zero(F, kn * kstride + offs[1], in && offs[1] >= 0);
+    const int k = k0 + kf;
+    const float b0 = im ? (float)ps0.y : (float)ps0.x;
+    const float b1 = im ? (float)ps1.y : (float)ps1.x;
+    const float dr = (float)pd.x, di = (float)pd.y;
+#pragma unroll
+    for (int g = 0; g < NGB; ++g) {
+      const float2 c = sc[g * nfp + k];
+      const float a = im ? -(c.x * di + c.y * dr) : (c.x * dr - c.y * di);
+      acc[g][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc[g][0], 0, 0, 0);
+      acc[g][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc[g][1], 0, 0, 0);
+    }
+    pd = npd;
+    ps0 = nps0;
+    ps1 = nps1;
+  }
+  // accumulator register r of lane l holds D[4*(l/16) + r][l%16] of a tile
+#pragma unroll
+  for (int g = 0; g < NGB; ++g) {
+    if (gb + g < ng) {
+      float* o = out + ((long long)v * ng + gb + g) * nm;
+      const float* scg = scale ? scale + (long long)(gb + g) * nm : nullptr;
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = sm[(t * 16 + 4 * kk + r) * 32 + u * 16 + i16];
+          if (m >= 0) o[m] = scg ? acc[g][u][r] * scg[m] : acc[g][u][r];
+        }
+    }
+  }
+}
+
 }  // namespace dogpup
 
 using namespace dogpup;
@@ -243,10 +364,47 @@ int dogpup_jacobian_tg(const double* d_field, const double* d_coef, const int* d
   DP_ARG(d_field && d_coef && d_link && d_out, "NULL argument");
   DP_ARG(nn > 0 && nf > 0 && ng > 0 && ncols > 0 && nsp > 0 && nm > 0 && nv > 0 && nv <= nn,
          "bad dimensions");
+  hipStream_t st = as_stream(stream);
+  // MFMA form when the links span at most 32 source and 32 detector columns
+  // without repeated pairs (the table is built from a host copy of link)
+  if (nf <= 128 && !getenv("DOGPUP_JTG_SCALAR")) {
+    std::vector<int> hl((size_t)2 * nm);
+    DP_HIP(hipMemcpyAsync(hl.data(), d_link, hl.size() * 4, hipMemcpyDeviceToHost, st));
+    DP_HIP(hipStreamSynchronize(st));
+    std::vector<int> tab(64 + 1024, -1);
+    int nd = 0, ns = 0;
+    bool ok = true;
+    auto slot = [&](int* cols, int& n, int c) -> int {
+      for (int i = 0; i < n; ++i)
+        if (cols[i] == c) return i;
+      if (n == 32) return -1;
+      cols[n] = c;
+      return n++;
+    };
+    for (int m = 0; m < nm && ok; ++m) {
+      const int s = hl[2 * m], d = hl[2 * m + 1];
+      DP_ARG(s >= 0 && s < ncols && d >= 0 && d < ncols, "link %d names a column outside the field", m);
+      const int j = slot(tab.data() + 32, ns, s), i = slot(tab.data(), nd, d);
+      if (i < 0 || j < 0 || tab[64 + i * 32 + j] >= 0) ok = false;
+      else tab[64 + i * 32 + j] = m;
+    }
+    if (ok) {
+      int* d_tab = nullptr;
+      DP_HIP(hipMalloc((void**)&d_tab, tab.size() * 4));
+      DP_HIP(hipMemcpyAsync(d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, st));
+      for (int gb = 0; gb < ng; gb += 8) {
+        k_jtg_mfma<8><<<div_up(nv, 2), 256, 0, st>>>(d_field, d_coef, d_tab, d_scale, d_out, nn, nf, ng, gb, ncols,
+                                                     nsp, nm, nv);
+        DP_LAUNCH_CHECK();
+      }
+      DP_HIP(hipStreamSynchronize(st));   // the table is freed after the kernels used it
+      DP_HIP(hipFree(d_tab));
+      return DOGPUP_OK;
+    }
+  }
   DP_ARG(ng <= 16, "jacobian_tg supports at most 16 gates");
   const size_t shm = (size_t)(nf * ncols + ng * nf) * sizeof(float2);
   DP_ARG(shm <= 60 * 1024, "nf*(ncols+ng) too large for one node's LDS stage (%zu bytes)", shm);
-  hipStream_t st = as_stream(stream);
   if (ng <= 8)
     k_jacobian_tg<8><<<nv, 256, shm, st>>>(d_field, d_coef, d_link, d_scale, d_out, nn, nf, ng, ncols,
                                            nsp, nm);

@@ -132,7 +132,7 @@ class AbsorptionTG(torch.autograd.Function):
     @staticmethod
     def forward(ctx, mua, mesh, gate_fc, exact=False, mode="fp64", tol=1e-12):
         mesh.update_properties(np.stack([mua.detach().cpu().numpy(), mesh.musp], 1), kappa=mesh.kappa.copy())
-        field = mesh.solve_point_sources("sd", mode=mode, tol=tol, method=method)
+        field = mesh.solve_point_sources("sd", mode=mode, tol=tol)
         ctx.mesh, ctx.field, ctx.gate_fc, ctx.exact = mesh, field, gate_fc, exact
         ctx.in_device, ctx.in_dtype = mua.device, mua.dtype
         return tg_data(mesh, field, gate_fc)

@@ -211,6 +211,44 @@ def test_jacobian_tg(setup):
     assert rel(np.transpose(J2.cpu().numpy(), (2, 1, 0)), ref) < 1e-5
 
 
+@pytest.mark.parametrize("ng,nf", [(11, 7), (3, 64)])
+def test_jacobian_tg_mfma_vs_scalar_and_fp64(monkeypatch, ng, nf):
+    """The MFMA J-TG kernel (link block as a 32x32 real GEMM per node and
+    gate) against the scalar kernel and an FP64 einsum, on random fields with
+    a shuffled partial link list, several column blocks, an odd frequency
+    count, gate blocks of 8 and a Rytov-type scale."""
+    _need_gpu()
+    from dogpup_amd import _lib
+    rng = np.random.default_rng(5)
+    nn, nsp, ncols = 1000, 8, 40
+    ncb = ncols // nsp
+    field = rng.standard_normal((nf * ncb, nn, nsp)) + 1j * rng.standard_normal((nf * ncb, nn, nsp))
+    F = field.reshape(nf, ncb, nn, nsp).transpose(0, 1, 3, 2).reshape(nf, ncols, nn)   # [k][col][v]
+    scol = rng.permutation(ncols)[:20]
+    dcol = rng.permutation(ncols)[:25]
+    pairs = np.array([(a, b) for a in scol for b in dcol])
+    link = pairs[rng.permutation(len(pairs))[:300]].astype(np.int32)
+    nm = link.shape[0]
+    coef = rng.standard_normal((ng, nf)) + 1j * rng.standard_normal((ng, nf))
+    scale = rng.uniform(0.5, 2.0, (ng, nm)).astype(np.float32)
+    ref = np.einsum("gk,kmv->vgm", coef, F[:, link[:, 0], :] * F[:, link[:, 1], :]).real * scale[None]
+    print(f"[case] jtg mfma ng {ng} nf {nf}", flush=True)
+    outs = {}
+    for flag in ("0", "1"):
+        if flag == "1":
+            monkeypatch.setenv("DOGPUP_JTG_SCALAR", "1")
+        out = torch.empty((nn, ng, nm), dtype=torch.float32, device="cuda")
+        _lib.call("dogpup_jacobian_tg", _lib.ptr(torch.view_as_real(dev(field, torch.complex128))),
+                  _lib.ptr(torch.view_as_real(dev(coef, torch.complex128))), _lib.ptr(dev(link, torch.int32)),
+                  _lib.ptr(dev(scale, torch.float32)), _lib.ptr(out), nn, nf, ng, ncols, nsp, nm, nn,
+                  _lib.stream_ptr())
+        outs[flag] = out.cpu().numpy()
+    monkeypatch.delenv("DOGPUP_JTG_SCALAR")
+    assert rel(outs["0"], ref) < 1e-5
+    assert rel(outs["1"], ref) < 1e-5
+    assert rel(outs["0"], outs["1"]) < 1e-5
+
+
 def test_autograd_vjp_vs_fd_and_reference_jacobian(setup):
     """AbsorptionTG: exact-mode gradient vs central finite differences in
     nodal mua (kappa fixed); nodal-mode gradient vs the contraction of the
