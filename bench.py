@@ -35,7 +35,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0
+HBM_PEAK_GBS = 8000.0            # MI355X HBM3E
+FP32_MFMA_PEAK_TFLOPS = 157.3    # MI355X dense FP32 matrix
 
 
 def log(msg):
@@ -62,9 +63,10 @@ def build_workload(small=False):
     return D, opt, t
 
 
-def timed_solves(D, which, mode, tol, steps, warmup, profile=False, method="bicgstab"):
-    """Run warmup + timed steps of (FSAI setup + solve); returns (seconds per
-    step, field of the last step, per-class profile or None)."""
+def timed_solves(D, which, mode, tol, steps, warmup, profile=False, method="bicgstab", post=None):
+    """Run warmup + timed steps of (FSAI setup + solve [+ post(field)]);
+    returns (seconds per step, field of the last step, per-class profile or
+    None)."""
     import torch
     from dogpup_amd import _lib
     ncols = sum(len((D.optode.s_bary if w == "s" else D.optode.d_bary)[0]) for w in which)
@@ -73,6 +75,8 @@ def timed_solves(D, which, mode, tol, steps, warmup, profile=False, method="bicg
     for _ in range(warmup):
         S._version = None
         phi = D.solve_point_sources(which, mode=mode, tol=tol, method=method)
+        if post:
+            post(phi)
     if profile:
         _lib.call("dogpup_solver_set_profile", S.h, 1)
     torch.cuda.synchronize()
@@ -80,6 +84,8 @@ def timed_solves(D, which, mode, tol, steps, warmup, profile=False, method="bicg
     for _ in range(steps):
         S._version = None                       # FSAI setup is part of every step
         phi = D.solve_point_sources(which, mode=mode, tol=tol, method=method)
+        if post:
+            post(phi)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     prof = None
@@ -230,10 +236,18 @@ def main(argv=None):
     nrhs = ns * opt.Nf
     extra = {}
 
-    # headline: mixed precision, timed with per-launch events
-    log("mixed solve (timed)")
+    # headline: mixed precision solve followed by fc2td of the nodal fluence
+    # to 256 bins of 20 ps, timed with per-launch events
+    from dogpup_amd import transforms as tr
+    t_td = np.arange(256) * 20e-12
+    td_out = {}
+
+    def to_td(phi):
+        td_out["td"] = tr.fc2td_field(phi, opt.fAxis, t_td)
+
+    log("mixed solve + fc2td (timed)")
     dt_mixed, phi_mixed, prof = timed_solves(D, "s", "mixed", a.tol, a.steps, a.warmup, profile=True,
-                                             method=a.method)
+                                             method=a.method, post=to_td)
     value = nrhs / dt_mixed
     extra["mixed_s_per_solve"] = dt_mixed
     extra["mixed_max_iters"] = int(phi_mixed.iters.max())
@@ -241,6 +255,25 @@ def main(argv=None):
     extra["mixed_status"] = np.unique(phi_mixed.status).tolist()
     extra["mixed_relres_max"] = float(phi_mixed.relres.max())
     x_mixed = phi_mixed.data.clone()
+    # the DFT alone (MFMA roofline): 2 * NT * 2NF flops and 16 + 4 NT / NF
+    # bytes per (column, frequency)
+    ncol_td = phi_mixed.data.numel() // opt.Nf
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 3
+    e0.record()
+    for _ in range(reps):
+        tr.fc2td_field(phi_mixed, opt.fAxis, t_td)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_td = e0.elapsed_time(e1) / reps
+    flops_td = 2.0 * t_td.size * 2 * opt.Nf * ncol_td
+    bytes_td = ncol_td * (16.0 * opt.Nf + 4.0 * t_td.size)
+    extra["fc2td"] = {"ms": ms_td, "TFLOPs": flops_td / ms_td / 1e9, "mfma_peak_TFLOPs": FP32_MFMA_PEAK_TFLOPS,
+                      "mfma_frac": flops_td / ms_td / 1e9 / FP32_MFMA_PEAK_TFLOPS,
+                      "GBps": bytes_td / ms_td / 1e6, "columns": ncol_td, "bins": int(t_td.size),
+                      "note": "fc2td of the mixed-solve nodal fluence (32 sources, 269,001 nodes) to 256 bins "
+                              "of 20 ps; part of every timed step"}
+    del td_out["td"]
     release(D)
     if not a.no_alt:
         alt = "cocg" if a.method == "bicgstab" else "bicgstab"
@@ -334,7 +367,8 @@ def main(argv=None):
         "dtype": "c64+f64-refine", "data": "synthetic",
         "config": {"workload": "slab 80x80x40 mm, 1 mm Kuhn grid (269,001 nodes), 32 sources x 64 "
                                f"frequencies = 2048 RHS, FSAI-preconditioned {a.method.upper()} in complex64 "
-                               f"with FP64 refinement to a {a.tol:g} FP64 relative residual"
+                               f"with FP64 refinement to a {a.tol:g} FP64 relative residual, then fc2td of the "
+                               f"nodal fluence to 256 bins of 20 ps (FP32 MFMA)"
                    if not a.small else "slab 60x60x30 mm 2 mm grid, 16 sources x 8 frequencies",
                    "nodes": D.nn, "rhs": nrhs, "tol": a.tol, "method": a.method},
         "roofline": {"bound": "hbm", "kernel": names[cls], "achieved": achieved, "peak": HBM_PEAK_GBS,

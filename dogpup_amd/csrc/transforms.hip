@@ -15,6 +15,15 @@
 // with C_gk = -2 Nt w_k conj(Gate_gk) dxyz^3 / s_fpsf(k) supplied by the host.
 // The 282 GB complex Jacobian of the reference is never materialised.
 //
+// fc2td of a whole solver-layout field (k_fc2td_mfma): every column e of the
+// block [k][e] (e = (group, node, column) in the solver order) is a real
+// (NT x 2NF) x (2NF x 1) product, y[t][e] = sum_kk Wr[t][kk] X[kk][e] with
+// Wr[t][(k,0)] = 2 w_k cos(f_k t), Wr[t][(k,1)] = -2 w_k sin(f_k t) and
+// X[(k,0)][e] = Re F_k(e), X[(k,1)][e] = Im F_k(e): one GEMM over all columns
+// on v_mfma_f32_16x16x4f32 with FP32 accumulation.  A persistent 1024-thread
+// block per CU holds Wr (up to 256 time bins per pass) in LDS; a wavefront
+// keeps 64 bins of 16 columns in accumulator registers.
+//
 // MFMA form (k_jtg_mfma, up to 32 distinct source and 32 distinct detector
 // columns): per node and gate the link block is a real GEMM
 //   J_g[d][s] = sum_kk A_g[d][kk] B[kk][s],  kk = (k, re/im),
@@ -29,6 +38,7 @@
 // Scalar form (k_jacobian_tg, any link list): one block per node stages that
 // node's fluence of every frequency in LDS (complex64) and every lane
 // produces whole (m, g) rows.
+#include <algorithm>
 #include <vector>
 
 #include "common.hpp"
@@ -213,12 +223,96 @@ __global__ __launch_bounds__(256) void k_jacobian_tg(
   }
 }
 
-// F[idx] if ok, else 0, as an unconditional global load (index 0 when not
-// ok) and a select: a conditional load would be compiled as a select between
-// the global address and a zero kept in scratch memory.
-__device__ inline double2 load_or_zero(const double2* __restrict__ F, long long idx, bool ok) {
-  const double2 v = F[ok ? idx : 0];
-  return make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);
+// The MFMA kernels load operands unconditionally from clamped, in-bounds
+// indices: entries past the frequency range meet zero rows of the DFT/gate
+// coefficients, and padding rows/columns are never stored, so the finite
+// values they read contribute nothing.  (Selecting zero after the load would
+// make the compiler wait for every load at once and defeat the prefetch.)
+//
+// Their K loops are unrolled so that each prefetch register is reloaded right
+// after its own use and read again one unrolled iteration later: values
+// rotated through register copies would make the copy wait for the load
+// within the same step.  The values entering the loop pass through
+// `primed`: if every value reaching a loop-carried register were a plain
+// load, the optimiser would turn the register into a rotated address and
+// issue the load where the value is used.  (x + 0.0 is not simplified away,
+// since it maps -0.0 to +0.0, which changes no product.)  A lane that needs
+// only the real or only the imaginary part loads that double alone: picking
+// the component of a loaded pair by lane would put the pair in scratch memory.
+__device__ inline double2 primed(double2 v) { return make_double2(v.x + 0.0, v.y + 0.0); }
+__device__ inline double primed(double v) { return v + 0.0; }
+
+// Wr (float, ntp x kks, zero padded) for time bins t[t0 .. t0 + ntp) of fc2td.
+__global__ void k_build_wr(float* __restrict__ Wr, const double* __restrict__ f, const double* __restrict__ t,
+                           int t0, int nt, int nf, int ntp, int kks) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntp * kks) return;
+  const int r = i / kks, kk = i % kks, k = kk >> 1;
+  float v = 0.f;
+  if (t0 + r < nt && k < nf) {
+    double sn, cs;
+    sincos(f[k] * t[t0 + r], &sn, &cs);
+    const double w = (k == 0) ? 1.0 : 2.0;   // 2 w_k with the halved DC term
+    v = (float)((kk & 1) ? -w * sn : w * cs);
+  }
+  Wr[i] = v;
+}
+
+// y[t0 + t][e] = sum_kk Wr[t][kk] X[kk][e] for t < min(th * NTT * 16, nt - t0).
+// F: complex128 [nf][E]; out: float [nt][E].  A wavefront computes NTT time
+// tiles (16 bins each) of 16 columns; th (1, 2 or 4) wavefronts share a
+// column tile, one per block of NTT*16 time bins (so a wavefront needs few
+// enough registers for four K steps of X loads in flight).
+template <int NTT>
+__global__ __launch_bounds__(1024) void k_fc2td_mfma(const double* __restrict__ F, const float* __restrict__ Wr,
+                                                     float* __restrict__ out, long long E, int nf, int kks,
+                                                     int t0, int nt, int th) {
+  extern __shared__ float sW[];   // th*NTT*16 rows x kks
+  for (int i = threadIdx.x; i < th * NTT * 16 * kks; i += 1024) sW[i] = Wr[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int half = wv % th, slot = wv / th, slots = 16 / th;
+  const int i16 = lane & 15, kk = lane >> 4, kf = kk >> 1, im = kk & 1;
+  const int nfp8 = (nf + 7) & ~7;   // Wr holds zero columns up to 2*nfp8
+  const long long ntiles = (E + 15) / 16;
+  const float* wbase = sW + (half * NTT * 16 + i16) * kks + kk;
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  for (long long mt = (long long)blockIdx.x * slots + slot; mt < ntiles; mt += (long long)gridDim.x * slots) {
+    const long long e = mt * 16 + i16;
+    const bool oke = e < E;
+    f4 acc[NTT];
+#pragma unroll
+    for (int i = 0; i < NTT; ++i) acc[i] = f4{0.f, 0.f, 0.f, 0.f};
+    // four K steps (eight frequencies) per iteration; each X register is
+    // reloaded for the next iteration right after its use
+    const long long ec = oke ? e : E - 1;
+    const double* Xe = F + 2 * ec + im;      // this lane's component of column ec
+    double xq[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) xq[j] = primed(Xe[2 * (long long)min(2 * j + kf, nf - 1) * E]);
+    for (int k0 = 0; k0 < nfp8; k0 += 8) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float b = (float)xq[j];
+        xq[j] = Xe[2 * (long long)min(k0 + 8 + 2 * j + kf, nf - 1) * E];
+        __builtin_amdgcn_sched_barrier(0);   // issue the reload before this step's MFMAs
+        const float* wrow = wbase + 2 * (k0 + 2 * j);
+#pragma unroll
+        for (int i = 0; i < NTT; ++i)
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(wrow[i * 16 * kks], b, acc[i], 0, 0, 0);
+      }
+    }
+    // accumulator register r of lane l holds D[4*(l/16) + r][l%16]
+    if (oke) {
+#pragma unroll
+      for (int i = 0; i < NTT; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int tt = t0 + (half * NTT + i) * 16 + 4 * kk + r;
+          if (tt < nt) out[(long long)tt * E + e] = acc[i][r];
+        }
+    }
+  }
 }
 
 // MFMA J-TG.  tab = [dcol(32) | scol(32) | mlink(32x32)]: field columns of
@@ -233,16 +327,16 @@ __global__ __launch_bounds__(256) void k_jtg_mfma(const double* __restrict__ fie
                                                   const int* __restrict__ tab, const float* __restrict__ scale,
                                                   float* __restrict__ out, int nn, int nf, int ng, int gb,
                                                   int ncols, int nsp, int nm, int nv) {
-  __shared__ float2 sc[NGB * 128];   // C_gk, k < 128
+  __shared__ float2 sc[NGB * 128];   // C_gk (zero for k >= nf), row stride 128
   __shared__ int sd[32], ss[32];
   __shared__ int sm[32 * 32];
-  const int nfp = (nf + 1) & ~1;
-  for (int i = threadIdx.x; i < NGB * nfp; i += 256) {
-    const int g = i / nfp, k = i % nfp;
+  const int nfp4 = (nf + 3) & ~3;     // K loop: four frequencies per iteration
+  for (int i = threadIdx.x; i < NGB * 128; i += 256) {
+    const int g = i / 128, k = i % 128;
     float2 c = make_float2(0.f, 0.f);
     if (gb + g < ng && k < nf)
       c = make_float2((float)coef[2 * ((gb + g) * nf + k)], (float)coef[2 * ((gb + g) * nf + k) + 1]);
-    sc[g * nfp + k] = c;
+    sc[i] = c;
   }
   if (threadIdx.x < 32) {
     sd[threadIdx.x] = tab[threadIdx.x];
@@ -257,13 +351,14 @@ __global__ __launch_bounds__(256) void k_jtg_mfma(const double* __restrict__ fie
   const int ncb = (ncols + nsp - 1) / nsp;
   // operand columns of this lane: detector t*16 + i16 (A row), sources
   // u*16 + i16 (B columns)
-  const int cd = sd[t * 16 + i16];
-  const long long offd = cd >= 0 ? ((long long)(cd / nsp) * nn + v) * nsp + cd % nsp : -1;
+  // padding rows/columns (-1) read column 0; they are never stored
+  const int cd = max(sd[t * 16 + i16], 0);
+  const long long offd = ((long long)(cd / nsp) * nn + v) * nsp + cd % nsp;
   long long offs[2];
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
-    const int cs = ss[u * 16 + i16];
-    offs[u] = cs >= 0 ? ((long long)(cs / nsp) * nn + v) * nsp + cs % nsp : -1;
+    const int cs = max(ss[u * 16 + i16], 0);
+    offs[u] = ((long long)(cs / nsp) * nn + v) * nsp + cs % nsp;
   }
   const long long kstride = (long long)ncb * nn * nsp;   // complex elements per frequency
   const double2* F = reinterpret_cast<const double2*>(field);
@@ -273,34 +368,41 @@ __global__ __launch_bounds__(256) void k_jtg_mfma(const double* __restrict__ fie
   for (int g = 0; g < NGB; ++g)
 #pragma unroll
     for (int u = 0; u < 2; ++u) acc[g][u] = f4{0.f, 0.f, 0.f, 0.f};
-  double2 pd, ps0, ps1;
-  {
-    const bool in = kf < nf;
-    pd = load_or_zero(F, kf * kstride + offd, in && offd >= 0);
-    ps0 = load_or_zero(F, kf * kstride + offs[0], in && offs[0] >= 0);
-    ps1 = load_or_zero(F, kf * kstride + offs[1], in && offs[1] >= 0);
-  }
-  for (int k0 = 0; k0 < nfp; k0 += 2) {
-    // prefetch the next K step (two frequencies ahead of this lane's)
-    const int kn = k0 + 2 + kf;
-    const bool in = kn < nf;
-    const double2 npd = load_or_zero(F, kn * kstride + offd, in && offd >= 0);
-    const double2 nps0 = load_or_zero(F, kn * kstride + offs[0], in && offs[0] >= 0);
-    const double2 nps1 = load_or_zero(F, kn * kstride + offs[1], in && offs[1] >= 0);
-    const int k = k0 + kf;
-    const float b0 = im ? (float)ps0.y : (float)ps0.x;
-    const float b1 = im ? (float)ps1.y : (float)ps1.x;
-    const float dr = (float)pd.x, di = (float)pd.y;
+  // two K steps (four frequencies) per iteration, each operand register
+  // reloaded for the next iteration right after its use; frequencies past
+  // nf read frequency nf-1 and meet C = 0
+  // B operands: this lane's component (re or im) of the source fluence
+  const double* Fs0 = field + 2 * offs[0] + im;
+  const double* Fs1 = field + 2 * offs[1] + im;
+  double2 pd[2];
+  double ps0[2], ps1[2];
 #pragma unroll
-    for (int g = 0; g < NGB; ++g) {
-      const float2 c = sc[g * nfp + k];
-      const float a = im ? -(c.x * di + c.y * dr) : (c.x * dr - c.y * di);
-      acc[g][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc[g][0], 0, 0, 0);
-      acc[g][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc[g][1], 0, 0, 0);
+  for (int j = 0; j < 2; ++j) {
+    const long long ko = (long long)min(2 * j + kf, nf - 1) * kstride;
+    pd[j] = primed(F[ko + offd]);
+    ps0[j] = primed(Fs0[2 * ko]);
+    ps1[j] = primed(Fs1[2 * ko]);
+  }
+  for (int k0 = 0; k0 < nfp4; k0 += 4) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int k = k0 + 2 * j + kf;
+      const float b0 = (float)ps0[j];
+      const float b1 = (float)ps1[j];
+      const float dr = (float)pd[j].x, di = (float)pd[j].y;
+      const long long ko = (long long)min(k + 4, nf - 1) * kstride;
+      pd[j] = F[ko + offd];
+      ps0[j] = Fs0[2 * ko];
+      ps1[j] = Fs1[2 * ko];
+      __builtin_amdgcn_sched_barrier(0);   // issue the reloads before this step's MFMAs
+#pragma unroll
+      for (int g = 0; g < NGB; ++g) {
+        const float2 c = sc[g * 128 + k];
+        const float a = im ? -(c.x * di + c.y * dr) : (c.x * dr - c.y * di);
+        acc[g][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc[g][0], 0, 0, 0);
+        acc[g][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc[g][1], 0, 0, 0);
+      }
     }
-    pd = npd;
-    ps0 = nps0;
-    ps1 = nps1;
   }
   // accumulator register r of lane l holds D[4*(l/16) + r][l%16] of a tile
 #pragma unroll
@@ -337,6 +439,42 @@ int dogpup_fc2td(const double* d_in, double* d_out, const double* d_f, const dou
   DP_ARG(d_in && d_out && d_f && d_t && nt >= 1 && nf >= 1 && ncol >= 0, "bad arguments");
   return apply_w(W_FC2TD, d_in, d_out, d_f, d_t, nullptr, nt, nf, nt, ncol, 1, 1, 1.0,
                  as_stream(stream));
+}
+
+int dogpup_fc2td_field(const double* d_field, float* d_out, const double* d_f, const double* d_t, int nf, int nt,
+                       long long ncol, void* stream) {
+  DP_ARG(d_field && d_out && d_f && d_t && nf >= 1 && nf <= 128 && nt >= 1 && ncol >= 1, "bad arguments");
+  hipStream_t st = as_stream(stream);
+  const int nfp8 = (nf + 7) & ~7;
+  const int kks = ((2 * nfp8 + 63) / 64) * 64 + 4;  // row stride = 4 mod 64 banks: conflict-free A reads
+  int dev = 0, ncu = 0;
+  DP_HIP(hipGetDevice(&dev));
+  DP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  float* Wr = nullptr;
+  DP_HIP(hipMalloc((void**)&Wr, (size_t)256 * kks * 4));
+  const long long ntiles = (ncol + 15) / 16;
+  hipError_t e = hipSuccess;
+  for (int t0 = 0; t0 < nt && e == hipSuccess; t0 += 256) {
+    const int left = nt - t0;
+    // 256-bin passes: two wavefronts of 8 time tiles per column tile;
+    // shorter tails: one wavefront of 8 or 4 tiles
+    const int ntt = 4, th = left > 128 ? 4 : (left > 64 ? 2 : 1);
+    const int ntp = th * ntt * 16;
+    const size_t shm = (size_t)ntp * kks * 4;
+    k_build_wr<<<div_up((long long)ntp * kks, 256), 256, 0, st>>>(Wr, d_f, d_t, t0, nt, nf, ntp, kks);
+    e = hipGetLastError();
+    if (e != hipSuccess) break;
+    const long long slots = 16 / th;
+    const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(ncu, (ntiles + slots - 1) / slots));
+    e = hipFuncSetAttribute((const void*)k_fc2td_mfma<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (e == hipSuccess) k_fc2td_mfma<4><<<grid, 1024, shm, st>>>(d_field, Wr, d_out, ncol, nf, kks, t0, nt, th);
+    if (e == hipSuccess) e = hipGetLastError();
+  }
+  const hipError_t e2 = hipStreamSynchronize(st);   // Wr is freed after the kernels used it
+  (void)hipFree(Wr);
+  if (e != hipSuccess) return fail(DOGPUP_ERR_HIP, "fc2td_field launch: %s", hipGetErrorString(e));
+  if (e2 != hipSuccess) return fail(DOGPUP_ERR_HIP, "fc2td_field: %s", hipGetErrorString(e2));
+  return DOGPUP_OK;
 }
 
 int dogpup_fc2tg(const double* d_in, double* d_out, const double* d_gates, int nf, int ng, int ncol,

@@ -61,6 +61,30 @@ def fc2td(fun_fc, f, t, dim):
     return torch.movedim(out, 0, dim)
 
 
+def fc2td_field(field, f, t):
+    """fc2td over a whole solver-layout field (fc2td.m:24-31 along the
+    frequency axis of phi) on FP32 MFMA: returns float32 [NT][groups][NN][NSP]
+    (the time-domain field in the solver's column layout;
+    td_field_to_reference gives the reference's NN x NT x NCOLS)."""
+    f = _dev(f, torch.float64).reshape(-1)
+    t = _dev(t, torch.float64).reshape(-1)
+    if f.numel() != field.nf:
+        raise ValueError("Size along given dimension does not match given fourier coefficients")
+    gpf = field.data.shape[0] // field.nf                   # groups per frequency
+    ncol = gpf * field.data.shape[1] * field.data.shape[2]
+    out = torch.empty((t.numel(), gpf, field.data.shape[1], field.data.shape[2]), dtype=torch.float32,
+                      device=DEV)
+    _lib.call("dogpup_fc2td_field", _lib.ptr(torch.view_as_real(field.data)), _lib.ptr(out), _lib.ptr(f),
+              _lib.ptr(t), f.numel(), t.numel(), ncol, _lib.stream_ptr())
+    return out
+
+
+def td_field_to_reference(td, ncols):
+    """[NT][groups][NN][NSP] -> NN x NT x NCOLS (fc2td(phi, f, t, 2) layout)."""
+    nt, gpf, nn, nsp = td.shape
+    return td.permute(2, 0, 1, 3).reshape(nn, nt, gpf * nsp)[:, :, :ncols]
+
+
 def fc2tg(fun_fc, gate_fc, f, nt, dim):
     """y_g = Nt 2 Re sum_k conj(G_gk) F'_k, F'_0 = F_0/2 (fc2tg.m:25-32)."""
     src = fun_fc if torch.is_tensor(fun_fc) else torch.as_tensor(np.asarray(fun_fc))

@@ -211,6 +211,24 @@ def test_jacobian_tg(setup):
     assert rel(np.transpose(J2.cpu().numpy(), (2, 1, 0)), ref) < 1e-5
 
 
+def test_fc2td_field_mfma(setup):
+    """fc2td over a solved nodal field on FP32 MFMA against the FP64 oracle
+    fc2td(phi, f, t, 2), for 300 time bins (a 256-bin pass and a tail)."""
+    from dogpup_amd import transforms as tr
+    from oracle import optode as oopt, solver as osol, transforms as otr
+    D, A, om, opt = setup["D"], setup["A"], setup["om"], setup["opt"]
+    sid, sw = opt.s_bary
+    phi = osol.direct(A, oopt.source_matrix(om, sid, sw, opt.s_fpsf))       # NN x NF x NS
+    t = setup["t"][:300]
+    ref = otr.fc2td(phi, setup["f"], t, 1)                                     # NN x NT x NS
+    fw = D.solve_point_sources("s", mode="fp64")
+    print("[case] fc2td_field", flush=True)
+    td = tr.fc2td_field(fw, setup["f"], t)
+    got = tr.td_field_to_reference(td, fw.ncols).cpu().numpy()
+    assert got.shape == ref.shape
+    assert rel(got, ref) < 1e-5
+
+
 @pytest.mark.parametrize("ng,nf", [(11, 7), (3, 64)])
 def test_jacobian_tg_mfma_vs_scalar_and_fp64(monkeypatch, ng, nf):
     """The MFMA J-TG kernel (link block as a 32x32 real GEMM per node and
