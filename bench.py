@@ -290,14 +290,6 @@ def main(argv=None):
         release(D)
 
     names = ["G/G^T SpMV", "A SpMV", "Krylov vector passes", "FP64 refinement passes"]
-    # kernel whose PMC traffic ratio stands for each class (the full-activity
-    # launch of the class's most frequent kernel)
-    if a.method == "bicgstab":
-        pmc_kernel = {0: "dogpup::k_spmv<float, 32, 1, 0>", 1: "dogpup::k_spmv<float, 32, 0, 0>",
-                      2: "dogpup::k_update_xr<float, 32>"}
-    else:
-        pmc_kernel = {0: "dogpup::k_spmv<float, 32, 1, 4>", 1: "dogpup::k_spmv<float, 32, 0, 1>",
-                      2: "dogpup::k_cg_update<float, 32>"}
     cls = int(np.argmax(prof[:, 0]))
     ms, byts, nl = prof[cls]
     achieved = byts / (ms * 1e-3) / 1e9
@@ -310,13 +302,13 @@ def main(argv=None):
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{a.method}_c64_slab269k_2048rhs.json")
-    if not a.small and os.path.exists(tfile) and cls in pmc_kernel:
+    if not a.small and os.path.exists(tfile):
         with open(tfile) as fh:
-            tr = json.load(fh)["kernels"].get(pmc_kernel[cls])
+            tr = json.load(fh)["classes"].get(str(cls))
         if tr:
-            # traffic/algorithmic ratio of the full-activity launch (rocprofv3 PMC passes
-            # FETCH_SIZE and WRITE_SIZE, calibrated on a plain copy), applied to this
-            # run's average launch
+            # traffic/algorithmic ratio of the class's full-activity launches (rocprofv3
+            # PMC passes FETCH_SIZE and WRITE_SIZE, calibrated on a plain copy; made by
+            # python -m dogpup_amd.prof_traffic), applied to this run's average launch
             traffic = {"bytes_per_launch": tr["ratio"] * byts / max(nl, 1), "ratio": tr["ratio"],
                        "source": os.path.relpath(tfile, ROOT)}
     if not a.no_fp64:
