@@ -1,11 +1,15 @@
-// Batched FSAI-preconditioned BiCGSTAB over every (frequency, column) RHS.
+// Batched FSAI-preconditioned Krylov solves over every (mesh, frequency,
+// column) RHS.
 //
 // Algorithm (behaviour of DOGPUP/dMesh.m:1247-1348, gpuBicstab_FSAIP):
 // right-preconditioned BiCGSTAB on A_k x = b with M^-1 = G_k^T G_k, x0 = 0,
 // r_hat0 = r0, unconjugated bilinear forms, omega = (Gt . Gs)/(Gt . Gt)
-// weighted by G, convergence on ||s||/||b|| or ||r||/||b||.
+// weighted by G, convergence on ||s||/||b|| or ||r||/||b||.  The fused forms
+// can run COCG instead (CG with the unconjugated form, for the complex
+// symmetric A_k and symmetric G_k^T G_k): same x, three operator passes per
+// iteration instead of seven.
 //
-// Three forms share the CSR kernels of kernels.hpp:
+// Three forms share the CSR kernels:
 //  * DOGPUP_MODE_C128: complex128, fused passes (dots computed inside the
 //    SpMV/update that produces the operand), per-RHS convergence with
 //    groups skipped once all their columns have converged.
@@ -16,12 +20,20 @@
 //    separate G, G^T and A passes, separate dot/axpy passes with
 //    temporaries, the reciprocal step `alph`, lockstep convergence on the
 //    maximum relative residual checked by the host twice per iteration.
+//
+// Batches of meshes.  A handle holds one or more independent meshes whose
+// CSR patterns the caller concatenates into one block-diagonal pattern (mesh
+// i owns rows [row0_i, row0_{i+1})).  A group is one (mesh, frequency, block
+// of NSP columns); its vectors cover only its mesh's rows.  Every kernel
+// launch walks one flat list of work items (group, row tile) over all groups
+// of all meshes, so meshes of very different sizes fill one launch together.
 #include <cmath>
 #include <cstring>
 #include <cstdlib>
 #include <algorithm>
 #include <map>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "fsai.hpp"
@@ -37,15 +49,60 @@ enum { MAT_A = 0, MAT_G = 1 };
 enum { EPI_STORE = 0, EPI_STORE_DOT = 1, EPI_DOT2 = 2, EPI_STORE_SELF = 3, EPI_AXPBY = 4, EPI_AXPBY_X = 5 };
 enum { ST_ACTIVE = -1, ST_SKIP = -2 };
 
-// Row tiles are assigned so that each XCD sweeps one contiguous range of
-// tiles: workgroups are dispatched round-robin over the 8 XCDs, and
-// gridDim.x is a multiple of 8, so blockIdx.x % 8 names the XCD.  A tile's
-// stencil neighbours then sit in the same XCD's L2.
-__device__ inline int xcd_tile(int bx) { return (bx & 7) * (gridDim.x >> 3) + (bx >> 3); }
+// Work list.  Workgroups are dispatched in order, round-robin over the 8
+// XCDs, so block b runs on XCD b % 8.  Each group's items are padded to a
+// multiple of 8 and ordered so that XCD x sweeps the contiguous tile range
+// [x*n/8, (x+1)*n/8) of the group (a tile's stencil neighbours then sit in
+// the same XCD's L2), and the groups follow one another: all XCDs work on
+// the same group at a time, whose vectors stay in the Infinity Cache.
+// Padding items have tile -1 and do nothing.
+struct Work {
+  const int* wgrp;        // work item -> group
+  const int* wtile;       // work item -> row tile within its group (-1: padding)
+  const int* wpart;       // work item -> slot of its partial sums (dense over real items)
+  const int* grow0;       // group -> first row of its mesh
+  const int* grows;       // group -> rows of its mesh
+  const long long* gvb;   // group -> node offset of its vector block
+  const double* gfsc;     // group -> multiplier of Im A (frequency scale)
+  const long long* ggv;   // group -> G value offset: entry p of the pattern is at ggv + p
+  const int* gwbeg;       // group -> first partial-sum slot
+  const int* gwcnt;       // group -> partial-sum slots (its row tiles)
+  const int* gcols;       // group -> columns holding a RHS (the rest is padding)
+  int w0, nw;             // this launch's work items [w0, w0 + nw), nw a multiple of 8
+  int pw0;                // partial-sum slot of its first real item
+  int g0;                 // this launch's first group
+  long long vb0;          // node offset of that group's vector block
+  int tile_rows;
+};
+
+// The block's work item: global group g, chunk-local group gl and item wl,
+// its rows [row0, row1) and vb, the chunk-local node index of row 0 of its
+// mesh (chunk-local node index of row r = vb + r).
+struct Item {
+  int g, gl, wl, row0, row1;
+  long long vb;
+};
+
+__device__ inline bool work_item(const Work& W, Item& it) {
+  if ((int)blockIdx.x >= W.nw) return false;
+  const int w = W.w0 + blockIdx.x;
+  const int t = W.wtile[w];
+  if (t < 0) return false;
+  const int g = W.wgrp[w];
+  const int r0 = W.grow0[g];
+  it.g = g;
+  it.gl = g - W.g0;
+  it.wl = W.wpart[w] - W.pw0;
+  it.row0 = r0 + t * W.tile_rows;
+  it.row1 = min(it.row0 + W.tile_rows, r0 + W.grows[g]);
+  it.vb = W.gvb[g] - W.vb0 - r0;
+  return true;
+}
 
 // ------------------------------------------------------------------ SpMV ---
-// y = M x for every group; MAT_A: M = v0 + i*fscale[f]*v1 (planar, shared by
-// all frequencies); MAT_G: M = per-frequency complex values v0[f][nnz].
+// y = M x for every work item; MAT_A: M = v0 + i*gfsc[g]*v1 (planar, shared
+// by all frequencies of a mesh); MAT_G: M = complex values per (mesh,
+// frequency), entry p at v0[ggv[g] + p].
 //
 // The block walks its row tile in chunks of kChunkRows rows.  Each chunk's
 // CSR row pointers, column indices and values are first copied into LDS with
@@ -56,8 +113,8 @@ __device__ inline int xcd_tile(int bx) { return (bx & 7) * (gridDim.x >> 3) + (b
 // default: 6 waves per SIMD with 8 loads in flight each measured faster than
 // 4 or 16 on the 15-point A and 8-point G rows); the entries past the row's
 // end gather the row's last column again with weight zero, which keeps the
-// loads unconditional.  A chunk with more
-// than kChunkCap nonzeros reads the CSR arrays from global memory instead.
+// loads unconditional.  A chunk with more than kChunkCap nonzeros reads the
+// CSR arrays from global memory instead.
 // Per-lane column coefficients (complex double per RHS) as T.
 template <typename T, int CPL>
 __device__ inline void load_coef(const double* __restrict__ c, long long rhs0, T re[CPL], T im[CPL]) {
@@ -72,14 +129,15 @@ constexpr int kChunkRows = 64;
 constexpr int kChunkCap = 1024;
 
 // Row products of one row tile [tile0, tile1): for every row, (ar, ai) =
-// (M x)(row) over this lane's CPL columns, handed to epi(row, ar, ai).
+// (M x)(row) over this lane's CPL columns, handed to epi(row, ar, ai).  xg
+// points at this lane's first column of x; row r of the mesh is at node vb + r.
 // srp/scol/sval are the caller's LDS chunk buffers.  All threads of the block
 // call it (it synchronises).
 template <typename T, int NSP, int MAT, int U, typename Epi>
 __device__ inline void spmv_rows(const int* __restrict__ rptr, const int* __restrict__ cptr,
                                  const T* __restrict__ v0, const T* __restrict__ v1, T fk,
-                                 const T* __restrict__ xg, int tile0, int tile1, int* srp, int* scol, T* sval,
-                                 Epi&& epi) {
+                                 const T* __restrict__ xg, long long vb, int tile0, int tile1, int* srp,
+                                 int* scol, T* sval, Epi&& epi) {
   using G = Geo<T, NSP>;
   constexpr int CPL = G::CPL;
   static_assert(kChunkRows % G::RPB == 0, "chunk");
@@ -134,7 +192,7 @@ __device__ inline void spmv_rows(const int* __restrict__ rptr, const int* __rest
         }
         T xr[U][CPL], xi[U][CPL];
 #pragma unroll
-        for (int q = 0; q < U; ++q) Pack<T>::load(xg + (size_t)cc[q] * NSP * 2, xr[q], xi[q]);
+        for (int q = 0; q < U; ++q) Pack<T>::load(xg + (vb + cc[q]) * (NSP * 2), xr[q], xi[q]);
 #pragma unroll
         for (int q = 0; q < U; ++q)
 #pragma unroll
@@ -152,39 +210,36 @@ __device__ inline void spmv_rows(const int* __restrict__ rptr, const int* __rest
 template <typename T, int NSP, int MAT, int EPI, int U>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(U <= 8 ? 6 : 4))) void k_spmv(
     const int* __restrict__ rptr, const int* __restrict__ cptr, const T* __restrict__ v0,
-    const T* __restrict__ v1, long long nnz, const double* __restrict__ fscale,
-    const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ w, const double* __restrict__ coef,
-    const double* __restrict__ coef2, T* __restrict__ xacc, double* __restrict__ part1,
-    double* __restrict__ part2, int nn, int ncb, int g0, int nbx, int tile_rows,
-    const int* __restrict__ grp_active) {
+    const T* __restrict__ v1, const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ w,
+    const double* __restrict__ coef, const double* __restrict__ coef2, T* __restrict__ xacc,
+    double* __restrict__ part1, double* __restrict__ part2, Work W, const int* __restrict__ grp_active) {
   using G = Geo<T, NSP>;
   constexpr int CPL = G::CPL;
-  const int g = blockIdx.y, bx = blockIdx.x;
-  if (grp_active && !grp_active[g]) return;
+  Item it;
+  if (!work_item(W, it)) return;
+  if (grp_active && !grp_active[it.gl]) return;
   __shared__ double sh[2 * 4 * NSP];
   __shared__ int srp[kChunkRows + 1];
   __shared__ int scol[kChunkCap];
   __shared__ T sval[2 * kChunkCap];
   const int lane = threadIdx.x & 63;
   const int c0 = (lane % G::LPR) * CPL;
-  const int f = (g0 + g) / ncb;
   T fk = T(0);
   const T* gv = v0;
-  if (MAT == MAT_A) fk = (T)fscale[f];
-  else gv = v0 + 2 * (long long)f * nnz;
-  const size_t gbase = (size_t)g * nn;
+  if (MAT == MAT_A) fk = (T)W.gfsc[it.g];
+  else gv = v0 + 2 * W.ggv[it.g];
   Acc<CPL> d1, d2;
   d1.zero();
   d2.zero();
   constexpr bool AXPBY = EPI == EPI_AXPBY || EPI == EPI_AXPBY_X;
+  const long long rhs0 = (long long)it.gl * NSP + c0;
   T cr[CPL], ci[CPL], c2r[CPL], c2i[CPL];
-  if (AXPBY) load_coef<T, CPL>(coef, (long long)g * NSP + c0, cr, ci);
-  if (EPI == EPI_AXPBY_X) load_coef<T, CPL>(coef2, (long long)g * NSP + c0, c2r, c2i);
-  const int tile0 = xcd_tile(bx) * tile_rows;
-  const int tile1 = min(tile0 + tile_rows, nn);
-  spmv_rows<T, NSP, MAT, U>(rptr, cptr, gv, v1, fk, x + (gbase * NSP + c0) * 2, tile0, tile1, srp, scol, sval,
+  if (AXPBY) load_coef<T, CPL>(coef, rhs0, cr, ci);
+  if (EPI == EPI_AXPBY_X) load_coef<T, CPL>(coef2, rhs0, c2r, c2i);
+  const long long vb = it.vb;
+  spmv_rows<T, NSP, MAT, U>(rptr, cptr, gv, v1, fk, x + c0 * 2, vb, it.row0, it.row1, srp, scol, sval,
                             [&](int row, T* ar, T* ai) {
-    const size_t off = ((gbase + row) * NSP + c0) * 2;
+    const long long off = ((vb + row) * NSP + c0) * 2;
     if (AXPBY) {
       T pr[CPL], pi[CPL];
       Pack<T>::load(y + off, pr, pi);
@@ -227,29 +282,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(U <= 8 ? 6 
     }
   });
   if (EPI != EPI_STORE && !AXPBY) {
-    block_reduce_store<T, NSP>(d1, sh, part1, (long long)g * nbx + bx);
+    block_reduce_store<T, NSP>(d1, sh, part1, it.wl);
     if (EPI == EPI_DOT2) {
       __syncthreads();
-      block_reduce_store<T, NSP>(d2, sh, part2, (long long)g * nbx + bx);
+      block_reduce_store<T, NSP>(d2, sh, part2, it.wl);
     }
   }
 }
 
+// Row loop of the vector passes: every row of the block's work item, LPR
+// lanes per row, CPL columns per lane; `off` indexes the row's first column
+// of this lane and rhs0 its first RHS in the chunk's per-RHS arrays.
 #define ROW_LOOP_BEGIN                                                         \
   using G = Geo<T, NSP>;                                                       \
   constexpr int CPL = G::CPL;                                                  \
-  const int g = blockIdx.y, bx = blockIdx.x;                                   \
-  if (grp_active && !grp_active[g]) return;                                    \
+  Item it;                                                                     \
+  if (!work_item(W, it)) return;                                               \
+  if (grp_active && !grp_active[it.gl]) return;                                \
   __shared__ double sh[2 * 4 * NSP];                                           \
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;                    \
-  const int rsub = lane / G::LPR, cl = lane % G::LPR, c0 = cl * CPL;           \
-  const long long rhs0 = (long long)g * NSP + c0;                              \
-  const size_t gbase = (size_t)g * nn;                                         \
-  (void)sh; (void)rhs0;                                                        \
-  for (int st = 0; st < tile_rows / G::RPB; ++st) {                            \
-    const int row = xcd_tile(bx) * tile_rows + st * G::RPB + wv * G::RPW + rsub; \
-    if (row >= nn) break;                                                      \
-    const size_t off = ((gbase + row) * NSP + c0) * 2;
+  const int rsub = lane / G::LPR, c0 = (lane % G::LPR) * CPL;                  \
+  const long long rhs0 = (long long)it.gl * NSP + c0;                          \
+  (void)sh; (void)rhs0;
+
+#define ROW_LOOP_ROWS                                                          \
+  for (int row = it.row0 + wv * G::RPW + rsub; row < it.row1; row += G::RPB) { \
+    const long long off = ((it.vb + row) * NSP + c0) * 2;
 
 #define ROW_LOOP_END }
 
@@ -257,13 +315,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(U <= 8 ? 6 
 template <typename T, int NSP>
 __global__ __launch_bounds__(256) void k_axpy_s(const T* __restrict__ r, const T* __restrict__ v,
                                                 T* __restrict__ s, const double* __restrict__ alpha,
-                                                double* __restrict__ part1, int nn, int nbx, int tile_rows,
+                                                double* __restrict__ part1, Work W,
                                                 const int* __restrict__ grp_active) {
-  Acc<Geo<T, NSP>::CPL> d1;
-  d1.zero();
-  T alr[Geo<T, NSP>::CPL], ali[Geo<T, NSP>::CPL];
   ROW_LOOP_BEGIN
-  if (st == 0) load_coef<T, CPL>(alpha, rhs0, alr, ali);
+  Acc<CPL> d1;
+  d1.zero();
+  T alr[CPL], ali[CPL];
+  load_coef<T, CPL>(alpha, rhs0, alr, ali);
+  ROW_LOOP_ROWS
   T rr[CPL], ri[CPL], vr[CPL], vi[CPL];
   Pack<T>::load(r + off, rr, ri);
   Pack<T>::load(v + off, vr, vi);
@@ -275,7 +334,7 @@ __global__ __launch_bounds__(256) void k_axpy_s(const T* __restrict__ r, const T
   }
   Pack<T>::store(s + off, rr, ri);
   ROW_LOOP_END
-  block_reduce_store<T, NSP>(d1, sh, part1, (long long)g * nbx + bx);
+  block_reduce_store<T, NSP>(d1, sh, part1, it.wl);
 }
 
 // x += alpha h + omega z (active columns only); r = s - omega t;
@@ -285,20 +344,19 @@ __global__ __launch_bounds__(256) void k_update_xr(
     T* __restrict__ x, const T* __restrict__ h, const T* __restrict__ z, T* __restrict__ r,
     const T* __restrict__ s, const T* __restrict__ t, const T* __restrict__ rhat,
     const double* __restrict__ alpha, const double* __restrict__ omega,
-    const int* __restrict__ status, double* __restrict__ part1, double* __restrict__ part2, int nn,
-    int nbx, int tile_rows, const int* __restrict__ grp_active) {
-  Acc<Geo<T, NSP>::CPL> d1, d2;
+    const int* __restrict__ status, double* __restrict__ part1, double* __restrict__ part2, Work W,
+    const int* __restrict__ grp_active) {
+  ROW_LOOP_BEGIN
+  Acc<CPL> d1, d2;
   d1.zero();
   d2.zero();
-  T alr[Geo<T, NSP>::CPL], ali[Geo<T, NSP>::CPL], omr[Geo<T, NSP>::CPL], omi[Geo<T, NSP>::CPL];
-  bool act[Geo<T, NSP>::CPL];
-  ROW_LOOP_BEGIN
-  if (st == 0) {
-    load_coef<T, CPL>(alpha, rhs0, alr, ali);
-    load_coef<T, CPL>(omega, rhs0, omr, omi);
+  T alr[CPL], ali[CPL], omr[CPL], omi[CPL];
+  bool act[CPL];
+  load_coef<T, CPL>(alpha, rhs0, alr, ali);
+  load_coef<T, CPL>(omega, rhs0, omr, omi);
 #pragma unroll
-    for (int e = 0; e < CPL; ++e) act[e] = status[rhs0 + e] == ST_ACTIVE;
-  }
+  for (int e = 0; e < CPL; ++e) act[e] = status[rhs0 + e] == ST_ACTIVE;
+  ROW_LOOP_ROWS
   T xr[CPL], xi[CPL], hr[CPL], hi[CPL], zr[CPL], zi[CPL];
   T sr[CPL], si[CPL], tr[CPL], ti[CPL], qr[CPL], qi[CPL];
   Pack<T>::load(x + off, xr, xi);
@@ -322,24 +380,22 @@ __global__ __launch_bounds__(256) void k_update_xr(
   Pack<T>::store(x + off, xr, xi);
   Pack<T>::store(r + off, sr, si);
   ROW_LOOP_END
-  block_reduce_store<T, NSP>(d1, sh, part1, (long long)g * nbx + bx);
+  block_reduce_store<T, NSP>(d1, sh, part1, it.wl);
   __syncthreads();
-  block_reduce_store<T, NSP>(d2, sh, part2, (long long)g * nbx + bx);
+  block_reduce_store<T, NSP>(d2, sh, part2, it.wl);
 }
 
 // p = r + beta (p - omega v)
 template <typename T, int NSP>
 __global__ __launch_bounds__(256) void k_update_p(T* __restrict__ p, const T* __restrict__ r,
-                                                  const T* __restrict__ v,
-                                                  const double* __restrict__ beta,
-                                                  const double* __restrict__ omega, int nn, int nbx, int tile_rows,
+                                                  const T* __restrict__ v, const double* __restrict__ beta,
+                                                  const double* __restrict__ omega, Work W,
                                                   const int* __restrict__ grp_active) {
-  T br[Geo<T, NSP>::CPL], bi[Geo<T, NSP>::CPL], omr[Geo<T, NSP>::CPL], omi[Geo<T, NSP>::CPL];
   ROW_LOOP_BEGIN
-  if (st == 0) {
-    load_coef<T, CPL>(beta, rhs0, br, bi);
-    load_coef<T, CPL>(omega, rhs0, omr, omi);
-  }
+  T br[CPL], bi[CPL], omr[CPL], omi[CPL];
+  load_coef<T, CPL>(beta, rhs0, br, bi);
+  load_coef<T, CPL>(omega, rhs0, omr, omi);
+  ROW_LOOP_ROWS
   T pr[CPL], pi[CPL], rr[CPL], ri[CPL], vr[CPL], vi[CPL];
   Pack<T>::load(p + off, pr, pi);
   Pack<T>::load(r + off, rr, ri);
@@ -353,7 +409,6 @@ __global__ __launch_bounds__(256) void k_update_p(T* __restrict__ p, const T* __
   }
   Pack<T>::store(p + off, pr, pi);
   ROW_LOOP_END
-  (void)bx;
 }
 
 // COCG residual step: r -= alpha q; part2 = sum |r|^2.  (x += alpha p is
@@ -361,13 +416,13 @@ __global__ __launch_bounds__(256) void k_update_p(T* __restrict__ p, const T* __
 template <typename T, int NSP>
 __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ r, const T* __restrict__ q,
                                                    const double* __restrict__ alpha, double* __restrict__ part2,
-                                                   int nn, int nbx, int tile_rows,
-                                                   const int* __restrict__ grp_active) {
-  Acc<Geo<T, NSP>::CPL> d2;
-  d2.zero();
-  T alr[Geo<T, NSP>::CPL], ali[Geo<T, NSP>::CPL];
+                                                   Work W, const int* __restrict__ grp_active) {
   ROW_LOOP_BEGIN
-  if (st == 0) load_coef<T, CPL>(alpha, rhs0, alr, ali);
+  Acc<CPL> d2;
+  d2.zero();
+  T alr[CPL], ali[CPL];
+  load_coef<T, CPL>(alpha, rhs0, alr, ali);
+  ROW_LOOP_ROWS
   T rr[CPL], ri[CPL], qr[CPL], qi[CPL];
   Pack<T>::load(r + off, rr, ri);
   Pack<T>::load(q + off, qr, qi);
@@ -379,7 +434,7 @@ __global__ __launch_bounds__(256) void k_cg_update(T* __restrict__ r, const T* _
   }
   Pack<T>::store(r + off, rr, ri);
   ROW_LOOP_END
-  block_reduce_store<T, NSP>(d2, sh, part2, (long long)g * nbx + bx);
+  block_reduce_store<T, NSP>(d2, sh, part2, it.wl);
 }
 
 // r = b, rhat = b, p = b, x = 0; part1 = sum |b|^2, part2 = sum b b
@@ -387,12 +442,13 @@ template <typename T, int NSP>
 __global__ __launch_bounds__(256) void k_init(const T* __restrict__ b, T* __restrict__ x,
                                               T* __restrict__ r, T* __restrict__ rhat,
                                               T* __restrict__ p, double* __restrict__ part1,
-                                              double* __restrict__ part2, int nn, int nbx, int tile_rows,
+                                              double* __restrict__ part2, Work W,
                                               const int* __restrict__ grp_active) {
-  Acc<Geo<T, NSP>::CPL> d1, d2;
+  ROW_LOOP_BEGIN
+  Acc<CPL> d1, d2;
   d1.zero();
   d2.zero();
-  ROW_LOOP_BEGIN
+  ROW_LOOP_ROWS
   T br[CPL], bi[CPL], zr[CPL], zi[CPL];
   Pack<T>::load(b + off, br, bi);
 #pragma unroll
@@ -408,9 +464,9 @@ __global__ __launch_bounds__(256) void k_init(const T* __restrict__ b, T* __rest
   Pack<T>::store(rhat + off, br, bi);
   Pack<T>::store(p + off, br, bi);
   ROW_LOOP_END
-  block_reduce_store<T, NSP>(d1, sh, part1, (long long)g * nbx + bx);
+  block_reduce_store<T, NSP>(d1, sh, part1, it.wl);
   __syncthreads();
-  block_reduce_store<T, NSP>(d2, sh, part2, (long long)g * nbx + bx);
+  block_reduce_store<T, NSP>(d2, sh, part2, it.wl);
 }
 
 // FP64 refinement residual: r64 = b - A x64 (complex128), with A x64 from
@@ -421,28 +477,26 @@ __global__ __launch_bounds__(256) void k_init(const T* __restrict__ b, T* __rest
 template <int NSP>
 __global__ __launch_bounds__(256) void k_residual64(
     const int* __restrict__ rptr, const int* __restrict__ cptr, const double* __restrict__ are,
-    const double* __restrict__ aim, const double* __restrict__ fscale,
-    const double* __restrict__ b, const double* __restrict__ x64, float* __restrict__ r32,
-    float* __restrict__ rhat32, float* __restrict__ p32, float* __restrict__ x32,
-    const int* __restrict__ ostatus, int write, double* __restrict__ part1,
-    double* __restrict__ part2, int nn, int ncb, int g0, int nbx, int tile_rows) {
+    const double* __restrict__ aim, const double* __restrict__ b, const double* __restrict__ x64,
+    float* __restrict__ r32, float* __restrict__ rhat32, float* __restrict__ p32, float* __restrict__ x32,
+    const int* __restrict__ ostatus, int write, double* __restrict__ part1, double* __restrict__ part2, Work W) {
   using G = Geo<double, NSP>;  // one complex128 per lane
-  const int g = blockIdx.y, bx = blockIdx.x;
+  Item it;
+  if (!work_item(W, it)) return;
   __shared__ double sh[2 * 4 * NSP];
   __shared__ int srp[kChunkRows + 1];
   __shared__ int scol[kChunkCap];
   __shared__ double sval[2 * kChunkCap];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int c = lane % G::LPR;
-  const long long rhs = (long long)g * NSP + c;
+  const long long rhs = (long long)it.gl * NSP + c;
   const bool act = ostatus ? (ostatus[rhs] == ST_ACTIVE) : true;
-  const int f = (g0 + g) / ncb;
-  const size_t gbase = (size_t)g * nn;
+  const long long vb = it.vb;
   Acc<1> d1, d2;
   d1.zero();
   d2.zero();
   auto epi = [&](int row, const double* yr, const double* yi) {
-    const size_t off = ((gbase + row) * NSP + c);
+    const long long off = (vb + row) * NSP + c;
     const double2 bv = *reinterpret_cast<const double2*>(b + 2 * off);
     double rr = bv.x - yr[0], ri = bv.y - yi[0];
     if (!act) { rr = 0.0; ri = 0.0; }
@@ -458,34 +512,31 @@ __global__ __launch_bounds__(256) void k_residual64(
       reinterpret_cast<float2*>(x32)[off] = make_float2(0.f, 0.f);
     }
   };
-  const int tile0 = xcd_tile(bx) * tile_rows;
-  const int tile1 = min(tile0 + tile_rows, nn);
   if (x64) {
-    spmv_rows<double, NSP, MAT_A, 8>(rptr, cptr, are, aim, fscale[f], x64 + (gbase * NSP + c) * 2, tile0, tile1,
-                                     srp, scol, sval, epi);
+    spmv_rows<double, NSP, MAT_A, 8>(rptr, cptr, are, aim, W.gfsc[it.g], x64 + c * 2, vb, it.row0, it.row1, srp,
+                                     scol, sval, epi);
   } else {
     const double z[1] = {0.0};
-    for (int row = tile0 + wv * G::RPW + lane / G::LPR; row < tile1; row += G::RPB) epi(row, z, z);
+    for (int row = it.row0 + wv * G::RPW + lane / G::LPR; row < it.row1; row += G::RPB) epi(row, z, z);
   }
-  block_reduce_store<double, NSP>(d1, sh, part1, (long long)g * nbx + bx);
+  block_reduce_store<double, NSP>(d1, sh, part1, it.wl);
   if (write) {
     __syncthreads();
-    block_reduce_store<double, NSP>(d2, sh, part2, (long long)g * nbx + bx);
+    block_reduce_store<double, NSP>(d2, sh, part2, it.wl);
   }
 }
 
 // x64 += x32 for columns that took part in the sweep
 template <int NSP>
-__global__ __launch_bounds__(256) void k_add_correction(double* __restrict__ x64,
-                                                        const float* __restrict__ x32,
-                                                        const int* __restrict__ istatus, int nn) {
-  const int g = blockIdx.y;
-  const long long nper = (long long)nn * NSP;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nper;
-       i += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % NSP);
-    if (istatus[(long long)g * NSP + c] == ST_SKIP) continue;
-    const long long o = (long long)g * nper + i;
+__global__ __launch_bounds__(256) void k_add_correction(double* __restrict__ x64, const float* __restrict__ x32,
+                                                        const int* __restrict__ istatus, Work W) {
+  Item it;
+  if (!work_item(W, it)) return;
+  const int nr = it.row1 - it.row0;
+  for (int i = threadIdx.x; i < nr * NSP; i += 256) {
+    const int c = i % NSP;
+    if (istatus[(long long)it.gl * NSP + c] == ST_SKIP) continue;
+    const long long o = (it.vb + it.row0 + i / NSP) * NSP + c;
     const float2 d = reinterpret_cast<const float2*>(x32)[o];
     double2 v = reinterpret_cast<double2*>(x64)[o];
     v.x += d.x;
@@ -494,15 +545,33 @@ __global__ __launch_bounds__(256) void k_add_correction(double* __restrict__ x64
   }
 }
 
+// Row permutation of a complex128 block between the caller's node order and
+// the solver's: dir 0 gathers (dst[row] = src[order[row]]), dir 1 scatters
+// back.  order is global (each mesh's rows map into the same mesh).
+template <int NSP>
+__global__ __launch_bounds__(256) void k_permute_rows(const double* __restrict__ src, double* __restrict__ dst,
+                                                      const int* __restrict__ order, int dir, Work W) {
+  Item it;
+  if (!work_item(W, it)) return;
+  const int nr = it.row1 - it.row0;
+  for (int i = threadIdx.x; i < nr * NSP; i += 256) {
+    const int row = it.row0 + i / NSP, c = i % NSP;
+    const long long a = (it.vb + row) * NSP + c, o = (it.vb + order[row]) * NSP + c;
+    const double2 v = reinterpret_cast<const double2*>(src)[dir == 0 ? o : a];
+    reinterpret_cast<double2*>(dst)[dir == 0 ? a : o] = v;
+  }
+}
+
 // ------------------------------------------------ reference-structure passes
 // part1 = sum a b (mode 0) or sum |a|^2 (mode 1)
 template <typename T, int NSP>
-__global__ __launch_bounds__(256) void k_dot(const T* __restrict__ a, const T* __restrict__ bb,
-                                             int mode, double* __restrict__ part1, int nn, int nbx, int tile_rows,
+__global__ __launch_bounds__(256) void k_dot(const T* __restrict__ a, const T* __restrict__ bb, int mode,
+                                             double* __restrict__ part1, Work W,
                                              const int* __restrict__ grp_active) {
-  Acc<Geo<T, NSP>::CPL> d1;
-  d1.zero();
   ROW_LOOP_BEGIN
+  Acc<CPL> d1;
+  d1.zero();
+  ROW_LOOP_ROWS
   T ar[CPL], ai[CPL], br[CPL], bi[CPL];
   Pack<T>::load(a + off, ar, ai);
   if (mode == 0) Pack<T>::load(bb + off, br, bi);
@@ -516,31 +585,29 @@ __global__ __launch_bounds__(256) void k_dot(const T* __restrict__ a, const T* _
     }
   }
   ROW_LOOP_END
-  block_reduce_store<T, NSP>(d1, sh, part1, (long long)g * nbx + bx);
+  block_reduce_store<T, NSP>(d1, sh, part1, it.wl);
 }
 
 // out = a + op(coef) b; op 0: 1/coef, 1: -1/coef, 2: coef, 3: -coef
 template <typename T, int NSP>
 __global__ __launch_bounds__(256) void k_axpby(T* __restrict__ out, const T* __restrict__ a,
-                                               const T* __restrict__ bb,
-                                               const double* __restrict__ coef, int op, int nn,
-                                               int nbx, int tile_rows, const int* __restrict__ grp_active) {
-  T cr[Geo<T, NSP>::CPL], ci[Geo<T, NSP>::CPL];
+                                               const T* __restrict__ bb, const double* __restrict__ coef, int op,
+                                               Work W, const int* __restrict__ grp_active) {
   ROW_LOOP_BEGIN
-  if (st == 0) {
+  T cr[CPL], ci[CPL];
 #pragma unroll
-    for (int e = 0; e < CPL; ++e) {
-      double xr = coef[2 * (rhs0 + e)], xi = coef[2 * (rhs0 + e) + 1];
-      if (op <= 1) {
-        const double d = xr * xr + xi * xi;
-        xr = xr / d;
-        xi = -xi / d;
-      }
-      if (op == 1 || op == 3) { xr = -xr; xi = -xi; }
-      cr[e] = (T)xr;
-      ci[e] = (T)xi;
+  for (int e = 0; e < CPL; ++e) {
+    double xr = coef[2 * (rhs0 + e)], xi = coef[2 * (rhs0 + e) + 1];
+    if (op <= 1) {
+      const double d = xr * xr + xi * xi;
+      xr = xr / d;
+      xi = -xi / d;
     }
+    if (op == 1 || op == 3) { xr = -xr; xi = -xi; }
+    cr[e] = (T)xr;
+    ci[e] = (T)xi;
   }
+  ROW_LOOP_ROWS
   T ar[CPL], ai[CPL], br[CPL], bi[CPL];
   Pack<T>::load(a + off, ar, ai);
   Pack<T>::load(bb + off, br, bi);
@@ -551,7 +618,6 @@ __global__ __launch_bounds__(256) void k_axpby(T* __restrict__ out, const T* __r
   }
   Pack<T>::store(out + off, ar, ai);
   ROW_LOOP_END
-  (void)bx;
 }
 
 // ------------------------------------------------------------- scalars ----
@@ -572,17 +638,19 @@ struct BState {
   int* counter;    // 4
 };
 
-// Sum of the per-block partials of every column of group g.  All threads of
-// the (kScalThreads-wide) block call it; thread c < nsp receives column c's
-// sum.  Each column is reduced by kScalThreads/nsp threads in parallel.
+// Sum of the per-work-item partials of every column of chunk-local group gl.
+// All threads of the (kScalThreads-wide) block call it; thread c < nsp
+// receives column c's sum.  Each column is reduced by kScalThreads/nsp
+// threads in parallel.
 constexpr int kScalThreads = 256;
-__device__ inline void block_sum_part(const double* __restrict__ part, long long g, int nsp, int nbx,
+__device__ inline void block_sum_part(const double* __restrict__ part, const Work& W, int gl, int nsp,
                                       double& re, double& im) {
   __shared__ double sr[kScalThreads], si[kScalThreads];
   const int t = threadIdx.x, c = t % nsp, q = t / nsp, nq = kScalThreads / nsp;
+  const int g = W.g0 + gl, wb = W.gwbeg[g] - W.pw0, wc = W.gwcnt[g];
   double a = 0.0, b = 0.0;
-  for (int bx = q; bx < nbx; bx += nq) {
-    const long long i = 2 * ((g * nbx + bx) * nsp + c);
+  for (int k = q; k < wc; k += nq) {
+    const long long i = 2 * ((long long)(wb + k) * nsp + c);
     a += part[i];
     b += part[i + 1];
   }
@@ -600,7 +668,9 @@ __device__ inline void block_sum_part(const double* __restrict__ part, long long
   __syncthreads();
 }
 
-__device__ inline void group_flag(BState s, int g, bool active) {
+// Marks chunk-local group gl active if any of its columns is, and counts
+// active columns, groups and the groups' rows in s.counter[0..2].
+__device__ inline void group_flag(BState s, const Work& W, int gl, bool active) {
   __shared__ int cnt;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
@@ -609,28 +679,28 @@ __device__ inline void group_flag(BState s, int g, bool active) {
   __syncthreads();
   if (threadIdx.x == 0) {
     const int n = cnt;
-    s.grp_active[g] = n > 0;
+    s.grp_active[gl] = n > 0;
     if (n) {
       atomicAdd(s.counter, n);
       atomicAdd(s.counter + 1, 1);
+      atomicAdd(s.counter + 2, W.grows[W.g0 + gl]);
     }
   }
 }
 
 // init from (part1 = |b|^2, part2 = b.b); `mask` (optional outer status)
-// leaves non-active columns skipped; columns >= ncols are padding.
+// leaves non-active columns skipped; columns >= gcols are padding.
 __global__ __launch_bounds__(kScalThreads) void k_sc_init(BState s, const double* __restrict__ part1,
                                                           const double* __restrict__ part2,
-                                                          const int* __restrict__ mask, int nsp,
-                                                          int nbx, int ncb, int ncols, int g0) {
+                                                          const int* __restrict__ mask, int nsp, Work W) {
   const int g = blockIdx.x, c = threadIdx.x;
   double n2 = 0, dummy = 0, rr = 0, ri = 0;
-  block_sum_part(part1, g, nsp, nbx, n2, dummy);
-  block_sum_part(part2, g, nsp, nbx, rr, ri);
+  block_sum_part(part1, W, g, nsp, n2, dummy);
+  block_sum_part(part2, W, g, nsp, rr, ri);
   bool active = false;
   if (c < nsp) {
     const long long i = (long long)g * nsp + c;
-    const int col = ((g0 + g) % ncb) * nsp + c;
+    const int ncols = W.gcols[W.g0 + g], col = c;
     s.iters[i] = 0;
     s.half[i] = 0;
     s.bd[i] = 0;
@@ -651,15 +721,15 @@ __global__ __launch_bounds__(kScalThreads) void k_sc_init(BState s, const double
       active = true;
     }
   }
-  group_flag(s, g, active);
+  group_flag(s, W, g, active);
 }
 
 __global__ __launch_bounds__(kScalThreads) void k_sc_alpha(BState s, const double* __restrict__ part1,
-                                                           int nsp, int nbx) {
+                                                           int nsp, Work W) {
   const int g = blockIdx.x, c = threadIdx.x;
   if (!s.grp_active[g]) return;
   double dr = 0, di = 0;
-  block_sum_part(part1, g, nsp, nbx, dr, di);
+  block_sum_part(part1, W, g, nsp, dr, di);
   if (c >= nsp) return;
   const long long i = (long long)g * nsp + c;
   if (s.status[i] != ST_ACTIVE) return;
@@ -673,11 +743,11 @@ __global__ __launch_bounds__(kScalThreads) void k_sc_alpha(BState s, const doubl
 }
 
 __global__ __launch_bounds__(kScalThreads) void k_sc_s(BState s, const double* __restrict__ part1, int nsp,
-                                                       int nbx) {
+                                                       Work W) {
   const int g = blockIdx.x, c = threadIdx.x;
   if (!s.grp_active[g]) return;
   double n2 = 0, dummy = 0;
-  block_sum_part(part1, g, nsp, nbx, n2, dummy);
+  block_sum_part(part1, W, g, nsp, n2, dummy);
   if (c >= nsp) return;
   const long long i = (long long)g * nsp + c;
   if (s.status[i] != ST_ACTIVE) return;
@@ -689,13 +759,12 @@ __global__ __launch_bounds__(kScalThreads) void k_sc_s(BState s, const double* _
 }
 
 __global__ __launch_bounds__(kScalThreads) void k_sc_omega(BState s, const double* __restrict__ part1,
-                                                           const double* __restrict__ part2, int nsp,
-                                                           int nbx) {
+                                                           const double* __restrict__ part2, int nsp, Work W) {
   const int g = blockIdx.x, c = threadIdx.x;
   if (!s.grp_active[g]) return;
   double nr = 0, ni = 0, dr = 0, di = 0;
-  block_sum_part(part1, g, nsp, nbx, nr, ni);
-  block_sum_part(part2, g, nsp, nbx, dr, di);
+  block_sum_part(part1, W, g, nsp, nr, ni);
+  block_sum_part(part2, W, g, nsp, dr, di);
   if (c >= nsp) return;
   const long long i = (long long)g * nsp + c;
   if (s.status[i] != ST_ACTIVE) return;
@@ -714,13 +783,13 @@ __global__ __launch_bounds__(kScalThreads) void k_sc_omega(BState s, const doubl
 }
 
 __global__ __launch_bounds__(kScalThreads) void k_sc_beta(BState s, const double* __restrict__ part1,
-                                                          const double* __restrict__ part2, int nsp,
-                                                          int nbx, int maxit) {
+                                                          const double* __restrict__ part2, int nsp, Work W,
+                                                          int maxit) {
   const int g = blockIdx.x, c = threadIdx.x;
   if (!s.grp_active[g]) return;   // inactive groups stay inactive
   double n2 = 0, dummy = 0, rr = 0, ri = 0;
-  block_sum_part(part2, g, nsp, nbx, n2, dummy);
-  block_sum_part(part1, g, nsp, nbx, rr, ri);
+  block_sum_part(part2, W, g, nsp, n2, dummy);
+  block_sum_part(part1, W, g, nsp, rr, ri);
   bool active = false;
   if (c < nsp) {
     const long long i = (long long)g * nsp + c;
@@ -754,7 +823,7 @@ __global__ __launch_bounds__(kScalThreads) void k_sc_beta(BState s, const double
       }
     }
   }
-  group_flag(s, g, active);
+  group_flag(s, W, g, active);
 }
 
 // ---- COCG scalars (conjugate orthogonal CG: CG with the unconjugated
@@ -765,10 +834,10 @@ __global__ __launch_bounds__(kScalThreads) void k_sc_beta(BState s, const double
 
 // After the start pass u = G r (part1 = u.u): rho = u.u, beta = 0.
 __global__ __launch_bounds__(kScalThreads) void k_cg_start(BState s, const double* __restrict__ part1,
-                                                           int nsp, int nbx) {
+                                                           int nsp, Work W) {
   const int g = blockIdx.x, c = threadIdx.x;
   double ur = 0, ui = 0;
-  block_sum_part(part1, g, nsp, nbx, ur, ui);
+  block_sum_part(part1, W, g, nsp, ur, ui);
   bool active = false;
   if (c < nsp) {
     const long long i = (long long)g * nsp + c;
@@ -784,16 +853,16 @@ __global__ __launch_bounds__(kScalThreads) void k_cg_start(BState s, const doubl
     s.alpha[2 * i] = s.alpha[2 * i + 1] = 0.0;
     s.beta[2 * i] = s.beta[2 * i + 1] = 0.0;
   }
-  group_flag(s, g, active);
+  group_flag(s, W, g, active);
 }
 
 // part1 = p.(A p)
 __global__ __launch_bounds__(kScalThreads) void k_cg_alpha(BState s, const double* __restrict__ part1,
-                                                           int nsp, int nbx) {
+                                                           int nsp, Work W) {
   const int g = blockIdx.x, c = threadIdx.x;
   if (!s.grp_active[g]) return;
   double dr = 0, di = 0;
-  block_sum_part(part1, g, nsp, nbx, dr, di);
+  block_sum_part(part1, W, g, nsp, dr, di);
   if (c >= nsp) return;
   const long long i = (long long)g * nsp + c;
   double ar = 0.0, ai = 0.0;
@@ -813,16 +882,16 @@ __global__ __launch_bounds__(kScalThreads) void k_cg_alpha(BState s, const doubl
 // part2 = |r|^2 (after the update), part1 = u.u with u = G r: convergence,
 // then beta and rho for the columns that go on.
 __global__ __launch_bounds__(kScalThreads) void k_cg_beta(BState s, const double* __restrict__ part1,
-                                                          const double* __restrict__ part2, int nsp,
-                                                          int nbx, int maxit) {
+                                                          const double* __restrict__ part2, int nsp, Work W,
+                                                          int maxit) {
   const int g = blockIdx.x, c = threadIdx.x;
   // grp_prev gates this iteration's G^T pass, which applies x += alpha p
   const int was = s.grp_active[g];
   if (c == 0) s.grp_prev[g] = was;
   if (!was) return;
   double n2 = 0, dummy = 0, ur = 0, ui = 0;
-  block_sum_part(part2, g, nsp, nbx, n2, dummy);
-  block_sum_part(part1, g, nsp, nbx, ur, ui);
+  block_sum_part(part2, W, g, nsp, n2, dummy);
+  block_sum_part(part1, W, g, nsp, ur, ui);
   bool active = false;
   if (c < nsp) {
     const long long i = (long long)g * nsp + c;
@@ -847,7 +916,7 @@ __global__ __launch_bounds__(kScalThreads) void k_cg_beta(BState s, const double
     s.beta[2 * i] = br;
     s.beta[2 * i + 1] = bi;
   }
-  group_flag(s, g, active);
+  group_flag(s, W, g, active);
 }
 
 // Outer (FP64) refinement state
@@ -862,17 +931,15 @@ struct RState {
 // After a residual pass: decide each column's outer status and its inner
 // tolerance.  part1 = |r64|^2.
 __global__ __launch_bounds__(kScalThreads) void k_rf_check(RState o, BState in,
-                                                           const double* __restrict__ part1, int nsp,
-                                                           int nbx, int ncb, int ncols, int g0,
-                                                           int sweep, double tol, double tol_in,
-                                                           int maxit) {
+                                                           const double* __restrict__ part1, int nsp, Work W,
+                                                           int sweep, double tol, double tol_in, int maxit) {
   const int g = blockIdx.x, c = threadIdx.x;
   double n2 = 0, dummy = 0;
-  block_sum_part(part1, g, nsp, nbx, n2, dummy);
+  block_sum_part(part1, W, g, nsp, n2, dummy);
   bool active = false;
   if (c < nsp) {
     const long long i = (long long)g * nsp + c;
-    const int col = ((g0 + g) % ncb) * nsp + c;
+    const int ncols = W.gcols[W.g0 + g], col = c;
     if (sweep == 0) {
       o.bnorm[i] = sqrt(n2);
       o.iters[i] = 0;
@@ -897,7 +964,7 @@ __global__ __launch_bounds__(kScalThreads) void k_rf_check(RState o, BState in,
       }
     }
   }
-  group_flag(in, g, active);
+  group_flag(in, W, g, active);
 }
 
 __global__ void k_rf_accum(RState o, BState in, int nsp) {
@@ -913,16 +980,15 @@ __global__ void k_rf_accum(RState o, BState in, int nsp) {
 // rho; op 5: init err0 = |r|, rho0 = rhat0.r.
 __global__ __launch_bounds__(kScalThreads) void k_plain_scal(BState s, int op,
                                                              const double* __restrict__ part1,
-                                                             const double* __restrict__ part2, int nsp,
-                                                             int nbx, int ncb, int ncols, int g0,
+                                                             const double* __restrict__ part2, int nsp, Work W,
                                                              unsigned long long* __restrict__ maxbits) {
   const int g = blockIdx.x, c = threadIdx.x;
   double ar = 0, ai = 0, br = 0, bi = 0;
-  block_sum_part(part1, g, nsp, nbx, ar, ai);
-  if (part2) block_sum_part(part2, g, nsp, nbx, br, bi);
+  block_sum_part(part1, W, g, nsp, ar, ai);
+  if (part2) block_sum_part(part2, W, g, nsp, br, bi);
   if (c >= nsp) return;
   const long long i = (long long)g * nsp + c;
-  const int col = ((g0 + g) % ncb) * nsp + c;
+  const int ncols = W.gcols[W.g0 + g], col = c;
   const bool valid = col < ncols && s.bnorm[i] > 0.0;
   switch (op) {
     case 5:
@@ -1009,9 +1075,23 @@ struct Buf {
 }  // namespace dogpup
 
 struct dogpup_solver {
-  int nn = 0, nf = 0, ncols = 0, nsp = 0, ncb = 0, ngroups = 0, mode = 0;
-  int nnzA = 0, nnzG = 0;
+  int nmesh = 0, nn = 0, nsp = 0, ngroups = 0, nwork = 0, mode = 0;
+  int nnzA = 0, nnzG = 0, nrhs_out = 0, nf_total = 0;
   int method = DOGPUP_METHOD_BICGSTAB;  // Krylov method of the fused forms
+  // meshes: rows [m_row0[i], m_row0[i+1]), nf, ncols, column blocks, first
+  // group, first A and G entries, G entries, G value base (complex elements,
+  // [f][nnzG_i] per mesh), first fscale entry, first output RHS
+  std::vector<int> m_row0, m_nf, m_ncols, m_ncb, m_g0, m_pA0, m_pG0, m_nnzG, m_fs0, m_rhs0;
+  std::vector<long long> m_gbase;
+  long long gvals = 0;  // complex G values per copy (sum of nf_i * nnzG_i)
+  // groups (host copies of the device table)
+  // h_gibeg: first work item of each group (items padded to multiples of
+  // 8); h_gwbeg/h_gwcnt: its partial-sum slots (one per row tile)
+  std::vector<int> h_grow0, h_grows, h_gcols, h_gmesh, h_gf, h_gcb, h_gibeg, h_gwbeg, h_gwcnt;
+  std::vector<long long> h_gvb, h_ggv;
+  int nparts = 0;
+  dogpup::Buf t_wgrp, t_wtile, t_wpart, t_grow0, t_grows, t_gvb, t_gfsc, t_ggv, t_gwbeg, t_gwcnt, t_gcols;
+  std::vector<int> chunk_g;  // chunk c = groups [chunk_g[c], chunk_g[c+1])
   // caller's patterns in the reference node order (FSAI setup runs on these)
   const int *rA0 = nullptr, *cA0 = nullptr, *rG0 = nullptr, *cG0 = nullptr;
   // patterns in solver node order (owned): A, G, G^T and the value maps
@@ -1019,16 +1099,17 @@ struct dogpup_solver {
   const int *rA = nullptr, *cA = nullptr, *rG = nullptr, *cG = nullptr;
   const int *rGT = nullptr, *cGT = nullptr;
   dogpup::Buf pAr, pAc, pGr, pGc, pGTr, pGTc, amap, gmap, gtmap, order;
-  int unroll_a = 8, unroll_g = 8;  // gathers in flight per row (k_spmv: 4, 8 or 16)
   // values: a64re/a64im and g64 in reference order; the rest in solver order
   dogpup::Buf a64re, a64im, a64pre, a64pim, a32re, a32im, fscale, g64, gp64, gtp64, g32, gt32;
   dogpup::Buf ws, part1, part2, state, rstate, bperm, xperm;
-  int chunk = 0, nbx = 0, tile_rows = 512, lds_pad = 0;
+  int tile_rows = 512, lds_pad = 0;
+  int unroll_a = 8, unroll_g = 8;  // gathers in flight per row (k_spmv: 4, 8 or 16)
   bool have_matrix = false;
   double stats[4] = {0, 0, 0, 0};
-  int* h_counter = nullptr;  // pinned: [0] active RHS, [1] active groups
+  int* h_counter = nullptr;  // pinned: [0] active RHS, [1] active groups, [2] their rows
   unsigned long long* h_max = nullptr;
   int active_groups = 0;
+  long long active_rows = 0;
   // optional per-launch timing: event pairs resolved after the solve
   struct Rec {
     hipEvent_t a, b;
@@ -1037,7 +1118,7 @@ struct dogpup_solver {
   };
   bool profile = false;
   std::vector<Rec> recs;
-  // captured BiCGSTAB iterations, keyed by (chunk, maxit, precision)
+  // captured Krylov iterations, keyed by (chunk, maxit, method, precision)
   bool use_graph = true;
   int graph_k = 4;
   std::map<long long, hipGraphExec_t> graphs;
@@ -1064,12 +1145,6 @@ struct dogpup_solver {
 
 namespace dogpup {
 
-__global__ void k_cast_f32(const double* __restrict__ a, float* __restrict__ b, long long n) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    b[i] = (float)a[i];
-}
-
 // out[p] = in[map[p]] (real values, type conversion)
 template <typename TO>
 __global__ void k_gather_real(const double* __restrict__ in, const int* __restrict__ map,
@@ -1079,36 +1154,22 @@ __global__ void k_gather_real(const double* __restrict__ in, const int* __restri
     out[i] = (TO)in[map[i]];
 }
 
-// out[f][p] = in[f][map[p]] (complex [f][nnz])
+// One mesh's G values into solver order: for p in [p0, p0 + nnz) and every
+// frequency f < nf, out[base + f*nnz + p - p0] = in[base + f*nnz + map[p] - p0]
+// (complex; map stays within the mesh).
 template <typename TO>
-__global__ void k_gather_cplx(const double* __restrict__ in, const int* __restrict__ map,
-                              TO* __restrict__ out, long long nnz, int nf) {
+__global__ void k_gather_cplx(const double* __restrict__ in, const int* __restrict__ map, TO* __restrict__ out,
+                              long long p0, long long nnz, int nf, long long base) {
   const long long total = nnz * nf;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (long long)gridDim.x * blockDim.x) {
     const long long f = i / nnz, p = i % nnz;
-    const long long s = f * nnz + map[p];
-    out[2 * i] = (TO)in[2 * s];
-    out[2 * i + 1] = (TO)in[2 * s + 1];
+    const long long d = base + f * nnz + p;
+    const long long s = base + f * nnz + (map[p0 + p] - p0);
+    out[2 * d] = (TO)in[2 * s];
+    out[2 * d + 1] = (TO)in[2 * s + 1];
   }
 }
-
-// Row permutation of a complex128 block [g][node][nsp]: dir 0 gathers into
-// solver order (dst[g][i] = src[g][order[i]]), dir 1 scatters back.
-__global__ void k_permute_rows(const double* __restrict__ src, double* __restrict__ dst,
-                               const int* __restrict__ order, int nn, int nsp, int gc, int dir) {
-  const long long per = (long long)nn * nsp;
-  const long long total = per * gc;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (long long)gridDim.x * blockDim.x) {
-    const long long g = i / per, r = i % per;
-    const int node = (int)(r / nsp), c = (int)(r % nsp);
-    const long long o = g * per + (long long)order[node] * nsp + c;
-    const double2 v = reinterpret_cast<const double2*>(src)[dir == 0 ? o : i];
-    reinterpret_cast<double2*>(dst)[dir == 0 ? i : o] = v;
-  }
-}
-
 
 static unsigned grid1(long long n) {
   long long b = (n + 255) / 256;
@@ -1128,15 +1189,46 @@ struct Vecs {
   T *x, *r, *rhat, *p, *v, *h, *s, *z, *t, *u, *w;
 };
 
+// The group table and work list of chunk c.
+static Work chunk_work(const dogpup_solver* S, int c) {
+  Work W;
+  W.wgrp = S->t_wgrp.as<int>();
+  W.wtile = S->t_wtile.as<int>();
+  W.wpart = S->t_wpart.as<int>();
+  W.grow0 = S->t_grow0.as<int>();
+  W.grows = S->t_grows.as<int>();
+  W.gvb = S->t_gvb.as<long long>();
+  W.gfsc = S->t_gfsc.as<double>();
+  W.ggv = S->t_ggv.as<long long>();
+  W.gwbeg = S->t_gwbeg.as<int>();
+  W.gwcnt = S->t_gwcnt.as<int>();
+  W.gcols = S->t_gcols.as<int>();
+  const int g0 = S->chunk_g[c], g1 = S->chunk_g[c + 1];
+  W.g0 = g0;
+  W.w0 = S->h_gibeg[g0];
+  W.nw = (g1 < S->ngroups ? S->h_gibeg[g1] : S->nwork) - W.w0;
+  W.pw0 = S->h_gwbeg[g0];
+  W.vb0 = S->h_gvb[g0];
+  W.tile_rows = S->tile_rows;
+  return W;
+}
+
+// nodes (rows summed over groups) of groups [g0, g1)
+static long long group_nodes(const dogpup_solver* S, int g0, int g1) {
+  return (g1 < S->ngroups ? S->h_gvb[g1] : S->h_gvb.back() + S->h_grows.back()) - S->h_gvb[g0];
+}
+
 template <typename T, int NSP>
 struct Chunk {
   dogpup_solver* S;
   hipStream_t st;
-  int gcount, g0;
+  int cidx, gcount;
+  Work W;
   BState bs;
   double *p1, *p2;
+  long long nodes;  // nodes of all groups of the chunk
 
-  dim3 vgrid() const { return dim3(S->nbx, gcount); }
+  dim3 vgrid() const { return dim3((unsigned)W.nw); }
   const T* a0() const { return sizeof(T) == 4 ? (const T*)S->a32re.p : (const T*)S->a64pre.p; }
   const T* a1() const { return sizeof(T) == 4 ? (const T*)S->a32im.p : (const T*)S->a64pim.p; }
   const T* gvals(bool tr) const {
@@ -1152,30 +1244,28 @@ struct Chunk {
     return DOGPUP_OK;
   }
   template <int MAT, int EPI, int U>
-  int launch(const int* rp, const int* cp, const T* m0, const T* m1, long long nz, const T* x, T* y,
-             const T* w, const int* ga, const double* coef, const double* coef2, T* xacc) {
+  int launch(const int* rp, const int* cp, const T* m0, const T* m1, const T* x, T* y, const T* w, const int* ga,
+             const double* coef, const double* coef2, T* xacc) {
     DP_TRY((pad_lds<MAT, EPI, U>()));
-    k_spmv<T, NSP, MAT, EPI, U><<<vgrid(), 256, S->lds_pad, st>>>(
-        rp, cp, m0, m1, nz, S->fscale.template as<double>(), x, y, w, coef, coef2, xacc, p1, p2, S->nn, S->ncb,
-        g0, S->nbx, S->tile_rows, ga);
+    k_spmv<T, NSP, MAT, EPI, U><<<vgrid(), 256, S->lds_pad, st>>>(rp, cp, m0, m1, x, y, w, coef, coef2, xacc, p1,
+                                                                  p2, W, ga);
     DP_LAUNCH_CHECK();
     return DOGPUP_OK;
   }
   template <int MAT, int EPI>
-  int launch_u(int u, const int* rp, const int* cp, const T* m0, const T* m1, long long nz, const T* x, T* y,
-               const T* w, const int* ga, const double* coef, const double* coef2, T* xacc) {
+  int launch_u(int u, const int* rp, const int* cp, const T* m0, const T* m1, const T* x, T* y, const T* w,
+               const int* ga, const double* coef, const double* coef2, T* xacc) {
     switch (u) {
-      case 4: return launch<MAT, EPI, 4>(rp, cp, m0, m1, nz, x, y, w, ga, coef, coef2, xacc);
-      case 8: return launch<MAT, EPI, 8>(rp, cp, m0, m1, nz, x, y, w, ga, coef, coef2, xacc);
-      default: return launch<MAT, EPI, 16>(rp, cp, m0, m1, nz, x, y, w, ga, coef, coef2, xacc);
+      case 4: return launch<MAT, EPI, 4>(rp, cp, m0, m1, x, y, w, ga, coef, coef2, xacc);
+      case 8: return launch<MAT, EPI, 8>(rp, cp, m0, m1, x, y, w, ga, coef, coef2, xacc);
+      default: return launch<MAT, EPI, 16>(rp, cp, m0, m1, x, y, w, ga, coef, coef2, xacc);
     }
   }
   template <int EPI>
   int spmvA(const T* x, T* y, const T* w, const int* ga) {
     pbegin();
-    DP_TRY((launch_u<MAT_A, EPI>(S->unroll_a, S->rA, S->cA, a0(), a1(), 0, x, y, w, ga, nullptr, nullptr,
-                                 nullptr)));
-    pend(1, groups(ga != nullptr) * vbytes() * ((EPI == EPI_STORE || w == x) ? 2 : 3) +
+    DP_TRY((launch_u<MAT_A, EPI>(S->unroll_a, S->rA, S->cA, a0(), a1(), x, y, w, ga, nullptr, nullptr, nullptr)));
+    pend(1, rows(ga != nullptr) * rowbytes() * ((EPI == EPI_STORE || w == x) ? 2 : 3) +
                 (double)S->nnzA * (2 * sizeof(T) + 4) + 4.0 * (S->nn + 1));
     return DOGPUP_OK;
   }
@@ -1183,24 +1273,26 @@ struct Chunk {
   int spmvG(bool tr, const T* x, T* y, const T* w, const int* ga, const double* coef = nullptr,
             const double* coef2 = nullptr, T* xacc = nullptr) {
     pbegin();
-    DP_TRY((launch_u<MAT_G, EPI>(S->unroll_g, tr ? S->rGT : S->rG, tr ? S->cGT : S->cG, gvals(tr), nullptr,
-                                 S->nnzG, x, y, w, ga, coef, coef2, xacc)));
+    DP_TRY((launch_u<MAT_G, EPI>(S->unroll_g, tr ? S->rGT : S->rG, tr ? S->cGT : S->cG, gvals(tr), nullptr, x, y,
+                                 w, ga, coef, coef2, xacc)));
     const int streams = (EPI == EPI_STORE || EPI == EPI_DOT2 || EPI == EPI_STORE_SELF) ? 2
                         : (EPI == EPI_AXPBY_X ? 5 : 3);
-    pend(0, groups(ga != nullptr) * (vbytes() * streams + (double)S->nnzG * (2 * sizeof(T) + 4) +
-                                    4.0 * (S->nn + 1)));
+    // G values: one copy per group, nnzG/nn entries per row on average
+    pend(0, rows(ga != nullptr) * (rowbytes() * streams + (double)S->nnzG / S->nn * (2 * sizeof(T) + 4)) +
+                4.0 * (S->nn + 1));
     return DOGPUP_OK;
   }
 
   int wait_count(int* out) {
-    DP_HIP(hipMemcpyAsync(S->h_counter, bs.counter, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    DP_HIP(hipMemcpyAsync(S->h_counter, bs.counter, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
     DP_HIP(hipStreamSynchronize(st));
     *out = S->h_counter[0];
     S->active_groups = S->h_counter[1];
+    S->active_rows = S->h_counter[2];
     return DOGPUP_OK;
   }
   int reset_count() {
-    k_zero_ints<<<1, 64, 0, st>>>(bs.counter, 2);
+    k_zero_ints<<<1, 64, 0, st>>>(bs.counter, 3);
     DP_LAUNCH_CHECK();
     return DOGPUP_OK;
   }
@@ -1219,12 +1311,12 @@ struct Chunk {
     (void)hipEventRecord(pb, st);
     S->recs.push_back({pa, pb, cls, bytes});
   }
-  double vbytes() const { return (double)S->nn * NSP * 2 * sizeof(T); }
-  double groups(bool skip) const { return skip ? (double)S->active_groups : (double)gcount; }
+  // bytes of one vector row (NSP columns) and the rows a pass touches: those
+  // of the active groups when groups are skipped (polled at the last
+  // convergence check), else all of the chunk's
+  double rowbytes() const { return (double)NSP * 2 * sizeof(T); }
+  double rows(bool skip) const { return skip ? (double)S->active_rows : (double)nodes; }
 
-  // Fused BiCGSTAB.  If b != nullptr the init pass runs from b (x = 0);
-  // otherwise r, rhat, p, x and the state are already initialised (the
-  // refinement residual pass did it).  Returns iterations run.
   // One BiCGSTAB iteration over every active group (no host synchronisation,
   // no memset nodes: it is captured into a HIP graph when profiling is off).
   int body(Vecs<T> V, int maxit) {
@@ -1233,33 +1325,33 @@ struct Chunk {
     DP_TRY(spmvG<EPI_STORE>(false, V.p, V.u, nullptr, ga));      // u = G p
     DP_TRY(spmvG<EPI_STORE>(true, V.u, V.h, nullptr, ga));       // h = G^T u
     DP_TRY(spmvA<EPI_STORE_DOT>(V.h, V.v, V.rhat, ga));          // v = A h, rhat.v
-    k_sc_alpha<<<gcount, kScalThreads, 0, st>>>(bs, p1, nsp, S->nbx);
+    k_sc_alpha<<<gcount, kScalThreads, 0, st>>>(bs, p1, nsp, W);
     DP_LAUNCH_CHECK();
     pbegin();
-    k_axpy_s<T, NSP><<<vgrid(), 256, 0, st>>>(V.r, V.v, V.s, bs.alpha, p1, S->nn, S->nbx, S->tile_rows, ga);
+    k_axpy_s<T, NSP><<<vgrid(), 256, 0, st>>>(V.r, V.v, V.s, bs.alpha, p1, W, ga);
     DP_LAUNCH_CHECK();
-    pend(2, groups(true) * vbytes() * 3);
-    k_sc_s<<<gcount, kScalThreads, 0, st>>>(bs, p1, nsp, S->nbx);
+    pend(2, rows(true) * rowbytes() * 3);
+    k_sc_s<<<gcount, kScalThreads, 0, st>>>(bs, p1, nsp, W);
     DP_LAUNCH_CHECK();
     DP_TRY(spmvG<EPI_STORE>(false, V.s, V.u, nullptr, ga));      // u = G s
     DP_TRY(spmvG<EPI_STORE>(true, V.u, V.z, nullptr, ga));       // z = G^T G s
     DP_TRY(spmvA<EPI_STORE>(V.z, V.t, nullptr, ga));             // t = A z
     DP_TRY(spmvG<EPI_DOT2>(false, V.t, nullptr, V.u, ga));       // (Gt.Gs), (Gt.Gt)
-    k_sc_omega<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, nsp, S->nbx);
+    k_sc_omega<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, nsp, W);
     DP_LAUNCH_CHECK();
     pbegin();
-    k_update_xr<T, NSP><<<vgrid(), 256, 0, st>>>(V.x, V.h, V.z, V.r, V.s, V.t, V.rhat, bs.alpha,
-                                                bs.omega, bs.status, p1, p2, S->nn, S->nbx, S->tile_rows, ga);
+    k_update_xr<T, NSP><<<vgrid(), 256, 0, st>>>(V.x, V.h, V.z, V.r, V.s, V.t, V.rhat, bs.alpha, bs.omega,
+                                                bs.status, p1, p2, W, ga);
     DP_LAUNCH_CHECK();
-    pend(2, groups(true) * vbytes() * 8);
+    pend(2, rows(true) * rowbytes() * 8);
     DP_TRY(reset_count());
-    k_sc_beta<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, nsp, S->nbx, maxit);
+    const double rows_before = rows(true);  // groups still active before this update
+    k_sc_beta<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, nsp, W, maxit);
     DP_LAUNCH_CHECK();
-    const int ag = S->active_groups;  // groups still active before this update
     pbegin();
-    k_update_p<T, NSP><<<vgrid(), 256, 0, st>>>(V.p, V.r, V.v, bs.beta, bs.omega, S->nn, S->nbx, S->tile_rows, ga);
+    k_update_p<T, NSP><<<vgrid(), 256, 0, st>>>(V.p, V.r, V.v, bs.beta, bs.omega, W, ga);
     DP_LAUNCH_CHECK();
-    pend(2, (double)ag * vbytes() * 4);
+    pend(2, rows_before * rowbytes() * 4);
     return DOGPUP_OK;
   }
 
@@ -1274,15 +1366,15 @@ struct Chunk {
     const int* ga = bs.grp_active;
     const int nsp = NSP;
     DP_TRY(spmvA<EPI_STORE_DOT>(V.p, V.v, V.p, ga));             // q = A p, p.q
-    k_cg_alpha<<<gcount, kScalThreads, 0, st>>>(bs, p1, nsp, S->nbx);
+    k_cg_alpha<<<gcount, kScalThreads, 0, st>>>(bs, p1, nsp, W);
     DP_LAUNCH_CHECK();
     pbegin();
-    k_cg_update<T, NSP><<<vgrid(), 256, 0, st>>>(V.r, V.v, bs.alpha, p2, S->nn, S->nbx, S->tile_rows, ga);
+    k_cg_update<T, NSP><<<vgrid(), 256, 0, st>>>(V.r, V.v, bs.alpha, p2, W, ga);
     DP_LAUNCH_CHECK();
-    pend(2, groups(true) * vbytes() * 3);
+    pend(2, rows(true) * rowbytes() * 3);
     DP_TRY(spmvG<EPI_STORE_SELF>(false, V.r, V.u, nullptr, ga));  // u = G r, u.u
     DP_TRY(reset_count());
-    k_cg_beta<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, nsp, S->nbx, maxit);
+    k_cg_beta<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, nsp, W, maxit);
     DP_LAUNCH_CHECK();
     // p = G^T u + beta p, x += alpha p
     DP_TRY(spmvG<EPI_AXPBY_X>(true, V.u, V.p, nullptr, bs.grp_prev, bs.beta, bs.alpha, V.x));
@@ -1295,7 +1387,7 @@ struct Chunk {
     const int* ga = bs.grp_active;
     DP_TRY(spmvG<EPI_STORE_SELF>(false, V.r, V.u, nullptr, ga));
     DP_TRY(reset_count());
-    k_cg_start<<<gcount, kScalThreads, 0, st>>>(bs, p1, NSP, S->nbx);
+    k_cg_start<<<gcount, kScalThreads, 0, st>>>(bs, p1, NSP, W);
     DP_LAUNCH_CHECK();
     DP_TRY(spmvG<EPI_AXPBY>(true, V.u, V.p, nullptr, ga, bs.beta));
     return DOGPUP_OK;
@@ -1310,10 +1402,10 @@ struct Chunk {
   int run_fused(Vecs<T> V, const T* b, int maxit, const int* mask, int* iters_run) {
     const int nsp = NSP;
     if (b) {
-      k_init<T, NSP><<<vgrid(), 256, 0, st>>>(b, V.x, V.r, V.rhat, V.p, p1, p2, S->nn, S->nbx, S->tile_rows, nullptr);
+      k_init<T, NSP><<<vgrid(), 256, 0, st>>>(b, V.x, V.r, V.rhat, V.p, p1, p2, W, nullptr);
       DP_LAUNCH_CHECK();
       DP_TRY(reset_count());
-      k_sc_init<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, mask, nsp, S->nbx, S->ncb, S->ncols, g0);
+      k_sc_init<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, mask, nsp, W);
       DP_LAUNCH_CHECK();
     }
     if (S->method == DOGPUP_METHOD_COCG) DP_TRY(cocg_start(V));
@@ -1330,7 +1422,7 @@ struct Chunk {
       // Replay a captured iteration K times between convergence polls;
       // converged groups make every kernel exit early, so replays past
       // convergence cost only launch latency.
-      const long long key = ((long long)g0 << 32) ^ ((long long)maxit << 3) ^ ((long long)S->method << 2) ^
+      const long long key = ((long long)cidx << 32) ^ ((long long)maxit << 3) ^ ((long long)S->method << 2) ^
                             (sizeof(T) == 4 ? 1 : 2);
       hipGraphExec_t ex = nullptr;
       auto found = S->graphs.find(key);
@@ -1417,10 +1509,18 @@ __global__ void k_fill(double* __restrict__ a, double v, int n) {
   if (i < n) a[i] = v;
 }
 
-// Copy per-RHS results of one chunk into host arrays indexed freq*ncols+col.
+// Output index (freq*ncols + col within its mesh, meshes one after the
+// other) of column c of group g, or -1 for padding.
+static int out_index(const dogpup_solver* S, int g, int c) {
+  if (c >= S->h_gcols[g]) return -1;
+  const int m = S->h_gmesh[g];
+  return S->m_rhs0[m] + S->h_gf[g] * S->m_ncols[m] + S->h_gcb[g] * S->nsp + c;
+}
+
+// Copy per-RHS results of one chunk into the host arrays.
 static int export_results(dogpup_solver* S, hipStream_t st, int g0, int gcount, const int* d_iters,
-                          const double* d_relres, const int* d_status, int* h_iters,
-                          double* h_relres, int* h_status) {
+                          const double* d_relres, const int* d_status, int* h_iters, double* h_relres,
+                          int* h_status) {
   const int nrhs = gcount * S->nsp;
   std::vector<int> it(nrhs), stt(nrhs);
   std::vector<double> rr(nrhs);
@@ -1428,42 +1528,52 @@ static int export_results(dogpup_solver* S, hipStream_t st, int g0, int gcount, 
   DP_HIP(hipMemcpyAsync(stt.data(), d_status, nrhs * 4, hipMemcpyDeviceToHost, st));
   DP_HIP(hipMemcpyAsync(rr.data(), d_relres, nrhs * 8, hipMemcpyDeviceToHost, st));
   DP_HIP(hipStreamSynchronize(st));
-  for (int g = 0; g < gcount; ++g) {
-    const int gg = g0 + g, f = gg / S->ncb, cb = gg % S->ncb;
+  for (int g = 0; g < gcount; ++g)
     for (int c = 0; c < S->nsp; ++c) {
-      const int col = cb * S->nsp + c;
-      if (col >= S->ncols) continue;
-      const int o = f * S->ncols + col, i = g * S->nsp + c;
+      const int o = out_index(S, g0 + g, c), i = g * S->nsp + c;
+      if (o < 0) continue;
       int code = stt[i];
       if (code < 0) code = DOGPUP_RHS_MAXIT;
       if (h_iters) h_iters[o] = it[i];
       if (h_relres) h_relres[o] = rr[i];
       if (h_status) h_status[o] = code;
     }
-  }
   return DOGPUP_OK;
 }
 
-// Row permutation between the caller's node order and the solver's.
-static int permute_block(dogpup_solver* S, hipStream_t st, const double* src, double* dst, int gc,
+template <int NSP>
+static int permute_block(dogpup_solver* S, hipStream_t st, const Work& W, const double* src, double* dst,
                          int dir) {
-  const long long n = (long long)gc * S->nn * S->nsp;
-  k_permute_rows<<<grid1(n), 256, 0, st>>>(src, dst, S->order.as<int>(), S->nn, S->nsp, gc, dir);
+  k_permute_rows<NSP><<<dim3((unsigned)W.nw), 256, 0, st>>>(src, dst, S->order.as<int>(), dir, W);
   DP_LAUNCH_CHECK();
   return DOGPUP_OK;
+}
+
+template <typename T, int NSP>
+static Chunk<T, NSP> make_chunk(dogpup_solver* S, hipStream_t st, int c) {
+  Chunk<T, NSP> C;
+  C.S = S;
+  C.st = st;
+  C.cidx = c;
+  const int g0 = S->chunk_g[c], g1 = S->chunk_g[c + 1];
+  C.gcount = g1 - g0;
+  C.W = chunk_work(S, c);
+  C.bs = carve_state(S->state, C.gcount * NSP, C.gcount);
+  C.p1 = S->part1.as<double>();
+  C.p2 = S->part2.as<double>();
+  C.nodes = group_nodes(S, g0, g1);
+  return C;
 }
 
 template <int NSP>
 static int solve_c128(dogpup_solver* S, hipStream_t st, const double* b, double* x, double tol,
                       int maxit, int* h_iters, double* h_relres, int* h_status) {
-  const size_t per_group = (size_t)S->nn * NSP * 2 * sizeof(double);
-  for (int g0 = 0; g0 < S->ngroups; g0 += S->chunk) {
-    const int gc = std::min(S->chunk, S->ngroups - g0);
-    Chunk<double, NSP> C{S, st, gc, g0, carve_state(S->state, gc * NSP, gc), S->part1.as<double>(),
-                         S->part2.as<double>()};
-    Vecs<double> V = carve<double>(S->ws, gc * per_group, false, false);
-    const size_t goff = (size_t)g0 * S->nn * NSP * 2;
-    DP_TRY(permute_block(S, st, b + goff, S->bperm.as<double>(), gc, 0));
+  for (int c = 0; c + 1 < (int)S->chunk_g.size(); ++c) {
+    Chunk<double, NSP> C = make_chunk<double, NSP>(S, st, c);
+    const int gc = C.gcount, g0 = S->chunk_g[c];
+    Vecs<double> V = carve<double>(S->ws, (size_t)C.nodes * NSP * 16, false, false);
+    const size_t goff = (size_t)C.W.vb0 * NSP * 2;
+    DP_TRY(permute_block<NSP>(S, st, C.W, b + goff, S->bperm.as<double>(), 0));
     V.x = S->xperm.as<double>();
     const double* bc = S->bperm.as<double>();
     k_fill<<<div_up(gc * NSP, 256), 256, 0, st>>>(C.bs.tol, tol, gc * NSP);
@@ -1472,41 +1582,33 @@ static int solve_c128(dogpup_solver* S, hipStream_t st, const double* b, double*
     DP_TRY(C.run_fused(V, bc, maxit, nullptr, &its));
     S->stats[0] = std::max(S->stats[0], (double)its);
     // true residual for reporting
-    k_residual64<NSP><<<dim3(S->nbx, gc), 256, 0, st>>>(
-        S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), S->fscale.as<double>(), bc, V.x,
-        nullptr, nullptr, nullptr, nullptr, nullptr, 0, C.p1, C.p2, S->nn, S->ncb, g0, S->nbx, S->tile_rows);
+    k_residual64<NSP><<<C.vgrid(), 256, 0, st>>>(S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(),
+                                                 bc, V.x, nullptr, nullptr, nullptr, nullptr, nullptr, 0, C.p1,
+                                                 C.p2, C.W);
     DP_LAUNCH_CHECK();
-    // relres = |b - Ax| / |b|: reuse k_rf_check's arithmetic through a small pass
-    std::vector<double> part((size_t)gc * S->nbx * NSP * 2);
+    const int g1 = S->chunk_g[c + 1];
+    const int nparts = (g1 < S->ngroups ? S->h_gwbeg[g1] : S->nparts) - C.W.pw0;
+    std::vector<double> part((size_t)nparts * NSP * 2);
     DP_HIP(hipMemcpyAsync(part.data(), C.p1, part.size() * 8, hipMemcpyDeviceToHost, st));
     std::vector<double> bn(gc * NSP);
     DP_HIP(hipMemcpyAsync(bn.data(), C.bs.bnorm, bn.size() * 8, hipMemcpyDeviceToHost, st));
-    DP_TRY(permute_block(S, st, V.x, x + goff, gc, 1));
-    DP_TRY(export_results(S, st, g0, gc, C.bs.iters, C.bs.relres, C.bs.status, h_iters, h_relres,
-                          h_status));
+    DP_TRY(permute_block<NSP>(S, st, C.W, V.x, x + goff, 1));
+    DP_TRY(export_results(S, st, g0, gc, C.bs.iters, C.bs.relres, C.bs.status, h_iters, h_relres, h_status));
+    // relres = |b - A x| / |b| from the residual pass's partials
     for (int g = 0; g < gc; ++g) {
-      const int gg = g0 + g, f = gg / S->ncb, cb = gg % S->ncb;
-      for (int c = 0; c < NSP; ++c) {
-        const int col = cb * NSP + c;
-        if (col >= S->ncols) continue;
+      const int wb = S->h_gwbeg[g0 + g] - C.W.pw0, wc = S->h_gwcnt[g0 + g];
+      int mx = 0;
+      for (int cc = 0; cc < NSP; ++cc) {
+        const int o = out_index(S, g0 + g, cc);
+        if (o < 0) continue;
         double s2 = 0.0;
-        for (int bx = 0; bx < S->nbx; ++bx) s2 += part[2 * (((size_t)g * S->nbx + bx) * NSP + c)];
-        const int i = g * NSP + c;
-        if (h_relres) h_relres[f * S->ncols + col] = bn[i] > 0 ? std::sqrt(s2) / bn[i] : 0.0;
+        for (int k = 0; k < wc; ++k) s2 += part[2 * ((size_t)(wb + k) * NSP + cc)];
+        const int i = g * NSP + cc;
+        if (h_relres) h_relres[o] = bn[i] > 0 ? std::sqrt(s2) / bn[i] : 0.0;
+        if (h_iters) mx = std::max(mx, h_iters[o]);
       }
+      S->stats[1] += mx;
     }
-    double sum_it = 0;
-    if (h_iters)
-      for (int g = 0; g < gc; ++g) {
-        const int gg = g0 + g, f = gg / S->ncb, cb = gg % S->ncb;
-        int mx = 0;
-        for (int c = 0; c < NSP; ++c) {
-          const int col = cb * NSP + c;
-          if (col < S->ncols) mx = std::max(mx, h_iters[f * S->ncols + col]);
-        }
-        sum_it += mx;
-      }
-    S->stats[1] += sum_it;
   }
   return DOGPUP_OK;
 }
@@ -1514,58 +1616,51 @@ static int solve_c128(dogpup_solver* S, hipStream_t st, const double* b, double*
 template <int NSP>
 static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, double* x, double tol,
                         int maxit, int* h_iters, double* h_relres, int* h_status) {
-  const size_t per_group32 = (size_t)S->nn * NSP * 2 * sizeof(float);
   const char* env_tol = getenv("DOGPUP_INNER_TOL");
   const double tol_in = env_tol ? atof(env_tol) : 1e-4;
-  for (int g0 = 0; g0 < S->ngroups; g0 += S->chunk) {
-    const int gc = std::min(S->chunk, S->ngroups - g0);
-    Chunk<float, NSP> C{S, st, gc, g0, carve_state(S->state, gc * NSP, gc), S->part1.as<double>(),
-                        S->part2.as<double>()};
+  for (int c = 0; c + 1 < (int)S->chunk_g.size(); ++c) {
+    Chunk<float, NSP> C = make_chunk<float, NSP>(S, st, c);
+    const int gc = C.gcount, g0 = S->chunk_g[c];
     RState R = carve_rstate(S->rstate, gc * NSP);
-    Vecs<float> V = carve<float>(S->ws, gc * per_group32, true, false);
-    const size_t goff = (size_t)g0 * S->nn * NSP * 2;
-    DP_TRY(permute_block(S, st, b + goff, S->bperm.as<double>(), gc, 0));
+    Vecs<float> V = carve<float>(S->ws, (size_t)C.nodes * NSP * 8, true, false);
+    const size_t goff = (size_t)C.W.vb0 * NSP * 2;
+    DP_TRY(permute_block<NSP>(S, st, C.W, b + goff, S->bperm.as<double>(), 0));
     double* xc = S->xperm.as<double>();
     const double* bc = S->bperm.as<double>();
-    DP_HIP(hipMemsetAsync(xc, 0, (size_t)gc * S->nn * NSP * 16, st));
-    const double e64 = (double)S->nn * NSP;  // complex elements per group
+    DP_HIP(hipMemsetAsync(xc, 0, (size_t)C.nodes * NSP * 16, st));
+    const double e64 = (double)C.nodes * NSP;  // complex elements of the chunk
     for (int sweep = 0;; ++sweep) {
       C.pbegin();
-      k_residual64<NSP><<<dim3(S->nbx, gc), 256, 0, st>>>(
-          S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), S->fscale.as<double>(), bc,
-          sweep == 0 ? nullptr : xc, V.r, S->method == DOGPUP_METHOD_COCG ? nullptr : V.rhat, V.p, V.x,
-          sweep == 0 ? nullptr : R.status, 1, C.p1, C.p2, S->nn, S->ncb, g0, S->nbx, S->tile_rows);
+      k_residual64<NSP><<<C.vgrid(), 256, 0, st>>>(
+          S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), bc, sweep == 0 ? nullptr : xc, V.r,
+          S->method == DOGPUP_METHOD_COCG ? nullptr : V.rhat, V.p, V.x, sweep == 0 ? nullptr : R.status, 1, C.p1,
+          C.p2, C.W);
       DP_LAUNCH_CHECK();
-      C.pend(3, (sweep == 0 ? gc : S->active_groups) * e64 * (16 + 16 + 4 * 8) +
-                    (double)S->nnzA * 20 + 4.0 * (S->nn + 1));
+      C.pend(3, e64 * (16 + 16 + 4 * 8) + (double)S->nnzA * 20 + 4.0 * (S->nn + 1));
       DP_TRY(C.reset_count());
-      k_rf_check<<<gc, kScalThreads, 0, st>>>(R, C.bs, C.p1, NSP, S->nbx, S->ncb, S->ncols, g0, sweep, tol,
-                                    tol_in, maxit);
+      k_rf_check<<<gc, kScalThreads, 0, st>>>(R, C.bs, C.p1, NSP, C.W, sweep, tol, tol_in, maxit);
       DP_LAUNCH_CHECK();
       int active = 0;
       DP_TRY(C.wait_count(&active));
       if (active == 0) break;
-      const int active_groups_at_sweep = S->active_groups;
+      const double sweep_rows = (double)S->active_rows;
       // inner state from the residual pass: bnorm = |r64|, rho = r32.r32
-      k_sc_init<<<gc, kScalThreads, 0, st>>>(C.bs, C.p1, C.p2, R.status, NSP, S->nbx, S->ncb, S->ncols, g0);
+      k_sc_init<<<gc, kScalThreads, 0, st>>>(C.bs, C.p1, C.p2, R.status, NSP, C.W);
       DP_LAUNCH_CHECK();
       int its = 0;
       DP_TRY(C.run_fused(V, nullptr, maxit, R.status, &its));
-      const int sweep_groups = active_groups_at_sweep;
       C.pbegin();
-      k_add_correction<NSP><<<dim3(grid1((long long)S->nn * NSP / 4), gc), 256, 0, st>>>(
-          xc, V.x, C.bs.status, S->nn);
+      k_add_correction<NSP><<<C.vgrid(), 256, 0, st>>>(xc, V.x, C.bs.status, C.W);
       DP_LAUNCH_CHECK();
-      C.pend(3, sweep_groups * e64 * (32 + 8));
+      C.pend(3, sweep_rows * NSP * (32 + 8));
       k_rf_accum<<<gc, kScalThreads, 0, st>>>(R, C.bs, NSP);
       DP_LAUNCH_CHECK();
       S->stats[0] = std::max(S->stats[0], (double)its);
       S->stats[2] += 1;
-      // active-group-iterations for bandwidth accounting (host side estimate)
       S->stats[3] += its;
       if (sweep > 64) break;
     }
-    DP_TRY(permute_block(S, st, xc, x + goff, gc, 1));
+    DP_TRY(permute_block<NSP>(S, st, C.W, xc, x + goff, 1));
     DP_TRY(export_results(S, st, g0, gc, R.iters, R.relres, R.status, h_iters, h_relres, h_status));
   }
   return DOGPUP_OK;
@@ -1574,22 +1669,21 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
 template <int NSP>
 static int solve_plain(dogpup_solver* S, hipStream_t st, const double* b, double* x, double tol,
                        int maxit, int* h_iters, double* h_relres, int* h_status) {
-  const size_t per_group = (size_t)S->nn * NSP * 2 * sizeof(double);
-  for (int g0 = 0; g0 < S->ngroups; g0 += S->chunk) {
-    const int gc = std::min(S->chunk, S->ngroups - g0);
-    Chunk<double, NSP> C{S, st, gc, g0, carve_state(S->state, gc * NSP, gc), S->part1.as<double>(),
-                         S->part2.as<double>()};
-    Vecs<double> V = carve<double>(S->ws, gc * per_group, false, true);
-    const size_t goff = (size_t)g0 * S->nn * NSP * 2;
-    DP_TRY(permute_block(S, st, b + goff, S->bperm.as<double>(), gc, 0));
+  for (int c = 0; c + 1 < (int)S->chunk_g.size(); ++c) {
+    Chunk<double, NSP> C = make_chunk<double, NSP>(S, st, c);
+    const int gc = C.gcount, g0 = S->chunk_g[c];
+    const Work& W = C.W;
+    const size_t vbytes = (size_t)C.nodes * NSP * 16;
+    Vecs<double> V = carve<double>(S->ws, vbytes, false, true);
+    const size_t goff = (size_t)W.vb0 * NSP * 2;
+    DP_TRY(permute_block<NSP>(S, st, W, b + goff, S->bperm.as<double>(), 0));
     V.x = S->xperm.as<double>();
     const double* bc = S->bperm.as<double>();
     unsigned long long* dmax = reinterpret_cast<unsigned long long*>(C.bs.counter);
     const dim3 vg = C.vgrid();
-    const int nn = S->nn, nbx = S->nbx;
     auto maxerr = [&](int op, const double* pa, double* out) -> int {
       DP_HIP(hipMemsetAsync(dmax, 0, 8, st));
-      k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, op, pa, nullptr, NSP, nbx, S->ncb, S->ncols, g0, dmax);
+      k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, op, pa, nullptr, NSP, W, dmax);
       DP_LAUNCH_CHECK();
       DP_HIP(hipMemcpyAsync(S->h_max, dmax, 8, hipMemcpyDeviceToHost, st));
       DP_HIP(hipStreamSynchronize(st));
@@ -1600,29 +1694,29 @@ static int solve_plain(dogpup_solver* S, hipStream_t st, const double* b, double
       return DOGPUP_OK;
     };
     // x = 0, r = b, rhat0 = r, p = r
-    k_init<double, NSP><<<vg, 256, 0, st>>>(bc, V.x, V.r, V.rhat, V.p, C.p1, C.p2, nn, nbx, S->tile_rows, nullptr);
+    k_init<double, NSP><<<vg, 256, 0, st>>>(bc, V.x, V.r, V.rhat, V.p, C.p1, C.p2, W, nullptr);
     DP_LAUNCH_CHECK();
-    k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 5, C.p1, C.p2, NSP, nbx, S->ncb, S->ncols, g0, dmax);
+    k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 5, C.p1, C.p2, NSP, W, dmax);
     DP_LAUNCH_CHECK();
     int it = 0;
     for (it = 1; it <= maxit; ++it) {
       DP_TRY(C.template spmvG<EPI_STORE>(false, V.p, V.u, nullptr, nullptr));   // h = G p
       DP_TRY(C.template spmvG<EPI_STORE>(true, V.u, V.h, nullptr, nullptr));    // h = G^T h
       DP_TRY(C.template spmvA<EPI_STORE>(V.h, V.v, nullptr, nullptr));          // v = A h
-      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.v, V.rhat, 0, C.p1, nn, nbx, S->tile_rows, nullptr);
+      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.v, V.rhat, 0, C.p1, W, nullptr);
       DP_LAUNCH_CHECK();
-      k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 0, C.p1, nullptr, NSP, nbx, S->ncb, S->ncols, g0, dmax);
+      k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 0, C.p1, nullptr, NSP, W, dmax);
       DP_LAUNCH_CHECK();
-      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.u, V.x, V.h, C.bs.alpha, 0, nn, nbx, S->tile_rows, nullptr);  // hx = x + h/alph
+      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.u, V.x, V.h, C.bs.alpha, 0, W, nullptr);  // hx = x + h/alph
       DP_LAUNCH_CHECK();
-      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.s, V.r, V.v, C.bs.alpha, 1, nn, nbx, S->tile_rows, nullptr);  // s = r - v/alph
+      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.s, V.r, V.v, C.bs.alpha, 1, W, nullptr);  // s = r - v/alph
       DP_LAUNCH_CHECK();
-      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.s, nullptr, 1, C.p1, nn, nbx, S->tile_rows, nullptr);
+      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.s, nullptr, 1, C.p1, W, nullptr);
       DP_LAUNCH_CHECK();
       double err = 0;
       DP_TRY(maxerr(1, C.p1, &err));
       if (err < tol) {
-        DP_HIP(hipMemcpyAsync(V.x, V.u, (size_t)gc * per_group, hipMemcpyDeviceToDevice, st));
+        DP_HIP(hipMemcpyAsync(V.x, V.u, vbytes, hipMemcpyDeviceToDevice, st));
         break;
       }
       std::swap(V.h, V.u);  // h now holds x + h/alph
@@ -1631,50 +1725,46 @@ static int solve_plain(dogpup_solver* S, hipStream_t st, const double* b, double
       DP_TRY(C.template spmvA<EPI_STORE>(V.z, V.t, nullptr, nullptr));          // t
       // Gtt = G t is materialised, as the reference does
       DP_TRY(C.template spmvG<EPI_STORE>(false, V.t, V.w, nullptr, nullptr));
-      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.w, V.u, 0, C.p1, nn, nbx, S->tile_rows, nullptr);
+      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.w, V.u, 0, C.p1, W, nullptr);
       DP_LAUNCH_CHECK();
-      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.w, V.w, 0, C.p2, nn, nbx, S->tile_rows, nullptr);
+      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.w, V.w, 0, C.p2, W, nullptr);
       DP_LAUNCH_CHECK();
-      k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 2, C.p1, C.p2, NSP, nbx, S->ncb, S->ncols, g0, dmax);
+      k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 2, C.p1, C.p2, NSP, W, dmax);
       DP_LAUNCH_CHECK();
-      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.x, V.h, V.z, C.bs.omega, 2, nn, nbx, S->tile_rows, nullptr);  // x = h + om z
+      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.x, V.h, V.z, C.bs.omega, 2, W, nullptr);  // x = h + om z
       DP_LAUNCH_CHECK();
-      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.r, V.s, V.t, C.bs.omega, 3, nn, nbx, S->tile_rows, nullptr);  // r = s - om t
+      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.r, V.s, V.t, C.bs.omega, 3, W, nullptr);  // r = s - om t
       DP_LAUNCH_CHECK();
-      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.p, V.p, V.v, C.bs.omega, 3, nn, nbx, S->tile_rows, nullptr);  // p = p - om v
+      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.p, V.p, V.v, C.bs.omega, 3, W, nullptr);  // p = p - om v
       DP_LAUNCH_CHECK();
-      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.r, nullptr, 1, C.p1, nn, nbx, S->tile_rows, nullptr);
+      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.r, nullptr, 1, C.p1, W, nullptr);
       DP_LAUNCH_CHECK();
       DP_TRY(maxerr(3, C.p1, &err));
       if (err < tol) break;
-      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.rhat, V.r, 0, C.p1, nn, nbx, S->tile_rows, nullptr);
+      k_dot<double, NSP><<<vg, 256, 0, st>>>(V.rhat, V.r, 0, C.p1, W, nullptr);
       DP_LAUNCH_CHECK();
-      k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 4, C.p1, nullptr, NSP, nbx, S->ncb, S->ncols, g0, dmax);
+      k_plain_scal<<<gc, kScalThreads, 0, st>>>(C.bs, 4, C.p1, nullptr, NSP, W, dmax);
       DP_LAUNCH_CHECK();
-      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.p, V.r, V.p, C.bs.beta, 2, nn, nbx, S->tile_rows, nullptr);  // p = r + beta p
+      k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.p, V.r, V.p, C.bs.beta, 2, W, nullptr);  // p = r + beta p
       DP_LAUNCH_CHECK();
     }
-    DP_TRY(permute_block(S, st, V.x, x + goff, gc, 1));
+    DP_TRY(permute_block<NSP>(S, st, W, V.x, x + goff, 1));
     S->stats[0] = std::max(S->stats[0], (double)std::min(it, maxit));
     S->stats[1] += (double)std::min(it, maxit) * gc;
     // status: converged if the lockstep loop met tol
-    std::vector<int> stt(gc * NSP);
     std::vector<double> rr(gc * NSP), bn(gc * NSP);
     DP_HIP(hipMemcpyAsync(rr.data(), C.bs.relres, rr.size() * 8, hipMemcpyDeviceToHost, st));
     DP_HIP(hipMemcpyAsync(bn.data(), C.bs.bnorm, bn.size() * 8, hipMemcpyDeviceToHost, st));
     DP_HIP(hipStreamSynchronize(st));
-    for (int g = 0; g < gc; ++g) {
-      const int gg = g0 + g, f = gg / S->ncb, cb = gg % S->ncb;
-      for (int c = 0; c < NSP; ++c) {
-        const int col = cb * NSP + c;
-        if (col >= S->ncols) continue;
-        const int o = f * S->ncols + col, i = g * NSP + c;
+    for (int g = 0; g < gc; ++g)
+      for (int cc = 0; cc < NSP; ++cc) {
+        const int o = out_index(S, g0 + g, cc), i = g * NSP + cc;
+        if (o < 0) continue;
         if (h_iters) h_iters[o] = std::min(it, maxit);
         if (h_relres) h_relres[o] = rr[i];
-        if (h_status) h_status[o] = bn[i] == 0.0 ? DOGPUP_RHS_ZERO
-                                    : (rr[i] < tol ? DOGPUP_RHS_CONVERGED : DOGPUP_RHS_MAXIT);
+        if (h_status) h_status[o] = bn[i] == 0.0 ? DOGPUP_RHS_ZERO : (rr[i] < tol ? DOGPUP_RHS_CONVERGED
+                                                                                  : DOGPUP_RHS_MAXIT);
       }
-    }
   }
   return DOGPUP_OK;
 }
@@ -1683,7 +1773,7 @@ static int solve_plain(dogpup_solver* S, hipStream_t st, const double* b, double
 
 namespace dogpup {
 
-// Solver-layout transforms and point-source right-hand sides
+// Solver-layout transforms and point-source right-hand sides (one mesh)
 __global__ void k_to_solver(const double* __restrict__ src, double* __restrict__ dst, int nn, int nf,
                             int ncols, int nsp, int ncb) {
   // dst (g, node, c) <- src(node, f, col) (MATLAB NN x NF x NS), zero padding
@@ -1746,9 +1836,8 @@ static int by_nsp(int nsp, A... a) {
     case 8: return F<8>::run(a...);
     case 16: return F<16>::run(a...);
     case 32: return F<32>::run(a...);
-    case 64: return F<64>::run(a...);
+    default: return F<64>::run(a...);
   }
-  return fail(DOGPUP_ERR_ARG, "unsupported group width %d", nsp);
 }
 
 template <int NSP>
@@ -1780,39 +1869,114 @@ static double ws_budget_bytes() {
   return gb * 1073741824.0;
 }
 
-}  // namespace dogpup
+static int upload(Buf& b, const void* h, size_t bytes) {
+  DP_TRY(b.ensure(bytes));
+  DP_HIP(hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice));
+  return DOGPUP_OK;
+}
 
-extern "C" {
+// Work list for the current tile size, the chunk plan for the workspace
+// budget, and the buffers sized by them.
+static int build_work(dogpup_solver* S) {
+  const int ng = S->ngroups, tr = S->tile_rows;
+  std::vector<int> wg, wt, wp;
+  S->h_gibeg.assign(ng, 0);
+  S->h_gwbeg.assign(ng, 0);
+  S->h_gwcnt.assign(ng, 0);
+  int parts = 0;
+  for (int g = 0; g < ng; ++g) {
+    const int nt = (S->h_grows[g] + tr - 1) / tr;
+    const int ntp = (nt + 7) / 8 * 8, per = ntp / 8;
+    S->h_gibeg[g] = (int)wg.size();
+    S->h_gwbeg[g] = parts;
+    S->h_gwcnt[g] = nt;
+    // item k runs on XCD k % 8 and takes tile (k % 8) * per + k / 8
+    for (int k = 0; k < ntp; ++k) {
+      const int t = (k & 7) * per + (k >> 3);
+      wg.push_back(g);
+      wt.push_back(t < nt ? t : -1);
+      wp.push_back(t < nt ? parts + t : 0);
+    }
+    parts += nt;
+  }
+  S->nwork = (int)wg.size();
+  S->nparts = parts;
+  DP_TRY(upload(S->t_wgrp, wg.data(), wg.size() * 4));
+  DP_TRY(upload(S->t_wtile, wt.data(), wt.size() * 4));
+  DP_TRY(upload(S->t_wpart, wp.data(), wp.size() * 4));
+  DP_TRY(upload(S->t_gwbeg, S->h_gwbeg.data(), ng * 4));
+  DP_TRY(upload(S->t_gwcnt, S->h_gwcnt.data(), ng * 4));
+  // chunks: consecutive groups whose vectors fit the workspace budget
+  const size_t csize = S->mode == DOGPUP_MODE_C64_REFINE ? 8 : 16;
+  const double per_node = (double)S->nsp * (csize * nvec_for(S->mode) + 2.0 * 16);
+  const double budget = ws_budget_bytes();
+  S->chunk_g.assign(1, 0);
+  double acc = 0;
+  for (int g = 0; g < ng; ++g) {
+    const double bg = per_node * S->h_grows[g];
+    if (acc > 0 && acc + bg > budget) {
+      S->chunk_g.push_back(g);
+      acc = 0;
+    }
+    acc += bg;
+  }
+  S->chunk_g.push_back(ng);
+  long long max_nodes = 0;
+  int max_groups = 0, max_work = 0;
+  for (size_t c = 0; c + 1 < S->chunk_g.size(); ++c) {
+    const int g0 = S->chunk_g[c], g1 = S->chunk_g[c + 1];
+    max_nodes = std::max(max_nodes, group_nodes(S, g0, g1));
+    max_groups = std::max(max_groups, g1 - g0);
+    max_work = std::max(max_work, (g1 < ng ? S->h_gwbeg[g1] : S->nparts) - S->h_gwbeg[g0]);
+  }
+  const size_t vbytes = align256((size_t)max_nodes * S->nsp * csize);
+  DP_TRY(S->ws.ensure((size_t)nvec_for(S->mode) * vbytes));
+  DP_TRY(S->bperm.ensure((size_t)max_nodes * S->nsp * 16));
+  DP_TRY(S->xperm.ensure((size_t)max_nodes * S->nsp * 16));
+  const size_t pbytes = (size_t)max_work * S->nsp * 16;
+  DP_TRY(S->part1.ensure(pbytes));
+  DP_TRY(S->part2.ensure(pbytes));
+  DP_TRY(S->state.ensure(state_bytes(max_groups * S->nsp, max_groups)));
+  DP_TRY(S->rstate.ensure(5 * align256((size_t)max_groups * S->nsp * 8)));
+  S->clear_graphs();
+  return DOGPUP_OK;
+}
 
-int dogpup_solver_create(dogpup_solver_t* out, int nn, int nfreq, int ncols, int nsp, int mode,
-                         int nnzA, const int* d_rptrA, const int* d_cptrA, int nnzG,
-                         const int* d_rptrG, const int* d_cptrG, const int* h_order) {
+static int create_batch(dogpup_solver_t* out, int nmesh, const int* h_row0, const int* h_nfreq, const int* h_ncols,
+                        int nsp, int mode, int nnzA, const int* d_rptrA, const int* d_cptrA, int nnzG,
+                        const int* d_rptrG, const int* d_cptrG, const int* h_order) {
   DP_ARG(out != nullptr, "out is NULL");
   *out = nullptr;
-  DP_ARG(nn > 0 && nfreq > 0 && ncols > 0 && nnzA > 0 && nnzG > 0, "bad dimensions");
+  DP_ARG(nmesh >= 1 && h_row0 && h_nfreq && h_ncols, "bad mesh description");
+  DP_ARG(h_row0[0] == 0, "row0[0] must be 0");
+  for (int i = 0; i < nmesh; ++i)
+    DP_ARG(h_row0[i + 1] > h_row0[i] && h_nfreq[i] > 0 && h_ncols[i] > 0, "mesh %d: bad rows, nfreq or ncols", i);
+  const int nn = h_row0[nmesh];
+  DP_ARG(nnzA > 0 && nnzG > 0, "bad dimensions");
   DP_ARG(check_nsp(nsp), "group width must be 8, 16, 32 or 64 (got %d)", nsp);
   DP_ARG(mode == DOGPUP_MODE_C128 || mode == DOGPUP_MODE_C64_REFINE || mode == DOGPUP_MODE_C128_PLAIN,
          "unknown mode %d", mode);
   DP_ARG(d_rptrA && d_cptrA && d_rptrG && d_cptrG, "NULL pattern");
   std::unique_ptr<dogpup_solver> S(new dogpup_solver());
-  S->nn = nn; S->nf = nfreq; S->ncols = ncols; S->nsp = nsp; S->mode = mode;
-  S->ncb = (ncols + nsp - 1) / nsp;
-  S->ngroups = nfreq * S->ncb;
+  S->nmesh = nmesh; S->nn = nn; S->nsp = nsp; S->mode = mode;
   S->nnzA = nnzA; S->nnzG = nnzG;
   S->rA0 = d_rptrA; S->cA0 = d_cptrA; S->rG0 = d_rptrG; S->cG0 = d_cptrG;
-  // Row tile of the vector/SpMV kernels (a multiple of 64 rows); gridDim.x
-  // is a multiple of 8 (one contiguous tile range per XCD).
+  // Row tile of the kernels (a multiple of 64 rows).
   if (const char* e = getenv("DOGPUP_TILE")) S->tile_rows = atoi(e);
   if (S->tile_rows < 64 || S->tile_rows % 64) S->tile_rows = 512;
   if (const char* e = getenv("DOGPUP_SPMV_LDS_PAD")) S->lds_pad = atoi(e);
-  S->nbx = (int)(div_up(div_up(nn, S->tile_rows), 8) * 8);
-  DP_ARG(S->ngroups <= 65535, "too many groups (%d)", S->ngroups);
+  if (const char* e = getenv("DOGPUP_SPMV_UA")) S->unroll_a = atoi(e);
+  if (const char* e = getenv("DOGPUP_SPMV_UG")) S->unroll_g = atoi(e);
 
   // solver node order and the permuted patterns (host-side, once per handle)
+  std::vector<int> mesh_of(nn);
+  for (int i = 0; i < nmesh; ++i)
+    for (int r = h_row0[i]; r < h_row0[i + 1]; ++r) mesh_of[r] = i;
   std::vector<int> order(nn), inv(nn, -1);
   for (int i = 0; i < nn; ++i) order[i] = h_order ? h_order[i] : i;
   for (int i = 0; i < nn; ++i) {
     DP_ARG(order[i] >= 0 && order[i] < nn && inv[order[i]] < 0, "node order is not a permutation");
+    DP_ARG(mesh_of[order[i]] == mesh_of[i], "node order moves row %d to another mesh", i);
     inv[order[i]] = i;
   }
   std::vector<int> rA(nn + 1), cA(nnzA), rG(nn + 1), cG(nnzG);
@@ -1822,8 +1986,12 @@ int dogpup_solver_create(dogpup_solver_t* out, int nn, int nfreq, int ncols, int
   DP_HIP(hipMemcpy(cG.data(), d_cptrG, (size_t)nnzG * 4, hipMemcpyDeviceToHost));
   DP_ARG(rA[0] == 0 && rA[nn] == nnzA && rG[0] == 0 && rG[nn] == nnzG, "CSR row pointers inconsistent with nnz");
   for (int i = 0; i < nn; ++i) DP_ARG(rA[i] <= rA[i + 1] && rG[i] <= rG[i + 1], "CSR row pointers decrease");
-  for (int v : cA) DP_ARG(v >= 0 && v < nn, "A column index out of range");
-  for (int v : cG) DP_ARG(v >= 0 && v < nn, "G column index out of range");
+  for (int r = 0; r < nn; ++r) {
+    for (int p = rA[r]; p < rA[r + 1]; ++p)
+      DP_ARG(cA[p] >= 0 && cA[p] < nn && mesh_of[cA[p]] == mesh_of[r], "A column index out of its mesh (row %d)", r);
+    for (int p = rG[r]; p < rG[r + 1]; ++p)
+      DP_ARG(cG[p] >= 0 && cG[p] < nn && mesh_of[cG[p]] == mesh_of[r], "G column index out of its mesh (row %d)", r);
+  }
   auto permute = [&](const std::vector<int>& rp, const std::vector<int>& cp, std::vector<int>& rq,
                      std::vector<int>& cq, std::vector<int>& map) {
     rq.assign(nn + 1, 0);
@@ -1858,39 +2026,61 @@ int dogpup_solver_create(dogpup_solver_t* out, int nn, int nfreq, int ncols, int
         gtmap[q] = gmap[p];
       }
   }
-  auto up = [&](dogpup::Buf& b, const std::vector<int>& v) -> int {
-    DP_TRY(b.ensure(v.size() * 4));
-    DP_HIP(hipMemcpy(b.p, v.data(), v.size() * 4, hipMemcpyHostToDevice));
-    return DOGPUP_OK;
-  };
+  auto up = [&](dogpup::Buf& b, const std::vector<int>& v) -> int { return upload(b, v.data(), v.size() * 4); };
   DP_TRY(up(S->pAr, rAp)); DP_TRY(up(S->pAc, cAp)); DP_TRY(up(S->amap, amap));
   DP_TRY(up(S->pGr, rGp)); DP_TRY(up(S->pGc, cGp)); DP_TRY(up(S->gmap, gmap));
   DP_TRY(up(S->pGTr, rGT)); DP_TRY(up(S->pGTc, cGT)); DP_TRY(up(S->gtmap, gtmap));
   DP_TRY(up(S->order, order));
-  if (const char* e = getenv("DOGPUP_SPMV_UA")) S->unroll_a = atoi(e);
-  if (const char* e = getenv("DOGPUP_SPMV_UG")) S->unroll_g = atoi(e);
   S->rA = S->pAr.as<int>(); S->cA = S->pAc.as<int>();
   S->rG = S->pGr.as<int>(); S->cG = S->pGc.as<int>();
   S->rGT = S->pGTr.as<int>(); S->cGT = S->pGTc.as<int>();
 
-  const size_t csize = mode == DOGPUP_MODE_C64_REFINE ? 8 : 16;
-  const size_t per_group = align256((size_t)nn * nsp * csize);
-  const size_t per_group64 = align256((size_t)nn * nsp * 16);
-  const int nvec = nvec_for(mode);
-  // vector workspace + permuted b and x (complex128) per group
-  long long chunk = (long long)(ws_budget_bytes() / ((double)per_group * nvec + 2.0 * per_group64));
-  if (chunk < 1) chunk = 1;
-  if (chunk > S->ngroups) chunk = S->ngroups;
-  S->chunk = (int)chunk;
-  DP_TRY(S->ws.ensure((size_t)nvec * align256(per_group * S->chunk)));
-  DP_TRY(S->bperm.ensure(per_group64 * S->chunk));
-  DP_TRY(S->xperm.ensure(per_group64 * S->chunk));
-  const size_t pbytes = (size_t)S->chunk * S->nbx * nsp * 16;
-  DP_TRY(S->part1.ensure(pbytes));
-  DP_TRY(S->part2.ensure(pbytes));
-  DP_TRY(S->state.ensure(state_bytes(S->chunk * nsp, S->chunk)));
-  DP_TRY(S->rstate.ensure(5 * align256((size_t)S->chunk * nsp * 8)));
-  DP_TRY(S->fscale.ensure((size_t)nfreq * 8));
+  // meshes and groups
+  S->m_row0.assign(h_row0, h_row0 + nmesh + 1);
+  S->m_nf.assign(h_nfreq, h_nfreq + nmesh);
+  S->m_ncols.assign(h_ncols, h_ncols + nmesh);
+  long long gbase = 0, vb = 0;
+  int fs = 0, rhs = 0, g = 0;
+  for (int i = 0; i < nmesh; ++i) {
+    const int ncb = (h_ncols[i] + nsp - 1) / nsp, rows = h_row0[i + 1] - h_row0[i];
+    const int pG0 = rG[h_row0[i]], nnzGi = rG[h_row0[i + 1]] - pG0;
+    S->m_ncb.push_back(ncb);
+    S->m_g0.push_back(g);
+    S->m_pA0.push_back(rA[h_row0[i]]);
+    S->m_pG0.push_back(pG0);
+    S->m_nnzG.push_back(nnzGi);
+    S->m_gbase.push_back(gbase);
+    S->m_fs0.push_back(fs);
+    S->m_rhs0.push_back(rhs);
+    for (int f = 0; f < h_nfreq[i]; ++f)
+      for (int cb = 0; cb < ncb; ++cb, ++g) {
+        S->h_grow0.push_back(h_row0[i]);
+        S->h_grows.push_back(rows);
+        S->h_gvb.push_back(vb);
+        S->h_ggv.push_back(gbase + (long long)f * nnzGi - pG0);
+        S->h_gcols.push_back(std::min(nsp, h_ncols[i] - cb * nsp));
+        S->h_gmesh.push_back(i);
+        S->h_gf.push_back(f);
+        S->h_gcb.push_back(cb);
+        vb += rows;
+      }
+    gbase += (long long)h_nfreq[i] * nnzGi;
+    fs += h_nfreq[i];
+    rhs += h_nfreq[i] * h_ncols[i];
+  }
+  S->ngroups = g;
+  S->gvals = gbase;
+  S->nf_total = fs;
+  S->nrhs_out = rhs;
+  DP_ARG(S->ngroups <= (1 << 30), "too many groups (%d)", S->ngroups);
+  DP_TRY(upload(S->t_grow0, S->h_grow0.data(), g * 4));
+  DP_TRY(upload(S->t_grows, S->h_grows.data(), g * 4));
+  DP_TRY(upload(S->t_gvb, S->h_gvb.data(), g * 8));
+  DP_TRY(upload(S->t_ggv, S->h_ggv.data(), g * 8));
+  DP_TRY(upload(S->t_gcols, S->h_gcols.data(), g * 4));
+  DP_TRY(S->t_gfsc.ensure((size_t)g * 8));
+  DP_TRY(S->fscale.ensure((size_t)fs * 8));
+  DP_TRY(build_work(S.get()));
   DP_HIP(hipHostMalloc((void**)&S->h_counter, 16, hipHostMallocDefault));
   DP_HIP(hipHostMalloc((void**)&S->h_max, 16, hipHostMallocDefault));
   DP_HIP(hipStreamCreateWithFlags(&S->own, hipStreamNonBlocking));
@@ -1900,6 +2090,26 @@ int dogpup_solver_create(dogpup_solver_t* out, int nn, int nfreq, int ncols, int
   if (const char* e = getenv("DOGPUP_GRAPH_K")) S->graph_k = std::max(1, atoi(e));
   *out = S.release();
   return DOGPUP_OK;
+}
+
+}  // namespace dogpup
+
+extern "C" {
+
+int dogpup_solver_create(dogpup_solver_t* out, int nn, int nfreq, int ncols, int nsp, int mode,
+                         int nnzA, const int* d_rptrA, const int* d_cptrA, int nnzG,
+                         const int* d_rptrG, const int* d_cptrG, const int* h_order) {
+  DP_ARG(nn > 0 && nfreq > 0 && ncols > 0, "bad dimensions");
+  const int row0[2] = {0, nn};
+  return create_batch(out, 1, row0, &nfreq, &ncols, nsp, mode, nnzA, d_rptrA, d_cptrA, nnzG, d_rptrG, d_cptrG,
+                      h_order);
+}
+
+int dogpup_batch_create(dogpup_solver_t* out, int nmesh, const int* h_row0, const int* h_nfreq,
+                        const int* h_ncols, int nsp, int mode, int nnzA, const int* d_rptrA, const int* d_cptrA,
+                        int nnzG, const int* d_rptrG, const int* d_cptrG, const int* h_order) {
+  return create_batch(out, nmesh, h_row0, h_nfreq, h_ncols, nsp, mode, nnzA, d_rptrA, d_cptrA, nnzG, d_rptrG,
+                      d_cptrG, h_order);
 }
 
 int dogpup_solver_destroy(dogpup_solver_t s) {
@@ -1920,35 +2130,52 @@ int dogpup_solver_set_matrix(dogpup_solver_t S, const double* d_valA_re, const d
   DP_TRY(S->a64im.ensure(nA * 8));
   DP_HIP(hipMemcpyAsync(S->a64re.p, d_valA_re, nA * 8, hipMemcpyDeviceToDevice, st));
   DP_HIP(hipMemcpyAsync(S->a64im.p, d_valA_im, nA * 8, hipMemcpyDeviceToDevice, st));
-  DP_HIP(hipMemcpyAsync(S->fscale.p, d_fscale, (size_t)S->nf * 8, hipMemcpyDeviceToDevice, st));
-  // FSAI factors in the reference node order (pattern = the caller's tril(A))
-  const size_t nG = (size_t)S->nnzG * S->nf;
+  DP_HIP(hipMemcpyAsync(S->fscale.p, d_fscale, (size_t)S->nf_total * 8, hipMemcpyDeviceToDevice, st));
+  // per-group frequency scale
+  std::vector<double> hf(S->nf_total), gf(S->ngroups);
+  DP_HIP(hipMemcpyAsync(hf.data(), d_fscale, hf.size() * 8, hipMemcpyDeviceToHost, st));
+  DP_HIP(hipStreamSynchronize(st));
+  for (int g = 0; g < S->ngroups; ++g) gf[g] = hf[S->m_fs0[S->h_gmesh[g]] + S->h_gf[g]];
+  DP_HIP(hipMemcpy(S->t_gfsc.p, gf.data(), gf.size() * 8, hipMemcpyHostToDevice));
+  // FSAI factors in the reference node order (pattern = the caller's tril(A)),
+  // one launch per mesh: mesh i's values for frequency f at
+  // m_gbase[i] + f*nnzG_i + (p - pG0_i)
+  const size_t nG = (size_t)S->gvals;
   DP_TRY(S->g64.ensure(nG * 16));
-  FsaiOut fo{S->g64.as<double>(), nullptr, S->nnzG, FSAI_OUT_SOLVER};
-  DP_TRY(fsai_launch(S->a64re.as<double>(), S->a64im.as<double>(), 0, S->rA0, S->cA0, S->rG0, S->cG0,
-                     S->nn, S->nf, S->fscale.as<double>(), fo, st));
+  for (int i = 0; i < S->nmesh; ++i) {
+    FsaiOut fo{S->g64.as<double>() + 2 * (S->m_gbase[i] - S->m_pG0[i]), nullptr, S->m_nnzG[i], FSAI_OUT_SOLVER};
+    DP_TRY(fsai_launch(S->a64re.as<double>(), S->a64im.as<double>(), 0, S->rA0, S->cA0, S->rG0 + S->m_row0[i],
+                       S->cG0, S->m_row0[i + 1] - S->m_row0[i], S->m_nf[i], S->fscale.as<double>() + S->m_fs0[i],
+                       fo, st));
+  }
   // values in solver node order
   DP_TRY(S->a64pre.ensure(nA * 8));
   DP_TRY(S->a64pim.ensure(nA * 8));
   k_gather_real<double><<<grid1(nA), 256, 0, st>>>(S->a64re.as<double>(), S->amap.as<int>(), S->a64pre.as<double>(), nA);
   k_gather_real<double><<<grid1(nA), 256, 0, st>>>(S->a64im.as<double>(), S->amap.as<int>(), S->a64pim.as<double>(), nA);
   DP_LAUNCH_CHECK();
+  auto gather_g = [&](auto* tag, Buf& dst, const Buf& map) -> int {
+    using TO = std::remove_pointer_t<decltype(tag)>;
+    DP_TRY(dst.ensure(nG * 2 * sizeof(TO)));
+    for (int i = 0; i < S->nmesh; ++i) {
+      const long long n = (long long)S->m_nnzG[i] * S->m_nf[i];
+      k_gather_cplx<TO><<<grid1(n), 256, 0, st>>>(S->g64.as<double>(), map.as<int>(), dst.as<TO>(), S->m_pG0[i],
+                                                  S->m_nnzG[i], S->m_nf[i], S->m_gbase[i]);
+      DP_LAUNCH_CHECK();
+    }
+    return DOGPUP_OK;
+  };
   if (S->mode == DOGPUP_MODE_C64_REFINE) {
     DP_TRY(S->a32re.ensure(nA * 4));
     DP_TRY(S->a32im.ensure(nA * 4));
     k_gather_real<float><<<grid1(nA), 256, 0, st>>>(S->a64re.as<double>(), S->amap.as<int>(), S->a32re.as<float>(), nA);
     k_gather_real<float><<<grid1(nA), 256, 0, st>>>(S->a64im.as<double>(), S->amap.as<int>(), S->a32im.as<float>(), nA);
-    DP_TRY(S->g32.ensure(nG * 8));
-    DP_TRY(S->gt32.ensure(nG * 8));
-    k_gather_cplx<float><<<grid1(nG), 256, 0, st>>>(S->g64.as<double>(), S->gmap.as<int>(), S->g32.as<float>(), S->nnzG, S->nf);
-    k_gather_cplx<float><<<grid1(nG), 256, 0, st>>>(S->g64.as<double>(), S->gtmap.as<int>(), S->gt32.as<float>(), S->nnzG, S->nf);
     DP_LAUNCH_CHECK();
+    DP_TRY(gather_g((float*)nullptr, S->g32, S->gmap));
+    DP_TRY(gather_g((float*)nullptr, S->gt32, S->gtmap));
   } else {
-    DP_TRY(S->gp64.ensure(nG * 16));
-    DP_TRY(S->gtp64.ensure(nG * 16));
-    k_gather_cplx<double><<<grid1(nG), 256, 0, st>>>(S->g64.as<double>(), S->gmap.as<int>(), S->gp64.as<double>(), S->nnzG, S->nf);
-    k_gather_cplx<double><<<grid1(nG), 256, 0, st>>>(S->g64.as<double>(), S->gtmap.as<int>(), S->gtp64.as<double>(), S->nnzG, S->nf);
-    DP_LAUNCH_CHECK();
+    DP_TRY(gather_g((double*)nullptr, S->gp64, S->gmap));
+    DP_TRY(gather_g((double*)nullptr, S->gtp64, S->gtmap));
   }
   S->have_matrix = true;
   return DOGPUP_OK;
@@ -2005,6 +2232,24 @@ int dogpup_solver_set_method(dogpup_solver_t S, int method) {
   return DOGPUP_OK;
 }
 
+int dogpup_solver_set_tuning(dogpup_solver_t S, int tile_rows, int unroll_a, int unroll_g, int graph_k) {
+  DP_ARG(S != nullptr, "NULL solver");
+  DP_ARG(tile_rows == 0 || (tile_rows >= 64 && tile_rows % 64 == 0 && tile_rows <= 65536),
+         "tile_rows must be a multiple of 64 (got %d)", tile_rows);
+  DP_ARG(unroll_a == 0 || unroll_a == 4 || unroll_a == 8 || unroll_a == 16, "unroll_a must be 4, 8 or 16");
+  DP_ARG(unroll_g == 0 || unroll_g == 4 || unroll_g == 8 || unroll_g == 16, "unroll_g must be 4, 8 or 16");
+  DP_ARG(graph_k >= 0, "graph_k must be >= 0");
+  if (unroll_a) S->unroll_a = unroll_a;
+  if (unroll_g) S->unroll_g = unroll_g;
+  if (graph_k) S->graph_k = graph_k;
+  S->clear_graphs();
+  if (tile_rows && tile_rows != S->tile_rows) {
+    S->tile_rows = tile_rows;
+    DP_TRY(build_work(S));
+  }
+  return DOGPUP_OK;
+}
+
 int dogpup_solver_set_profile(dogpup_solver_t S, int on) {
   DP_ARG(S != nullptr, "NULL solver");
   S->profile = on != 0;
@@ -2032,8 +2277,7 @@ int dogpup_solver_stats(dogpup_solver_t S, double* h_out, int n) {
 int dogpup_solver_get_fsai(dogpup_solver_t S, double* d_out, void* stream) {
   DP_ARG(S && d_out, "NULL argument");
   DP_ARG(S->have_matrix, "no matrix set");
-  DP_HIP(hipMemcpyAsync(d_out, S->g64.p, (size_t)S->nnzG * S->nf * 16, hipMemcpyDeviceToDevice,
-                        as_stream(stream)));
+  DP_HIP(hipMemcpyAsync(d_out, S->g64.p, (size_t)S->gvals * 16, hipMemcpyDeviceToDevice, as_stream(stream)));
   return DOGPUP_OK;
 }
 

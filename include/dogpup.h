@@ -143,6 +143,29 @@ typedef struct dogpup_solver* dogpup_solver_t;
 int dogpup_solver_create(dogpup_solver_t* out, int nn, int nfreq, int ncols, int nsp, int mode,
                          int nnzA, const int* d_rptrA, const int* d_cptrA, int nnzG,
                          const int* d_rptrG, const int* d_cptrG, const int* h_order);
+/* Batch of independent meshes in one handle (one launch per pass over all
+ * of them).  The caller concatenates the meshes' patterns into one
+ * block-diagonal CSR: mesh i owns rows [row0[i], row0[i+1]) (row0[0] = 0,
+ * nn = row0[nmesh]) and its column indices stay inside that range, in A and in
+ * the FSAI pattern G alike; h_order (optional) must map each mesh's rows
+ * into the same mesh.  Mesh i has nfreq[i] frequencies and ncols[i] RHS
+ * columns.  dogpup_solver_set_matrix then takes the concatenated values and
+ * sum(nfreq) frequency scales, mesh after mesh; b and x hold mesh after mesh
+ * the single-mesh solver layout of each (nfreq[i] * ceil(ncols[i]/nsp)
+ * groups of row0[i+1]-row0[i] rows); the per-RHS outputs hold mesh after mesh
+ * freq*ncols[i] + col.  A single mesh through dogpup_solver_create is the
+ * case nmesh = 1.  No counterpart in the reference, which solves one mesh per
+ * call. */
+int dogpup_batch_create(dogpup_solver_t* out, int nmesh, const int* h_row0, const int* h_nfreq,
+                        const int* h_ncols, int nsp, int mode, int nnzA, const int* d_rptrA, const int* d_cptrA,
+                        int nnzG, const int* d_rptrG, const int* d_cptrG, const int* h_order);
+
+/* Kernel shapes of the following solves (0 keeps the current value): row
+ * tile of the SpMV and vector passes (a multiple of 64), x gathers in flight
+ * per row in the A and G SpMVs (4, 8 or 16), captured iterations replayed
+ * between convergence polls.  Chosen by dogpup_amd.autotune. */
+int dogpup_solver_set_tuning(dogpup_solver_t s, int tile_rows, int unroll_a, int unroll_g, int graph_k);
+
 int dogpup_solver_destroy(dogpup_solver_t s);
 
 /* Set A (FP64 values on the handle's pattern; frequency k uses
