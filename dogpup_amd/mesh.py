@@ -261,13 +261,13 @@ class FluenceSolver:
     """Device-resident batched FSAI-BiCGSTAB for one mesh, column count and
     mode (wraps dogpup_solver_*)."""
 
-    def __init__(self, mesh: DMesh, ncols: int, mode: int, order=None):
+    def __init__(self, mesh: DMesh, ncols: int, mode: int, order=None, nsp=None):
         if mesh.optode is None:
             raise ValueError("add an optode first (the frequency axis comes from it)")
         self.mesh = mesh
         self.nf = mesh.optode.Nf
         self.ncols = ncols
-        self.nsp = group_width(ncols)
+        self.nsp = int(nsp) if nsp else group_width(ncols)
         self.ncb = -(-ncols // self.nsp)
         self.mode = mode
         d = mesh._dev

@@ -1,0 +1,75 @@
+"""Batches of uneven meshes in one solver handle (dogpup_batch_create).
+
+A batch runs each group with the same arithmetic as a handle of its own mesh
+(same group width, tiles and reductions), so its fields must equal the
+single-mesh solves bit for bit; the FP64 batch must match the oracle's LU
+solve of every mesh.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel, small_slab
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _batch_meshes():
+    from dogpup_amd import meshgen, mesh as dm
+    t = np.arange(0, 501) * 20e-12
+    specs = [((20.0, 20.0, 10.0), 2.0, 3, 4), ((14.0, 10.0, 8.0), 1.0, 5, 6), ((30.0, 24.0, 12.0), 3.0, 12, 3)]
+    out = []
+    for k, (ext, h, ns, nf) in enumerate(specs):
+        m = small_slab(seed=k + 1, extent=ext, h=h)
+        nx = int(np.ceil(np.sqrt(ns)))
+        ny = -(-ns // nx)
+        src = meshgen.top_grid(m.extent, nx, ny, 0.25 * ext[0], 0.25 * ext[1], 0.5 * ext[0] / max(nx - 1, 1),
+                               0.5 * ext[1] / max(ny - 1, 1))[:ns]
+        det = meshgen.top_grid(m.extent, 2, 2, 0.3 * ext[0], 0.3 * ext[1], 0.4 * ext[0], 0.4 * ext[1])
+        D = dm.DMesh(m.node, m.elem, m.nr, np.stack([m.mua, m.musp], 1))
+        D.add_optode(dm.DOptode(src, det, meshgen.all_pairs(ns, 4), 1e3, meshgen.gaussian_tpsf(t, 250e-12), t, nf))
+        out.append((D, m))
+    return out
+
+
+@pytest.mark.parametrize("mode,method", [("mixed", "cocg"), ("mixed", "bicgstab"), ("fp64", "cocg")])
+def test_batch_equals_single_mesh_solves(mode, method):
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dogpup_amd import batch, mesh as dm
+    meshes = [D for D, _ in _batch_meshes()]
+    print(f"[case] batch {mode} {method}", flush=True)
+    B = batch.MeshBatch(meshes, "s", mode=mode, nsp=8, method=method)
+    tol = 1e-10 if mode == "mixed" else 1e-12
+    fields = B.solve(tol=tol)
+    for D, f in zip(meshes, fields):
+        assert (f.status == 0).all(), f.status
+        S = dm.FluenceSolver(D, f.ncols, dm.MODES[mode], nsp=8)
+        S.set_method(method)
+        nodes, w = D._bary_dev("s")
+        spec = torch.as_tensor(D.optode.s_fpsf, dtype=torch.complex128, device="cuda")
+        b = S.empty_block()
+        from dogpup_amd import _lib
+        _lib.call("dogpup_point_sources", _lib.ptr(torch.view_as_real(b)), D.nn, S.nf, f.ncols, S.nsp,
+                  _lib.ptr(nodes), _lib.ptr(w), _lib.ptr(torch.view_as_real(spec)), _lib.stream_ptr())
+        single = S.solve(b, tol=tol)
+        assert torch.equal(single.data, f.data)
+        assert (single.iters == f.iters).all()
+
+
+def test_batch_fp64_vs_direct():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dogpup_amd import batch
+    from oracle import fem, optode as oopt, solver as osol, transforms as otr
+    pairs = _batch_meshes()
+    B = batch.MeshBatch([D for D, _ in pairs], "s", mode="fp64", nsp=8, method="cocg")
+    fields = B.solve(tol=1e-12)
+    for (D, m), f in zip(pairs, fields):
+        om = fem.make_mesh(m.node, m.elem, m.nr, m.mua, m.musp)
+        _, df = otr.freq_axis(D.optode.tAxis, D.optode.Nf)
+        A = fem.full_system(om, df)
+        sid, sw = D.optode.s_bary
+        ref = osol.direct(A, oopt.source_matrix(om, sid, sw, D.optode.s_fpsf))
+        got = f.to_reference_layout().cpu().numpy()
+        assert rel(got, ref) < 1e-9
