@@ -207,6 +207,82 @@ def _cpu_solve_freq1(A, Q, F):
     return x, it, None
 
 
+def batch_leg(tol):
+    """The 48-mesh uneven batch: host prep, autotuned shapes, one timed
+    mixed solve (FSAI setup + solve) of all 38,431 RHS, the same with default
+    shapes, and the mixed-vs-FP64 error of every instance."""
+    import gc
+    import torch
+    from dogpup_amd import autotune, batch, meshgen, mesh as dm
+    t0 = time.time()
+    specs = meshgen.batch_48_uneven()
+    t = np.arange(0, 1001) * 20e-12
+    tpsf = meshgen.gaussian_tpsf(t, 250e-12)
+    meshes = []
+    for sp in specs:
+        m = sp["mesh"]
+        D = dm.DMesh(m.node, m.elem, m.nr, np.stack([m.mua, m.musp], 1))
+        D.add_optode(dm.DOptode(sp["src"], sp["det"], sp["link"], 1e3, tpsf, t, sp["nf"]))
+        meshes.append(D)
+    nrhs = sum(len(sp["src"]) * sp["nf"] for sp in specs)
+    out = {"instances": len(meshes), "nodes": int(sum(D.nn for D in meshes)), "rhs": nrhs,
+           "node_rhs": float(sum(D.nn * len(sp["src"]) * sp["nf"] for D, sp in zip(meshes, specs))),
+           "host_prep_s": time.time() - t0}
+    log(f"batch: {len(meshes)} meshes, {out['nodes']} nodes, {nrhs} RHS; autotuning")
+    t1 = time.time()
+    best, table = autotune.tune(meshes, "s", "mixed", "cocg", use_cache=False)
+    out["tune_s"] = time.time() - t1
+    out["tuned"] = best
+    out["tune_table_ms_per_iter"] = {f"nsp{r['nsp']}_tile{r['tile_rows']}_u{r['unroll']}": r["s_per_iter"] * 1e3
+                                     for r in table}
+
+    def timed(B):
+        b = B.rhs()
+        B._versions = None
+        B.solve(b, tol=tol)                      # warm-up (graph capture)
+        torch.cuda.synchronize()
+        B._versions = None                       # FSAI setup is part of the timed solve
+        t2 = time.perf_counter()
+        f = B.solve(b, tol=tol)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t2, f, b
+
+    default_nsp = min(dm.group_width(max(len(sp["src"]) for sp in specs)), 32)
+    log(f"batch: default shapes (nsp {default_nsp}, tile 512, unroll 8)")
+    B = batch.MeshBatch(meshes, "s", "mixed", nsp=default_nsp, method="cocg")
+    dt_def, _, b = timed(B)
+    del B, b, _
+    gc.collect()
+    torch.cuda.empty_cache()
+    log(f"batch: tuned shapes {best}")
+    B = batch.MeshBatch(meshes, "s", "mixed", nsp=best["nsp"], method="cocg",
+                        tuning=(best["tile_rows"], best["unroll"], best["unroll"], 0))
+    dt, fields, b = timed(B)
+    out["mixed_s"] = dt
+    out["rhs_per_s"] = nrhs / dt
+    out["default_shapes_s"] = dt_def
+    out["tuning_speedup"] = dt_def / dt
+    out["max_iters"] = int(max(f.iters.max() for f in fields))
+    out["status"] = sorted({int(v) for f in fields for v in np.unique(f.status)})
+    xm = [f.data.clone() for f in fields]
+    del B, fields
+    gc.collect()
+    torch.cuda.empty_cache()
+    log("batch: FP64 reference solve")
+    B64 = batch.MeshBatch(meshes, "s", "fp64", nsp=best["nsp"], method="cocg")
+    t3 = time.perf_counter()
+    f64 = B64.solve(b, tol=1e-12)
+    torch.cuda.synchronize()
+    out["fp64_s"] = time.perf_counter() - t3
+    errs = [float(torch.linalg.vector_norm(a - f.data) / torch.linalg.vector_norm(f.data)) for a, f in zip(xm, f64)]
+    out["mixed_vs_fp64_rel_err_per_instance"] = errs
+    out["mixed_vs_fp64_rel_err_max"] = max(errs)
+    del B64, f64, xm, b
+    gc.collect()
+    torch.cuda.empty_cache()
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +293,7 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-jacobian", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 48-mesh uneven batch")
     ap.add_argument("--method", default="cocg", choices=["bicgstab", "cocg"],
                     help="Krylov method of the headline mixed solve")
     ap.add_argument("--no-alt", action="store_true", help="skip the mixed solve with the other method")
@@ -345,6 +422,10 @@ def main(argv=None):
         extra["jacobian_shape"] = list(J.shape)
         del J
         release(D)
+    if not a.no_batch and not a.small:
+        release(D)
+        log("48-mesh uneven batch")
+        extra["batch48"] = batch_leg(a.tol)
     cpu = None
     if not a.no_cpu and not a.small:
         log("CPU baseline (oracle, 1 thread)")

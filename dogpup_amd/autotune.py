@@ -1,0 +1,111 @@
+"""Autotuner for the batched solver's kernel shapes.
+
+For a given set of meshes (one mesh or a MeshBatch), it times a few capped
+Krylov iterations of the real problem for each candidate:
+- group width NSP, the RHS block of the SpMV: 8, 16 or 32 columns per group.
+  Narrow groups waste no padding columns on meshes with few sources; wide
+  groups read each matrix entry once for more RHS.
+- row tile of the SpMV and vector passes: 256, 512 or 1024 rows.
+- x gathers in flight per row in the A and G SpMVs: 4 or 8.
+
+It keeps the fastest per RHS-iteration. Results are cached in a JSON file
+(``$DOGPUP_TUNE_CACHE``, default ``~/.cache/dogpup_amd/tune.json``) under a
+signature of the problem: node, frequency and column counts per mesh, mode and
+method.
+"""
+from __future__ import annotations
+
+import gc
+import json
+import os
+import time
+
+import torch
+
+from .batch import MeshBatch
+
+TILES = (256, 512, 1024)
+UNROLLS = (4, 8)
+WIDTHS = (8, 16, 32)
+
+
+def _cache_path():
+    return os.environ.get("DOGPUP_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "dogpup_amd", "tune.json"))
+
+
+def signature(meshes, which, mode, method):
+    cols = [sum(len((m.optode.s_bary if w == "s" else m.optode.d_bary)[0]) for w in which) for m in meshes]
+    return json.dumps({"nn": [m.nn for m in meshes], "nf": [m.optode.Nf for m in meshes], "ncols": cols,
+                       "mode": mode, "method": method})
+
+
+def _time_iterations(B, iters):
+    """Seconds per Krylov iteration of the whole batch (a capped solve with an
+    unreachable tolerance, after one warm-up that captures the graph)."""
+    b = B.rhs()
+    B.solve(b, tol=1e-30, maxit=iters)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    B.solve(b, tol=1e-30, maxit=iters)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    del b
+    return dt / iters
+
+
+def tune(meshes, which="s", mode="mixed", method="cocg", iters=6, widths=WIDTHS, tiles=TILES, unrolls=UNROLLS,
+         use_cache=True, verbose=False):
+    """Returns ({"nsp", "tile_rows", "unroll", "s_per_iter"}, table of every
+    candidate's seconds per iteration)."""
+    meshes = list(meshes)
+    key = signature(meshes, which, mode, method)
+    path = _cache_path()
+    cache = {}
+    if use_cache and os.path.exists(path):
+        try:
+            with open(path) as fh:
+                cache = json.load(fh)
+        except (OSError, ValueError):
+            cache = {}
+        if key in cache:
+            return cache[key]["best"], cache[key]["table"]
+    maxcols = max(sum(len((m.optode.s_bary if w == "s" else m.optode.d_bary)[0]) for w in which)
+                  for m in meshes)
+    table = []
+    best = None
+    for nsp in widths:
+        if nsp > 8 and nsp // 2 >= maxcols:   # wider than twice the widest mesh: pure padding
+            continue
+        B = MeshBatch(meshes, which, mode=mode, nsp=nsp, method=method)
+        B.ensure_matrix()
+        for tile in tiles:
+            for u in unrolls:
+                B.set_tuning(tile, u, u, 0)
+                s = _time_iterations(B, iters)
+                row = {"nsp": nsp, "tile_rows": tile, "unroll": u, "s_per_iter": s}
+                table.append(row)
+                if verbose:
+                    print(f"[tune] nsp {nsp:2d} tile {tile:4d} unroll {u:2d}: {s * 1e3:.2f} ms/iteration", flush=True)
+                if best is None or s < best["s_per_iter"]:
+                    best = dict(row)
+        del B
+        gc.collect()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    if use_cache:
+        cache[key] = {"best": best, "table": table}
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            with open(path, "w") as fh:
+                json.dump(cache, fh)
+        except OSError:
+            pass
+    return best, table
+
+
+def tuned_batch(meshes, which="s", mode="mixed", method="cocg", **kw):
+    """A MeshBatch with the tuned group width and kernel shapes."""
+    best, _ = tune(meshes, which, mode, method, **kw)
+    return MeshBatch(meshes, which, mode=mode, nsp=best["nsp"], method=method,
+                     tuning=(best["tile_rows"], best["unroll"], best["unroll"], 0))

@@ -1862,11 +1862,18 @@ struct SolvePlain {
   }
 };
 
+// Workspace budget: DOGPUP_WORKSPACE_GB, else the smaller of 160 GB and half
+// of the device memory free when the handle is made (the caller's b and x
+// and other handles share the rest); larger batches run in group chunks.
 static double ws_budget_bytes() {
-  const char* e = getenv("DOGPUP_WORKSPACE_GB");
-  double gb = e ? atof(e) : 160.0;
-  if (gb <= 0) gb = 160.0;
-  return gb * 1073741824.0;
+  if (const char* e = getenv("DOGPUP_WORKSPACE_GB")) {
+    const double gb = atof(e);
+    if (gb > 0) return gb * 1073741824.0;
+  }
+  size_t free_b = 0, total_b = 0;
+  double budget = 160.0 * 1073741824.0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0) budget = std::min(budget, 0.5 * (double)free_b);
+  return budget;
 }
 
 static int upload(Buf& b, const void* h, size_t bytes) {

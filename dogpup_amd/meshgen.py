@@ -152,3 +152,66 @@ def boxcar_gates(t, n_gates=8, width=500e-12, spacing=200e-12, start=0.0):
         t0 = start + i * spacing
         g[i] = ((t >= t0) & (t < t0 + width)).astype(float)
     return g
+
+
+def cylinder(radius, height, h=2.0, mua=0.01, musp=1.0, nr=1.4, seed=0):
+    """Cylinder of ``radius`` and ``height`` mm (axis z, centre (radius,
+    radius)): the Kuhn lattice of its bounding box keeps the tetrahedra whose
+    four nodes lie inside the radius (a staircase side wall, flat top and
+    bottom); unused nodes are dropped and the rest renumbered in lattice
+    order."""
+    n = int(round(2.0 * radius / h)) + 1
+    nz = int(round(height / h)) + 1
+    node, elem = kuhn_lattice(n, n, nz, h)
+    c = (n - 1) * h / 2.0
+    inside = (node[:, 0] - c) ** 2 + (node[:, 1] - c) ** 2 <= radius ** 2 * (1.0 + 1e-9)
+    elem = elem[inside[elem].all(axis=1)]
+    used = np.unique(elem)
+    remap = np.full(node.shape[0], -1, np.int64)
+    remap[used] = np.arange(used.size)
+    node = node[used]
+    elem = orient_positive(node, remap[elem]).astype(np.int32)
+    return SlabMesh(node=node, elem=elem, mua=np.full(node.shape[0], float(mua)),
+                    musp=np.full(node.shape[0], float(musp)), nr=float(nr), shape=(n, n, nz), h=float(h),
+                    extent=(2.0 * c, 2.0 * c, (nz - 1) * h), meta={"seed": seed, "kind": "cylinder",
+                                                                 "radius": radius})
+
+
+def uneven_batch(n=48, seed=0, nn_range=(5000, 200000)):
+    """Seeded batch of slab and cylinder meshes of uneven size: node counts
+    log-uniform in ``nn_range``, lattice spacing 1, 2 or 3 mm, mua U[0.005,
+    0.02]/mm and musp U[0.5, 1.5]/mm, 8-32 sources on the top face, 8-64
+    frequencies of 50 MHz spacing (T = 20 ns).  Returns a list of dicts with
+    ``mesh``, ``src``, ``det`` (4 detectors), ``link`` and ``nf``."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        target = float(np.exp(rng.uniform(np.log(nn_range[0]), np.log(nn_range[1]))))
+        h = float(rng.choice([1.0, 2.0, 3.0]))
+        mua = float(rng.uniform(0.005, 0.02))
+        musp = float(rng.uniform(0.5, 1.5))
+        if k % 2 == 0:
+            # slab with aspect 1 : 0.8 : 0.5
+            nx = max(int(round((target / 0.4) ** (1.0 / 3.0))), 6)
+            ny, nz = max(int(round(0.8 * nx)), 5), max(int(round(0.5 * nx)), 4)
+            m = slab(((nx - 1) * h, (ny - 1) * h, (nz - 1) * h), h, mua, musp, 1.4, seed=seed * 1000 + k)
+        else:
+            # cylinder of height = radius, about target nodes (pi/4 of its box)
+            nb = max(int(round((target / (np.pi / 4.0) * 2.0) ** (1.0 / 3.0))), 8)
+            m = cylinder((nb - 1) * h / 2.0, (nb // 2) * h, h, mua, musp, 1.4, seed=seed * 1000 + k)
+        ns = int(rng.integers(8, 33))
+        nf = int(rng.integers(8, 65))
+        gx = int(np.ceil(np.sqrt(ns)))
+        gy = -(-ns // gx)
+        ext = m.extent
+        src = top_grid(ext, gx, gy, 0.2 * ext[0], 0.2 * ext[1], 0.6 * ext[0] / max(gx - 1, 1),
+                       0.6 * ext[1] / max(gy - 1, 1))[:ns]
+        det = top_grid(ext, 2, 2, 0.3 * ext[0], 0.3 * ext[1], 0.4 * ext[0], 0.4 * ext[1])
+        out.append({"mesh": m, "src": src, "det": det, "link": all_pairs(ns, 4), "nf": nf})
+    return out
+
+
+def batch_48_uneven(seed=1):
+    """The 48-instance uneven batch: seed 1 draws 2.59M nodes in total (mean
+    54k), 38,431 RHS and 1.96e9 node x RHS."""
+    return uneven_batch(48, seed)

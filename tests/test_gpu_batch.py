@@ -73,3 +73,20 @@ def test_batch_fp64_vs_direct():
         ref = osol.direct(A, oopt.source_matrix(om, sid, sw, D.optode.s_fpsf))
         got = f.to_reference_layout().cpu().numpy()
         assert rel(got, ref) < 1e-9
+
+
+def test_autotune_picks_a_measured_shape(tmp_path, monkeypatch):
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dogpup_amd import autotune
+    monkeypatch.setenv("DOGPUP_TUNE_CACHE", str(tmp_path / "tune.json"))
+    meshes = [D for D, _ in _batch_meshes()]
+    best, table = autotune.tune(meshes, "s", "mixed", "cocg", iters=3, widths=(8, 16), tiles=(256, 512),
+                                unrolls=(4, 8))
+    assert len(table) == 8 and best in table
+    assert best["s_per_iter"] == min(r["s_per_iter"] for r in table)
+    again, _ = autotune.tune(meshes, "s", "mixed", "cocg")            # served from the cache
+    assert again == best
+    B = autotune.tuned_batch(meshes, "s", "mixed", "cocg")
+    for f in B.solve(tol=1e-10):
+        assert (f.status == 0).all()
