@@ -283,6 +283,62 @@ def batch_leg(tol):
     return out
 
 
+def lm_leg(max_iter=10):
+    """Config 5: weighted LM absorption reconstruction.  Data: FP64 solve on
+    a 0.75 mm mesh of the slab with two inclusions, 32 x 32 pairs x 8 gates,
+    Poisson noise at 1e6 peak counts; inversion on the 1 mm (269k-node) mesh
+    from the homogeneous background, in the mixed COCG form and in the
+    reference-structure FP64 form."""
+    import gc
+    import torch
+    from dogpup_amd import lm, meshgen, mesh as dm
+    t = np.arange(0, 1001) * 20e-12
+    tpsf = meshgen.gaussian_tpsf(t, 250e-12)
+    gates = meshgen.boxcar_gates(t, 8, 500e-12, 200e-12, start=500e-12)
+    out = {}
+    t0 = time.time()
+    mf, src, det, link, inc = meshgen.slab_80x80x40_two_inclusions(0.75, seed=0)
+    Df = dm.DMesh(mf.node, mf.elem, mf.nr, np.stack([mf.mua, mf.musp], 1))
+    Df.add_optode(dm.DOptode(src, det, link, 1e3, tpsf, t, 64))
+    log(f"LM data mesh {Df.nn} nodes (host prep {time.time() - t0:.1f} s); FP64 forward solve")
+    t1 = time.time()
+    data = lm.simulate_tg_data(Df, gates, mode="fp64", tol=1e-10, peak_counts=1e6, seed=0)
+    out["data_mesh_nodes"] = Df.nn
+    out["data_s"] = time.time() - t1
+    del Df
+    gc.collect()
+    torch.cuda.empty_cache()
+    m, src, det, link, inc = meshgen.slab_80x80x40_two_inclusions(1.0, seed=0)
+    truth = m.mua.copy()
+    xs = np.arange(8.0, 73.0, 8.0)
+    pos_s = np.stack(np.meshgrid(xs, xs, [30.0, 25.0, 20.0], indexing="ij"), -1).reshape(-1, 3)
+    for mode, method, tol in (("mixed", "cocg", 1e-8), ("plain", "bicgstab", 1e-12)):
+        D = dm.DMesh(m.node, m.elem, m.nr, np.stack([np.full(m.node.shape[0], 0.01), m.musp], 1))
+        D.add_optode(dm.DOptode(src, det, link, 1e3, tpsf, t, 64))
+        D.dxyz = 1.0
+        log(f"LM {mode} ({method}), {max_iter} iterations at most")
+        t2 = time.time()
+        # lam = (LM damping, spot reshaping); 1e-1 keeps reshape_J's inverse of the
+        # 8192-row J J^T stable under the Poisson noise (1e-2 diverged)
+        r = lm.wghtd_lm_recon(D, data, gates, pos_s, (1e2, 1e-1), mode=mode, tol=tol, max_iter=max_iter,
+                              method=method)
+        wall = time.time() - t2
+        inside = [np.linalg.norm(m.node - np.asarray(c)[None, :], axis=1) <= rad for c, rad, _ in inc]
+        out[mode] = {"s": wall, "iterations": r.count, "exit": r.exit, "errPerc": r.errPerc,
+                     "rmse": float(np.sqrt(np.mean((r.mua - truth) ** 2))),
+                     "contrast": [float(r.mua[k].mean() / 0.01) for k in inside],
+                     "true_contrast": [float(mu / 0.01) for _, _, mu in inc],
+                     "t_forward_s": r.t_forward, "t_algebra_s": r.t_algebra}
+        out[mode + "_mua"] = r.mua
+        del D
+        gc.collect()
+        torch.cuda.empty_cache()
+    a, b = out.pop("mixed_mua"), out.pop("plain_mua")
+    out["mixed_vs_plain_mua_rel_diff"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out["speedup_mixed_vs_plain"] = out["plain"]["s"] / out["mixed"]["s"]
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,6 +350,7 @@ def main(argv=None):
     ap.add_argument("--no-jacobian", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-mesh uneven batch")
+    ap.add_argument("--lm", action="store_true", help="also run the LM reconstruction (config 5; minutes)")
     ap.add_argument("--method", default="cocg", choices=["bicgstab", "cocg"],
                     help="Krylov method of the headline mixed solve")
     ap.add_argument("--no-alt", action="store_true", help="skip the mixed solve with the other method")
@@ -426,6 +483,10 @@ def main(argv=None):
         release(D)
         log("48-mesh uneven batch")
         extra["batch48"] = batch_leg(a.tol)
+    if a.lm and not a.small:
+        release(D)
+        log("LM reconstruction (config 5)")
+        extra["lm"] = lm_leg()
     cpu = None
     if not a.no_cpu and not a.small:
         log("CPU baseline (oracle, 1 thread)")

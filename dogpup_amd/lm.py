@@ -16,6 +16,7 @@ the reference.
 from __future__ import annotations
 
 import math
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -56,12 +57,16 @@ class LMResult:
     errDel: list = field(default_factory=list)
     lam: list = field(default_factory=list)
     exit: str = ""
+    # seconds per iteration: forward+adjoint solve, J-TG and data; the dense
+    # algebra (reshape_J, tik_inv)
+    t_forward: list = field(default_factory=list)
+    t_algebra: list = field(default_factory=list)
 
 
-def _forward(mesh, gate_fc, mode, tol):
+def _forward(mesh, gate_fc, mode, tol, method):
     """TG data (NM x NG, float64) and the Rytov-normalised TG Jacobian as
     rows x NV float32, rows ordered m + NM*g."""
-    f = mesh.solve_point_sources("sd", mode=mode, tol=tol)
+    f = mesh.solve_point_sources("sd", mode=mode, tol=tol, method=method)
     y = tg_data(mesh, f, gate_fc)                                  # NM x NG
     J, _ = jacobian_tg(mesh, gate_fc, field=f, scale=(1.0 / y).to(torch.float32))
     nv, ng, nm = J.shape
@@ -70,10 +75,11 @@ def _forward(mesh, gate_fc, mode, tol):
 
 
 def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed", tol=1e-8,
-                   max_iter=10, spot_fwhm=1.5, verbose=False):
+                   max_iter=10, spot_fwhm=1.5, verbose=False, method="bicgstab"):
     """wghtd_LM_recon(mesh, data_m, gates, pos_s, lam): returns LMResult with
     the reconstructed nodal mua.  data_m: NM x NG measured TG data; gates:
-    NG x NT gating functions on the optode's time axis."""
+    NG x NT gating functions on the optode's time axis.  mode/method: the
+    solver form of the forward+adjoint solves ('plain' is BiCGSTAB only)."""
     opt = mesh.optode
     data_m = np.asarray(data_m, float)
     incl = id_thresh(torch.as_tensor(data_m), 0.8).cpu().numpy().ravel(order="F")   # :25-26
@@ -89,10 +95,15 @@ def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed
     res = LMResult(mua=mesh.mua.copy(), count=0)
     y_prev = J_prev = d_mu = None
     while True:
-        yc, J = _forward(mesh, gate_fc, mode, tol)                                 # :58-61
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        yc, J = _forward(mesh, gate_fc, mode, tol, method)                         # :58-61
+        torch.cuda.synchronize()
+        res.t_forward.append(time.perf_counter() - t0)
         yc = yc.cpu().numpy()
         dc = np.log(yc.ravel(order="F"))[incl]                                     # :62-63
         J = J.index_select(0, rows)                                                # :64-65
+        t1 = time.perf_counter()
         y = ym - dc                                                                # :67
         if spots is not None:                                                      # :71-77
             w = recon.reshape_J(J, spots, lam2)                                    # NW x rows (FP64)
@@ -124,6 +135,7 @@ def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed
             lam1 = res.err[-1]
         res.lam.append(lam1)
         upd = recon.tik_inv(Jg, lam1, y).cpu().numpy()                             # :128
+        res.t_algebra.append(time.perf_counter() - t1)
         d_mu = G * upd                                                             # :130-134
         if np.any(np.isnan(d_mu)):                                                 # :137-141
             res.exit = "nan"
@@ -139,3 +151,22 @@ def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed
         y_prev, J_prev = y, J                                                      # :165-166
     res.mua = mesh.mua.copy()
     return res
+
+
+def simulate_tg_data(mesh, gates, mode="fp64", tol=1e-10, peak_counts=None, seed=0, method="cocg"):
+    """Time-gated boundary data of ``mesh`` (NM x NG): fc2tg(meas_flu(phi))
+    of a forward solve, with Poisson noise when ``peak_counts`` is given (the
+    data scaled so that their maximum is peak_counts photons, drawn, and
+    scaled back)."""
+    from .transforms import td2fc
+    opt = mesh.optode
+    f = mesh.solve_point_sources("s", mode=mode, tol=tol, method=method)
+    gate_fc = td2fc(torch.as_tensor(np.asarray(gates, float)), opt.fAxis, opt.tAxis, 1)
+    y = tg_data(mesh, f, gate_fc).cpu().numpy()
+    del f
+    if peak_counts is None:
+        return y
+    scale = peak_counts / y.max()
+    rng = np.random.default_rng(seed)
+    counts = rng.poisson(np.clip(y * scale, 0.0, None)).astype(float)
+    return counts / scale

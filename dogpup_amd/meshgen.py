@@ -215,3 +215,18 @@ def batch_48_uneven(seed=1):
     """The 48-instance uneven batch: seed 1 draws 2.59M nodes in total (mean
     54k), 38,431 RHS and 1.96e9 node x RHS."""
     return uneven_batch(48, seed)
+
+
+def slab_80x80x40_two_inclusions(h=1.0, seed=0):
+    """80x80x40 mm slab on an h mm Kuhn grid, background mua 0.01/mm and
+    musp 1.0/mm, with two absorbing spheres (radius 5 mm, mua 0.02/mm and
+    radius 7 mm, mua 0.025/mm) at seeded depths of 10-20 mm, 24 mm apart; 32
+    sources and 32 detectors as in slab_80x80x40_h1.  Returns (mesh, src, det,
+    link, inclusions)."""
+    rng = np.random.default_rng(seed)
+    d1, d2 = rng.uniform(10.0, 20.0, size=2)
+    inc = [((28.0, 40.0, 40.0 - d1), 5.0, 0.02), ((52.0, 40.0, 40.0 - d2), 7.0, 0.025)]
+    m = slab((80.0, 80.0, 40.0), h, 0.01, 1.0, 1.4, jitter=0.0, seed=seed, inclusions=inc)
+    src = top_grid(m.extent, 4, 8, 16.0, 5.0, 16.0, 10.0)
+    det = top_grid(m.extent, 4, 8, 24.0, 10.0, 16.0, 10.0)
+    return m, src, det, all_pairs(32, 32), inc
