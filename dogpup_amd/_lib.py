@@ -62,6 +62,12 @@ SIGNATURES = {
     "dogpup_solver_set_profile": [vp, i32],
     "dogpup_solver_set_method": [vp, i32],
     "dogpup_solver_profile": [vp, vp, i32],
+    "dogpup_locator_create": [P(vp), vp, i32, vp, i32, vp],
+    "dogpup_locator_destroy": [vp],
+    "dogpup_locator_stats": [vp, vp],
+    "dogpup_locator_find": [vp, vp, C.c_longlong, f64, vp, vp, vp, vp],
+    "dogpup_grid_to_mesh": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp],
+    "dogpup_ell4_apply": [vp, vp, vp, vp, C.c_longlong, C.c_longlong, i32, i32, vp],
 }
 
 

@@ -9,7 +9,11 @@ with the gate spectra in one kernel:
     C_gk = -2 Nt w_k conj(Gate_gk) dxyz^3 / s_fpsf(k),  w_0 = 1/2,
 
 which equals ``fc2tg(J_complex(...), gates, f, Nt, 2)`` (dMesh.m:206-247,
-fc2tg.m:25-32) on the nodal basis (grid = lattice nodes, m2g = I).
+fc2tg.m:25-32).  On the nodal basis (no voxel grid attached: grid = mesh
+nodes, m2g = I) v runs over the nodes; with a grid from ``mesh.mesh2grid`` the
+fields are first interpolated to the in-mesh voxels (m2g, dMesh.m:239-240)
+and v runs over those (``mesh.basis.in_idx``; the reference's columns outside
+the mesh are zero).
 ``AbsorptionTG`` exposes the map mua -> time-gated data with this Jacobian as
 its derivative, as a torch.autograd.Function.
 """
@@ -51,13 +55,15 @@ def tg_coefficients(optode, gate_fc, dxyz):
 def jacobian_tg(mesh, gate_fc, mode="mixed", tol=1e-10, dxyz=None, scale=None, field=None,
                 method="bicgstab"):
     """J_TG as float32 [NV][NG][NM] (= the reference's NM x NG x NV,
-    column-major).  ``field``: a forward+adjoint FluenceField from
+    column-major); NV = nodes, or in-mesh voxels when ``mesh.basis`` is set.
+    ``field``: a forward+adjoint FluenceField from
     ``mesh.solve_point_sources('sd')`` to reuse; ``scale``: optional NM x NG
     factor (e.g. 1/data for the Rytov normalisation)."""
     opt = mesh.optode
     ns, nd = len(opt.s_bary[0]), len(opt.d_bary[0])
+    basis = getattr(mesh, "basis", None)
     if dxyz is None:
-        dxyz = getattr(mesh, "dxyz", None)
+        dxyz = basis.dxyz if basis is not None else getattr(mesh, "dxyz", None)
         if dxyz is None:
             raise ValueError("voxel side dxyz is needed (set mesh.dxyz or pass dxyz)")
     t0 = time.perf_counter()
@@ -65,16 +71,17 @@ def jacobian_tg(mesh, gate_fc, mode="mixed", tol=1e-10, dxyz=None, scale=None, f
         field = mesh.solve_point_sources("sd", mode=mode, tol=tol, method=method)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    vfield = basis.field_to_voxels(field) if basis is not None else field
     coef = tg_coefficients(opt, gate_fc, dxyz)
-    ng, nm, nn = coef.shape[0], opt.link.shape[0], mesh.nn
+    ng, nm, nn = coef.shape[0], opt.link.shape[0], vfield.nn
     link = torch.as_tensor(np.stack([opt.link[:, 0], opt.link[:, 1] + ns], 1).astype(np.int32), device=DEV)
     out = torch.empty((nn, ng, nm), dtype=torch.float32, device=DEV)
     sc = None
     if scale is not None:
         sc = torch.as_tensor(scale, dtype=torch.float32, device=DEV).t().contiguous()   # NG x NM
-    x = torch.view_as_real(field.data)
+    x = torch.view_as_real(vfield.data)
     _lib.call("dogpup_jacobian_tg", _lib.ptr(x), _lib.ptr(torch.view_as_real(coef)), _lib.ptr(link),
-              _lib.ptr(sc), _lib.ptr(out), nn, opt.Nf, ng, field.ncols, field.nsp, nm, nn,
+              _lib.ptr(sc), _lib.ptr(out), nn, opt.Nf, ng, vfield.ncols, vfield.nsp, nm, nn,
               _lib.stream_ptr())
     torch.cuda.synchronize()
     t2 = time.perf_counter()

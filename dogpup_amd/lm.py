@@ -1,9 +1,12 @@
 """Weighted Levenberg-Marquardt absorption reconstruction on the GPU path.
 
 Host logic mirrors ``DOGPUP/reconstruction/wghtd_LM_recon.m:1-169`` (with
-``id_thresh.m`` and ``dMesh.m:250-282`` target_spots) on the nodal basis
-(grid = mesh nodes: m2g = g2m = I, gridinmesh all true).  Per iteration the
-heavy work runs in libdogpup_hip.so:
+``id_thresh.m`` and ``dMesh.m:250-282`` target_spots).  The unknowns live on
+the mesh's voxel basis when ``mesh.mesh2grid`` has attached one (the update
+is solved for the in-mesh voxels, scaled by m2g*mua and mapped back with g2m,
+:101-134), otherwise on the nodal basis (grid = mesh nodes: m2g = g2m = I,
+gridinmesh all true).  Per iteration the heavy work runs in
+libdogpup_hip.so:
   - one batched forward+adjoint solve (sources and detectors as columns);
   - the time-gated data fc2tg(meas_flu(phi));
   - the fused J-TG kernel with the Rytov scaling 1/data;
@@ -29,10 +32,12 @@ from .transforms import id_thresh, td2fc
 DEV = "cuda"
 
 
-def target_spots(node, pos, w, kind="gauss"):
-    """Sensitivity target spots (dMesh.m:250-282) on the nodal grid:
-    NW x NV, each row normalised to sum 1 and negated.  As in the reference,
-    the Gaussian width is s = 2*sqrt(2 ln 2) * w."""
+def target_spots(node, pos, w, kind="gauss", inmesh=None):
+    """Sensitivity target spots (dMesh.m:250-282) on the grid points ``node``
+    (mesh nodes on the nodal basis): NW x NV, each row normalised to sum 1
+    over all grid points and negated, then zero at points outside the mesh
+    (``inmesh``).  As in the reference, the Gaussian width is
+    s = 2*sqrt(2 ln 2) * w."""
     node = np.asarray(node, float)
     pos = np.atleast_2d(np.asarray(pos, float))
     if kind == "gauss":
@@ -44,8 +49,10 @@ def target_spots(node, pos, w, kind="gauss"):
         t = np.all(np.abs(node[None, :, :] - pos[:, None, :]) <= h, axis=2).astype(float)
     else:
         raise ValueError("Check spot type is gauss or square")
-    t = t / t.sum(axis=1, keepdims=True)
-    return -t
+    t = -(t / t.sum(axis=1, keepdims=True))
+    if inmesh is not None:
+        t[:, ~np.asarray(inmesh, bool)] = 0.0
+    return t
 
 
 @dataclass
@@ -85,10 +92,28 @@ def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed
     incl = id_thresh(torch.as_tensor(data_m), 0.8).cpu().numpy().ravel(order="F")   # :25-26
     ym = np.log(data_m.ravel(order="F"))[incl]                                     # :28-30
     gate_fc = td2fc(torch.as_tensor(np.asarray(gates, float)), opt.fAxis, opt.tAxis, 1)  # :32
+    basis = getattr(mesh, "basis", None)
     spots = None
     if pos_s is not None and len(pos_s):
-        spots = torch.as_tensor(target_spots(mesh.node, pos_s, spot_fwhm, "gauss"),
-                                dtype=torch.float32, device=DEV)                   # :35-37
+        if basis is not None:
+            t = target_spots(basis.grid, pos_s, spot_fwhm, "gauss", basis.gridinmesh)[:, basis.in_idx]
+        else:
+            t = target_spots(mesh.node, pos_s, spot_fwhm, "gauss")
+        spots = torch.as_tensor(t, dtype=torch.float32, device=DEV)              # :35-37
+
+    def to_grid(v):
+        """m2g * v restricted to the unknowns (in-mesh voxels or nodes)."""
+        if basis is None:
+            return np.asarray(v, float).copy()
+        return basis.m2g_in.apply(v).cpu().numpy()
+
+    def to_mesh(xu):
+        """g2m * x with x zero outside the mesh (nodal basis: x itself)."""
+        if basis is None:
+            return xu
+        xg = np.zeros(basis.nv)
+        xg[basis.in_idx] = xu
+        return basis.to_mesh(xg).cpu().numpy()
     rows = torch.as_tensor(np.flatnonzero(incl), device=DEV)
     k = 10 ** 0.25
     lam1, lam2 = lam[0] * k, lam[1]
@@ -124,10 +149,10 @@ def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed
             break
         res.count += 1
         if res.count == 1 or np.sign(res.errDel[-1]) == -1:                       # :101-106
-            G = mesh.mua.copy()
+            G = to_grid(mesh.mua)
             lam1 = lam1 / k
         else:                                                                       # :107-116
-            G = mesh.mua - d_mu
+            G = to_grid(mesh.mua - d_mu)
             y, J = y_prev, J_prev
             lam1 = lam1 * k
         Jg = J * torch.as_tensor(G, dtype=torch.float32, device=DEV)[None, :]     # J .* G'
@@ -136,7 +161,7 @@ def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed
         res.lam.append(lam1)
         upd = recon.tik_inv(Jg, lam1, y).cpu().numpy()                             # :128
         res.t_algebra.append(time.perf_counter() - t1)
-        d_mu = G * upd                                                             # :130-134
+        d_mu = to_mesh(G * upd)                                                    # :130-134
         if np.any(np.isnan(d_mu)):                                                 # :137-141
             res.exit = "nan"
             break

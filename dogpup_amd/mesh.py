@@ -17,7 +17,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from . import _lib, topology
+from . import _lib, grid, topology
 from .transforms import td2fc
 
 DEV = "cuda"
@@ -75,10 +75,12 @@ class DOptode:
         """dOptode.m:77-97: detectors snapped to the surface, sources snapped
         then moved 1/min(musp) along the inward normal."""
         d_pos, _ = topology.snap_to_surface(mesh.node, mesh.face, mesh.bnd, self.d_positions)
-        d_id, d_w = topology.locate_points(mesh.node, mesh.elem, d_pos)
         s_pos, s_dir = topology.snap_to_surface(mesh.node, mesh.face, mesh.bnd, self.s_positions)
         s_in = s_pos + (1.0 / mesh.musp.min()) * s_dir
-        s_id, s_w = topology.locate_points(mesh.node, mesh.elem, s_in)
+        eid, w, _ = mesh.locator().find(np.concatenate([d_pos, s_in]))
+        eid, w = eid.cpu().numpy().astype(np.int64), w.cpu().numpy()
+        nd = d_pos.shape[0]
+        d_id, d_w, s_id, s_w = eid[:nd], w[:nd], eid[nd:], w[nd:]
         if (d_id < 0).any() or (s_id < 0).any():
             raise ValueError("an optode could not be located inside the mesh")
         self.d_positions, self.d_dirs = d_pos, None
@@ -128,6 +130,8 @@ class DMesh:
         self.kappa = 1.0 / (3.0 * (self.mua + self.musp))          # :74
         self.reference_faces = reference_faces
         self.optode = None
+        self.basis = None            # voxel basis from mesh2grid (None: nodal basis)
+        self._locator = None
         self._solvers = {}
         self.prepare_mesh()
         self.update_properties()
@@ -191,6 +195,38 @@ class DMesh:
         A_k = K + i*k*(df/c)*M."""
         d = self._dev
         return d["rA"], d["cA"], self.K, self.M
+
+    # -- point location, voxel basis, interpolation (dMesh.m:286-411) ---------
+    def locator(self) -> grid.Locator:
+        if self._locator is None:
+            self._locator = grid.Locator(self._dev["node"], self._dev["elem"])
+        return self._locator
+
+    def mesh2grid(self, x, y, z, tol=1e-9):
+        """Attach the voxel basis of grid axes x, y, z (dMesh.m:286-380):
+        grid, gridSize, dxyz, m2g, g2m, gridinmesh (and ``basis``, which the
+        Jacobian and the LM driver use)."""
+        b = grid.mesh2grid(self._dev["node"], self._dev["elem"], x, y, z, tol, locator=self.locator())
+        self.basis = b
+        self.grid, self.gridSize, self.dxyz = b.grid, b.gridSize, b.dxyz
+        self.m2g, self.g2m, self.gridinmesh = b.m2g, b.g2m, b.gridinmesh
+        return self
+
+    def clear_grid(self):
+        """Back to the nodal basis (m2g = g2m = I)."""
+        self.basis = None
+        return self
+
+    def mesh2mesh(self, val, old_node, old_elem):
+        """``val`` on the mesh (old_node, old_elem) interpolated to this
+        mesh's nodes (dMesh.m:383-411)."""
+        return grid.mesh2mesh(self.node, val, old_node, old_elem)
+
+    def mesh_slice(self, plane):
+        return grid.mesh_slice(self._dev["node"], self._dev["elem"], plane, locator=self.locator())
+
+    def mesh_slice3(self):
+        return grid.mesh_slice3(self._dev["node"], self._dev["elem"], locator=self.locator())
 
     # -- optodes -------------------------------------------------------------
     def add_optode(self, optode: DOptode):

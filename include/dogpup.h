@@ -293,6 +293,49 @@ int dogpup_jacobian_vjp(const double* d_field, const double* d_D, const int* d_l
                         int nsp, int ns, int nm, int exact, double dxyz3, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Point location and grid <-> mesh interpolation.  These replace the host
+ * triangulation/pointLocation/delaunay calls of mesh2grid (dMesh.m:286-380),
+ * mesh2mesh (:383-411), mesh_slice/mesh_slice3 (:414-517) and the optode
+ * location of snap2mesh (dOptode.m:77-97).  Device pointers; node NN x 3
+ * float64 and elem NE x 4 int32 (0-based), row-major.
+ * ------------------------------------------------------------------------- */
+
+typedef struct dogpup_locator* dogpup_locator_t;
+
+/* Bins the tetrahedra of a mesh (a copy of node/elem is kept). */
+int dogpup_locator_create(dogpup_locator_t* out, const double* d_node, int nn, const int* d_elem, int ne,
+                          void* stream);
+int dogpup_locator_destroy(dogpup_locator_t h);
+/* h_out[0..2] bins per axis, h_out[3] element-bin pairs. */
+int dogpup_locator_stats(dogpup_locator_t h, long long* h_out);
+
+/* Containing tetrahedron of each point (pts: npts x 3): eid (-1 outside),
+ * barycentric weights w (npts x 4, NaN outside) and, if d_nodes is not NULL,
+ * its 4 node ids (npts x 4, -1 outside).  A point is inside when the largest,
+ * over tetrahedra, of the smallest weight is >= -tol; ties go to the smaller
+ * element id. */
+int dogpup_locator_find(dogpup_locator_t h, const double* d_pts, long long npts, double tol, int* d_eid,
+                        double* d_w, int* d_nodes, void* stream);
+
+/* Grid -> mesh interpolation weights of mesh2grid (dMesh.m:331-373): each
+ * node is located in the Kuhn split of the voxel grid (origin, spacing, dims
+ * per axis; voxel v = ix + nx*(iy + ny*iz)); vertices outside the mesh
+ * (inmesh[v] == 0) are dropped and the rest renormalised to sum 1 (a zero
+ * sum is left unscaled, as the reference does); a node with no in-mesh
+ * vertex takes its nearest in-mesh voxel with weight 1.  Output idx/w:
+ * NN x 4 (unused slots -1 and 0).  h_status_counts (optional, 2 ints):
+ * nodes that took the nearest voxel, nodes outside the grid (an error). */
+int dogpup_grid_to_mesh(const double* d_node, int nn, const double* h_origin, const double* h_spacing,
+                        const int* h_dims, const unsigned char* d_inmesh, int* d_idx, double* d_w,
+                        int* h_status_counts, void* stream);
+
+/* Sparse interpolation with at most 4 entries per row (m2g, g2m, mesh2mesh):
+ * out[g][r][c] = sum_q w[r][q] in[g][idx[r][q]][c] over idx >= 0;
+ * rows of `width` doubles (2*nsp for a complex128 solver-layout field). */
+int dogpup_ell4_apply(const double* d_in, double* d_out, const int* d_idx, const double* d_w,
+                      long long nrows_out, long long nrows_in, int width, int ngroups, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Dense algebra of the weighted LM step (reconstruction/tik_inv.m:11-19,
  * reshape_J.m:13-15): normal matrices on FP32 MFMA with FP32 accumulation,
  * FP64 Cholesky of the damped system.  Row-major, device pointers.
