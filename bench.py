@@ -478,6 +478,26 @@ def main(argv=None):
         extra["jacobian_tg_kernel_s"] = info["kernel_s"]
         extra["jacobian_shape"] = list(J.shape)
         del J
+        # voxel basis: mesh2grid on 1 mm and 2 mm grids over the slab (point
+        # location of every voxel, Kuhn-split weights of every node), then
+        # J-TG over the 2 mm grid's voxels from the same fields (m2g applied to
+        # the whole forward+adjoint field first)
+        log("voxel basis: mesh2grid 1 mm / 2 mm, J-TG on 2 mm voxels")
+        ext = np.asarray(D.node.max(axis=0))
+        for hv in (1.0, 2.0):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            D.mesh2grid(*(np.arange(0.0, e + 0.5 * hv, hv) for e in ext))
+            torch.cuda.synchronize()
+            extra[f"mesh2grid_{int(hv)}mm_s"] = time.perf_counter() - t0
+        extra["voxels_2mm"] = [int(D.basis.nv), int(D.basis.nv_in)]
+        t0 = time.perf_counter()
+        Jv, iv = jacobian_tg(D, gfc, field=info["field"])
+        torch.cuda.synchronize()
+        extra["jacobian_tg_voxel2mm_s"] = time.perf_counter() - t0
+        extra["jacobian_voxel2mm_shape"] = list(Jv.shape)
+        D.clear_grid()
+        del Jv, iv, info
         release(D)
     if not a.no_batch and not a.small:
         release(D)
