@@ -22,6 +22,7 @@ import time
 
 import torch
 
+from . import _lib
 from .batch import MeshBatch
 
 TILES = (256, 512, 1024)
@@ -77,22 +78,33 @@ def tune(meshes, which="s", mode="mixed", method="cocg", iters=6, widths=WIDTHS,
     for nsp in widths:
         if nsp > 8 and nsp // 2 >= maxcols:   # wider than twice the widest mesh: pure padding
             continue
-        B = MeshBatch(meshes, which, mode=mode, nsp=nsp, method=method)
-        B.ensure_matrix()
-        for tile in tiles:
-            for u in unrolls:
-                B.set_tuning(tile, u, u, 0)
-                s = _time_iterations(B, iters)
-                row = {"nsp": nsp, "tile_rows": tile, "unroll": u, "s_per_iter": s}
-                table.append(row)
-                if verbose:
-                    print(f"[tune] nsp {nsp:2d} tile {tile:4d} unroll {u:2d}: {s * 1e3:.2f} ms/iteration", flush=True)
-                if best is None or s < best["s_per_iter"]:
-                    best = dict(row)
+        B = None
+        try:
+            B = MeshBatch(meshes, which, mode=mode, nsp=nsp, method=method)
+            B.ensure_matrix()
+            for tile in tiles:
+                for u in unrolls:
+                    B.set_tuning(tile, u, u, 0)
+                    s = _time_iterations(B, iters)
+                    row = {"nsp": nsp, "tile_rows": tile, "unroll": u, "s_per_iter": s}
+                    table.append(row)
+                    if verbose:
+                        print(f"[tune] nsp {nsp:2d} tile {tile:4d} unroll {u:2d}: {s * 1e3:.2f} ms/iteration",
+                              flush=True)
+                    if best is None or s < best["s_per_iter"]:
+                        best = dict(row)
+        except (torch.OutOfMemoryError, _lib.DogpupError) as e:
+            # a width whose padded vectors do not fit is not a candidate
+            if isinstance(e, _lib.DogpupError) and e.code != _lib.ERR_NOMEM:
+                raise
+            if verbose:
+                print(f"[tune] nsp {nsp:2d}: does not fit in device memory, skipped", flush=True)
         del B
         gc.collect()
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
+    if best is None:
+        raise RuntimeError("no candidate shape fits in device memory")
     if use_cache:
         cache[key] = {"best": best, "table": table}
         try:

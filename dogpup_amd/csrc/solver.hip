@@ -1104,6 +1104,7 @@ struct dogpup_solver {
   dogpup::Buf ws, part1, part2, state, rstate, bperm, xperm;
   int tile_rows = 512, lds_pad = 0;
   int unroll_a = 8, unroll_g = 8;  // gathers in flight per row (k_spmv: 4, 8 or 16)
+  double ws_budget = 0;             // workspace budget, fixed when the handle is made
   bool have_matrix = false;
   double stats[4] = {0, 0, 0, 0};
   int* h_counter = nullptr;  // pinned: [0] active RHS, [1] active groups, [2] their rows
@@ -1863,16 +1864,20 @@ struct SolvePlain {
 };
 
 // Workspace budget: DOGPUP_WORKSPACE_GB, else the smaller of 160 GB and half
-// of the device memory free when the handle is made (the caller's b and x
-// and other handles share the rest); larger batches run in group chunks.
-static double ws_budget_bytes() {
+// of the device memory free when the handle is made once the caller's
+// complex128 b and x of this handle (io_bytes each) are set aside, whether or
+// not they exist yet; larger batches run in group chunks.  It is computed
+// once per handle, so re-tuning does not re-plan the chunks around memory the
+// caller's allocator happens to hold at the time.
+static double ws_budget_bytes(double io_bytes) {
   if (const char* e = getenv("DOGPUP_WORKSPACE_GB")) {
     const double gb = atof(e);
     if (gb > 0) return gb * 1073741824.0;
   }
   size_t free_b = 0, total_b = 0;
   double budget = 160.0 * 1073741824.0;
-  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0) budget = std::min(budget, 0.5 * (double)free_b);
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
+    budget = std::min(budget, 0.5 * std::max(0.0, (double)free_b - 2.0 * io_bytes));
   return budget;
 }
 
@@ -1916,7 +1921,12 @@ static int build_work(dogpup_solver* S) {
   // chunks: consecutive groups whose vectors fit the workspace budget
   const size_t csize = S->mode == DOGPUP_MODE_C64_REFINE ? 8 : 16;
   const double per_node = (double)S->nsp * (csize * nvec_for(S->mode) + 2.0 * 16);
-  const double budget = ws_budget_bytes();
+  if (S->ws_budget <= 0) {
+    double io_rows = 0;
+    for (int g = 0; g < ng; ++g) io_rows += S->h_grows[g];
+    S->ws_budget = ws_budget_bytes(io_rows * S->nsp * 16.0);
+  }
+  const double budget = S->ws_budget;
   S->chunk_g.assign(1, 0);
   double acc = 0;
   for (int g = 0; g < ng; ++g) {
