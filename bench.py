@@ -248,7 +248,7 @@ def batch_leg(tol):
         return time.perf_counter() - t2, f, b
 
     default_nsp = min(dm.group_width(max(len(sp["src"]) for sp in specs)), 32)
-    log(f"batch: default shapes (nsp {default_nsp}, tile 512, unroll 8)")
+    log(f"batch: library default shapes (nsp {default_nsp})")
     B = batch.MeshBatch(meshes, "s", "mixed", nsp=default_nsp, method="cocg")
     dt_def, _, b = timed(B)
     del B, b, _

@@ -46,7 +46,10 @@ enum { MAT_A = 0, MAT_G = 1 };
 // and y.y (y not stored); EPI_STORE_SELF y and y.y; EPI_AXPBY y <- y_new +
 // coef*y_old (in place, one complex coefficient per column); EPI_AXPBY_X
 // also xacc += coef2*y_old.  All dots are unconjugated.
-enum { EPI_STORE = 0, EPI_STORE_DOT = 1, EPI_DOT2 = 2, EPI_STORE_SELF = 3, EPI_AXPBY = 4, EPI_AXPBY_X = 5 };
+// EPI_AXPY_SELF y <- y_old - coef*(M x) (in place) with y.y and |y|^2;
+// EPI_STORE_SELF2 y with y.y and |y|^2.
+enum { EPI_STORE = 0, EPI_STORE_DOT = 1, EPI_DOT2 = 2, EPI_STORE_SELF = 3, EPI_AXPBY = 4, EPI_AXPBY_X = 5,
+       EPI_AXPY_SELF = 6, EPI_STORE_SELF2 = 7 };
 enum { ST_ACTIVE = -1, ST_SKIP = -2 };
 
 // Work list.  Workgroups are dispatched in order, round-robin over the 8
@@ -206,6 +209,156 @@ __device__ inline void spmv_rows(const int* __restrict__ rptr, const int* __rest
   }
 }
 
+// Epilogue of the solver SpMV for one row: stores and fused dot products
+// (see the EPI_* list above), shared by k_spmv and k_spmv_halo.
+template <typename T, int NSP, int EPI>
+struct SpmvEpi {
+  static constexpr int CPL = Geo<T, NSP>::CPL;
+  static constexpr bool AXPBY = EPI == EPI_AXPBY || EPI == EPI_AXPBY_X;
+  T* y;
+  const T* w;
+  T* xacc;
+  long long vb;
+  int c0;
+  T cr[CPL], ci[CPL], c2r[CPL], c2i[CPL];
+  Acc<CPL> d1, d2;
+
+  __device__ SpmvEpi(T* y_, const T* w_, T* xacc_, const double* coef, const double* coef2, long long vb_,
+                     int c0_, long long rhs0)
+      : y(y_), w(w_), xacc(xacc_), vb(vb_), c0(c0_) {
+    d1.zero();
+    d2.zero();
+    if (AXPBY || EPI == EPI_AXPY_SELF) load_coef<T, CPL>(coef, rhs0, cr, ci);
+    if (EPI == EPI_AXPBY_X) load_coef<T, CPL>(coef2, rhs0, c2r, c2i);
+  }
+  __device__ inline void operator()(int row, T* ar, T* ai) {
+    const long long off = ((vb + row) * NSP + c0) * 2;
+    if (AXPBY) {
+      T pr[CPL], pi[CPL];
+      Pack<T>::load(y + off, pr, pi);
+      if (EPI == EPI_AXPBY_X) {
+        T xr[CPL], xi[CPL];
+        Pack<T>::load(xacc + off, xr, xi);
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) {
+          xr[e] += c2r[e] * pr[e] - c2i[e] * pi[e];
+          xi[e] += c2r[e] * pi[e] + c2i[e] * pr[e];
+        }
+        Pack<T>::store(xacc + off, xr, xi);
+      }
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) {
+        ar[e] += cr[e] * pr[e] - ci[e] * pi[e];
+        ai[e] += cr[e] * pi[e] + ci[e] * pr[e];
+      }
+    }
+    if (EPI == EPI_AXPY_SELF) {
+      T pr[CPL], pi[CPL];
+      Pack<T>::load(y + off, pr, pi);
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) {
+        const T tr = ar[e], ti = ai[e];
+        ar[e] = pr[e] - (cr[e] * tr - ci[e] * ti);
+        ai[e] = pi[e] - (cr[e] * ti + ci[e] * tr);
+      }
+    }
+    if (EPI != EPI_DOT2) Pack<T>::store(y + off, ar, ai);
+    if (EPI == EPI_STORE_SELF || EPI == EPI_AXPY_SELF || EPI == EPI_STORE_SELF2) {
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) {
+        d1.re[e] += (double)ar[e] * ar[e] - (double)ai[e] * ai[e];
+        d1.im[e] += 2.0 * (double)ar[e] * ai[e];
+        if (EPI != EPI_STORE_SELF) d2.re[e] += (double)ar[e] * ar[e] + (double)ai[e] * ai[e];
+      }
+    }
+    if (EPI == EPI_STORE_DOT || EPI == EPI_DOT2) {
+      T wr[CPL], wi[CPL];
+      Pack<T>::load(w + off, wr, wi);
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) {
+        d1.re[e] += (double)wr[e] * ar[e] - (double)wi[e] * ai[e];
+        d1.im[e] += (double)wr[e] * ai[e] + (double)wi[e] * ar[e];
+        if (EPI == EPI_DOT2) {
+          d2.re[e] += (double)ar[e] * ar[e] - (double)ai[e] * ai[e];
+          d2.im[e] += 2.0 * (double)ar[e] * ai[e];
+        }
+      }
+    }
+  }
+  // operands the epilogue reads (y_old or w, and xacc) for row `row`, as
+  // 16-byte packs: the halo kernel loads them ahead of its compute phase
+  static constexpr int NOPS = (EPI == EPI_AXPBY_X) ? 2
+                              : (AXPBY || EPI == EPI_AXPY_SELF || EPI == EPI_STORE_DOT || EPI == EPI_DOT2) ? 1
+                                                                                                      : 0;
+  template <typename V>
+  __device__ inline void load_ops(int row, V& o1, V& o2) const {
+    const long long off = ((vb + row) * NSP + c0) * 2;
+    if (NOPS >= 1) o1 = *reinterpret_cast<const V*>((EPI == EPI_STORE_DOT || EPI == EPI_DOT2 ? w : y) + off);
+    if (NOPS >= 2) o2 = *reinterpret_cast<const V*>(xacc + off);
+  }
+  // operator() with the operands already loaded (CPL = 2, complex64)
+  template <typename V>
+  __device__ inline void apply(int row, T* ar, T* ai, const V& o1, const V& o2) {
+    static_assert(CPL == 2 && sizeof(T) == 4, "complex64 packs");
+    const long long off = ((vb + row) * NSP + c0) * 2;
+    const T pr[2] = {o1[0], o1[2]}, pi[2] = {o1[1], o1[3]};
+    if (AXPBY) {
+      if (EPI == EPI_AXPBY_X) {
+        T xr[2] = {o2[0], o2[2]}, xi[2] = {o2[1], o2[3]};
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) {
+          xr[e] += c2r[e] * pr[e] - c2i[e] * pi[e];
+          xi[e] += c2r[e] * pi[e] + c2i[e] * pr[e];
+        }
+        Pack<T>::store(xacc + off, xr, xi);
+      }
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) {
+        ar[e] += cr[e] * pr[e] - ci[e] * pi[e];
+        ai[e] += cr[e] * pi[e] + ci[e] * pr[e];
+      }
+    }
+    if (EPI == EPI_AXPY_SELF) {
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) {
+        const T tr = ar[e], ti = ai[e];
+        ar[e] = pr[e] - (cr[e] * tr - ci[e] * ti);
+        ai[e] = pi[e] - (cr[e] * ti + ci[e] * tr);
+      }
+    }
+    if (EPI != EPI_DOT2) Pack<T>::store(y + off, ar, ai);
+    if (EPI == EPI_STORE_SELF || EPI == EPI_AXPY_SELF || EPI == EPI_STORE_SELF2) {
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) {
+        d1.re[e] += (double)ar[e] * ar[e] - (double)ai[e] * ai[e];
+        d1.im[e] += 2.0 * (double)ar[e] * ai[e];
+        if (EPI != EPI_STORE_SELF) d2.re[e] += (double)ar[e] * ar[e] + (double)ai[e] * ai[e];
+      }
+    }
+    if (EPI == EPI_STORE_DOT || EPI == EPI_DOT2) {
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) {
+        d1.re[e] += (double)pr[e] * ar[e] - (double)pi[e] * ai[e];
+        d1.im[e] += (double)pr[e] * ai[e] + (double)pi[e] * ar[e];
+        if (EPI == EPI_DOT2) {
+          d2.re[e] += (double)ar[e] * ar[e] - (double)ai[e] * ai[e];
+          d2.im[e] += 2.0 * (double)ar[e] * ai[e];
+        }
+      }
+    }
+  }
+  // the block's partial sums (all threads call it)
+  __device__ inline void finish(double* sh, double* part1, double* part2, int wl) {
+    if (EPI != EPI_STORE && !AXPBY) {
+      block_reduce_store<T, NSP>(d1, sh, part1, wl);
+      if (EPI == EPI_DOT2 || EPI == EPI_AXPY_SELF || EPI == EPI_STORE_SELF2) {
+        __syncthreads();
+        block_reduce_store<T, NSP>(d2, sh, part2, wl);
+      }
+    }
+  }
+};
+
 // 6 waves per SIMD (at most 80 VGPRs) measured best for U = 8.
 template <typename T, int NSP, int MAT, int EPI, int U>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(U <= 8 ? 6 : 4))) void k_spmv(
@@ -228,66 +381,208 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(U <= 8 ? 6 
   const T* gv = v0;
   if (MAT == MAT_A) fk = (T)W.gfsc[it.g];
   else gv = v0 + 2 * W.ggv[it.g];
-  Acc<CPL> d1, d2;
-  d1.zero();
-  d2.zero();
-  constexpr bool AXPBY = EPI == EPI_AXPBY || EPI == EPI_AXPBY_X;
-  const long long rhs0 = (long long)it.gl * NSP + c0;
-  T cr[CPL], ci[CPL], c2r[CPL], c2i[CPL];
-  if (AXPBY) load_coef<T, CPL>(coef, rhs0, cr, ci);
-  if (EPI == EPI_AXPBY_X) load_coef<T, CPL>(coef2, rhs0, c2r, c2i);
-  const long long vb = it.vb;
-  spmv_rows<T, NSP, MAT, U>(rptr, cptr, gv, v1, fk, x + c0 * 2, vb, it.row0, it.row1, srp, scol, sval,
-                            [&](int row, T* ar, T* ai) {
-    const long long off = ((vb + row) * NSP + c0) * 2;
-    if (AXPBY) {
-      T pr[CPL], pi[CPL];
-      Pack<T>::load(y + off, pr, pi);
-      if (EPI == EPI_AXPBY_X) {
-        T xr[CPL], xi[CPL];
-        Pack<T>::load(xacc + off, xr, xi);
-#pragma unroll
-        for (int e = 0; e < CPL; ++e) {
-          xr[e] += c2r[e] * pr[e] - c2i[e] * pi[e];
-          xi[e] += c2r[e] * pi[e] + c2i[e] * pr[e];
-        }
-        Pack<T>::store(xacc + off, xr, xi);
-      }
-#pragma unroll
-      for (int e = 0; e < CPL; ++e) {
-        ar[e] += cr[e] * pr[e] - ci[e] * pi[e];
-        ai[e] += cr[e] * pi[e] + ci[e] * pr[e];
-      }
-    }
-    if (EPI != EPI_DOT2) Pack<T>::store(y + off, ar, ai);
-    if (EPI == EPI_STORE_SELF) {
-#pragma unroll
-      for (int e = 0; e < CPL; ++e) {
-        d1.re[e] += (double)ar[e] * ar[e] - (double)ai[e] * ai[e];
-        d1.im[e] += 2.0 * (double)ar[e] * ai[e];
-      }
-    }
-    if (EPI == EPI_STORE_DOT || EPI == EPI_DOT2) {
-      T wr[CPL], wi[CPL];
-      Pack<T>::load(w + off, wr, wi);
-#pragma unroll
-      for (int e = 0; e < CPL; ++e) {
-        d1.re[e] += (double)wr[e] * ar[e] - (double)wi[e] * ai[e];
-        d1.im[e] += (double)wr[e] * ai[e] + (double)wi[e] * ar[e];
-        if (EPI == EPI_DOT2) {
-          d2.re[e] += (double)ar[e] * ar[e] - (double)ai[e] * ai[e];
-          d2.im[e] += 2.0 * (double)ar[e] * ai[e];
-        }
-      }
-    }
-  });
-  if (EPI != EPI_STORE && !AXPBY) {
-    block_reduce_store<T, NSP>(d1, sh, part1, it.wl);
-    if (EPI == EPI_DOT2) {
-      __syncthreads();
-      block_reduce_store<T, NSP>(d2, sh, part2, it.wl);
-    }
+  SpmvEpi<T, NSP, EPI> epi(y, w, xacc, coef, coef2, it.vb, c0, (long long)it.gl * NSP + c0);
+  spmv_rows<T, NSP, MAT, U>(rptr, cptr, gv, v1, fk, x + c0 * 2, it.vb, it.row0, it.row1, srp, scol, sval, epi);
+  epi.finish(sh, part1, part2, it.wl);
+}
+
+// x-halo form of k_spmv for complex64 (NSP <= 32).  For every 64-row chunk
+// of a mesh the handle holds the sorted distinct columns its rows reference
+// (the chunk's halo: about 184 rows for A and 124 for G or G^T on the
+// 15-point Kuhn stencil, against 934 and 499 gathers) and, per nonzero, the
+// column's slot in that list.  Each chunk's halo rows of x are copied into
+// LDS once and every row product reads x from there, so L2 requests fall by
+// the halo ratio (5.1x for A, 4.0x for G).
+//
+// The LDS holds one chunk (kHaloCap rows of NSP complex64 columns: two
+// blocks per CU at NSP = 32), so latency is hidden by software pipelining
+// over the chunks of the block's tile instead of by occupancy: while chunk c
+// is computed from LDS, chunk c+1's x rows and matrix slice are in flight to
+// registers and chunk c+2's halo list is being fetched.  Every load comes
+// from an index clamped into range (a clamped lane writes the same value to
+// the same slot as the lane that owns it), so no load sits behind a branch.
+constexpr int kHaloCap = 240;
+constexpr int kHaloTileRows = 1024;
+// native 4-float vector: copies of it stay in registers (HIP's float4 struct
+// is copied with memcpy, which kept the staged rows in scratch memory)
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+struct Halo {
+  const int* hptr;             // halo chunk -> first entry of its list
+  const int* hcol;             // halo lists (columns, mesh-global numbering)
+  const unsigned char* hloc;   // nonzero -> slot in its chunk's list
+  const int* gchb;             // group -> first halo chunk of its mesh
+  const unsigned char* gok;    // group -> its mesh's chunks all fit the LDS halo
+  const int* cptr;             // CSR columns (the gather path of meshes that do not fit)
+};
+
+template <int NSP, int MAT, int EPI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_spmv_halo(
+    const int* __restrict__ rptr, const float* __restrict__ v0, const float* __restrict__ v1,
+    const float* __restrict__ x, float* __restrict__ y, const float* __restrict__ w,
+    const double* __restrict__ coef, const double* __restrict__ coef2, float* __restrict__ xacc,
+    double* __restrict__ part1, double* __restrict__ part2, Work W, Halo H, const int* __restrict__ grp_active) {
+  using G = Geo<float, NSP>;
+  constexpr int CPL = G::CPL, LPR = G::LPR;
+  constexpr int PPT = (kHaloCap * LPR + 255) / 256;   // 16-byte halo pieces per thread
+  constexpr int EPT = kChunkCap / 256;                // chunk nonzeros per thread
+  constexpr int MAXCH = kHaloTileRows / kChunkRows;
+  Item it;
+  if (!work_item(W, it)) return;
+  if (grp_active && !grp_active[it.gl]) return;
+  __shared__ double sh[2 * 4 * NSP];
+  __shared__ int srp[kHaloTileRows + 1];
+  __shared__ int shp[MAXCH + 1];
+  __shared__ unsigned char scol[kChunkCap];
+  __shared__ float sval[2 * kChunkCap];
+  __shared__ f4 xs[kHaloCap * LPR];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int rsub = lane / LPR, cl = lane % LPR, c0 = cl * CPL;
+  float fk = 0.f;
+  const float* gv = v0;
+  if (MAT == MAT_A) fk = (float)W.gfsc[it.g];
+  else gv = v0 + 2 * W.ggv[it.g];
+  SpmvEpi<float, NSP, EPI> epi(y, w, xacc, coef, coef2, it.vb, c0, (long long)it.gl * NSP + c0);
+  if (!H.gok[it.g]) {
+    // a mesh with a chunk whose halo exceeds the LDS: the gather row loop,
+    // so that a mesh takes the same path alone and in a batch
+    spmv_rows<float, NSP, MAT, 4>(rptr, H.cptr, gv, v1, fk, x + c0 * 2, it.vb, it.row0, it.row1, srp,
+                                  reinterpret_cast<int*>(xs), sval, epi);
+    epi.finish(sh, part1, part2, it.wl);
+    return;
   }
+  const f4* xrow = reinterpret_cast<const f4*>(x);   // node n, piece l at n * LPR + l
+  const int nrow = it.row1 - it.row0;
+  const int nch = (nrow + kChunkRows - 1) / kChunkRows;
+  const int ch0 = H.gchb[it.g] + (it.row0 - W.grow0[it.g]) / kChunkRows;
+  for (int q = tid; q <= nrow; q += 256) srp[q] = rptr[it.row0 + q];
+  for (int q = tid; q <= nch; q += 256) shp[q] = H.hptr[ch0 + q];
+  __syncthreads();
+
+  // staged state of the next chunk (registers; passed to the stages by
+  // reference, which keeps them out of the lambdas' captures and in VGPRs)
+  unsigned char ln[EPT];
+  float mn0[EPT], mn1[EPT];
+  f4 xn[PPT];
+  int hcA[PPT], hcB[PPT];
+  using XN = f4[PPT];
+  using LN = unsigned char[EPT];
+  using MN = float[EPT];
+  auto load_hcol = [&](int c, int* hc) __attribute__((always_inline)) {
+    const int hs = shp[c], npc = (shp[c + 1] - hs) * LPR;
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) hc[i] = H.hcol[hs + min(tid + 256 * i, npc - 1) / LPR];
+  };
+  auto load_next = [&](int c, const int* hc, XN& xv, LN& lv, MN& m0v, MN& m1v) __attribute__((always_inline)) {
+    const int es = srp[c * kChunkRows], ne = srp[min((c + 1) * kChunkRows, nrow)] - es;
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+      const int q = min(tid + 256 * i, ne - 1);
+      lv[i] = H.hloc[es + q];
+      if (MAT == MAT_A) {
+        m0v[i] = v0[es + q];
+        m1v[i] = v1[es + q];
+      } else {
+        m0v[i] = gv[2 * (long long)(es + q)];
+        m1v[i] = gv[2 * (long long)(es + q) + 1];
+      }
+    }
+    const int npc = (shp[c + 1] - shp[c]) * LPR;
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+      const int q = min(tid + 256 * i, npc - 1);
+      xv[i] = xrow[(it.vb + hc[i]) * LPR + q % LPR];
+    }
+  };
+  auto store_next = [&](int c, const XN& xv, const LN& lv, const MN& m0v, const MN& m1v)
+                        __attribute__((always_inline)) {
+    const int es = srp[c * kChunkRows], ne = srp[min((c + 1) * kChunkRows, nrow)] - es;
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+      const int q = min(tid + 256 * i, ne - 1);
+      scol[q] = lv[i];
+      sval[2 * q] = m0v[i];
+      sval[2 * q + 1] = MAT == MAT_A ? fk * m1v[i] : m1v[i];
+    }
+    const int npc = (shp[c + 1] - shp[c]) * LPR;
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) xs[min(tid + 256 * i, npc - 1)] = xv[i];
+  };
+  // epilogue operands of this thread's rows of the current chunk, loaded
+  // before the next chunk's prefetch so that waiting for them leaves the
+  // prefetch in flight (the vector-memory counter retires in order)
+  constexpr int NST = kChunkRows / G::RPB;
+  f4 eo1[NST], eo2[NST];
+  auto load_ops = [&](int c) __attribute__((always_inline)) {
+    const int rb = c * kChunkRows;
+#pragma unroll
+    for (int st = 0; st < NST; ++st) {
+      const int lr = min(rb + st * G::RPB + wv * G::RPW + rsub, nrow - 1);
+      epi.load_ops(it.row0 + lr, eo1[st], eo2[st]);
+    }
+  };
+  auto compute = [&](int c) __attribute__((always_inline)) {
+    const int rb = c * kChunkRows, re = min(rb + kChunkRows, nrow), es = srp[rb];
+#pragma unroll
+    for (int st = 0; st < NST; ++st) {
+      const int lr = rb + st * G::RPB + wv * G::RPW + rsub;
+      if (lr >= re) continue;
+      float ar[CPL], ai[CPL];
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) { ar[e] = 0.f; ai[e] = 0.f; }
+      const int s0 = srp[lr] - es, s1 = srp[lr + 1] - es;
+      for (int p = s0; p < s1; p += 4) {
+        int li[4];
+        float xr[4][CPL], xi[4][CPL], mr[4], mi[4];
+        f4 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) li[q] = scol[min(p + q, s1 - 1)];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int pq = min(p + q, s1 - 1);
+          v[q] = xs[li[q] * LPR + cl];
+          mr[q] = sval[2 * pq];
+          mi[q] = sval[2 * pq + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          Pack<float>::load(reinterpret_cast<const float*>(&v[q]), xr[q], xi[q]);
+          if (p + q >= s1) { mr[q] = 0.f; mi[q] = 0.f; }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int e = 0; e < CPL; ++e) {
+            ar[e] += mr[q] * xr[q][e] - mi[q] * xi[q][e];
+            ai[e] += mr[q] * xi[q][e] + mi[q] * xr[q][e];
+          }
+      }
+      epi.apply(it.row0 + lr, ar, ai, eo1[st], eo2[st]);
+    }
+  };
+  // one pipeline step: LDS holds chunk c; hc1 = halo list of c+1, hc2
+  // receives that of c+2
+  auto step = [&](int c, const int* hc1, int* hc2, XN& xv, LN& lv, MN& m0v, MN& m1v)
+                  __attribute__((always_inline)) {
+    const bool has1 = c + 1 < nch;
+    load_ops(c);
+    if (has1) load_next(c + 1, hc1, xv, lv, m0v, m1v);
+    if (c + 2 < nch) load_hcol(c + 2, hc2);
+    compute(c);
+    __syncthreads();
+    if (has1) store_next(c + 1, xv, lv, m0v, m1v);
+    __syncthreads();
+  };
+  load_hcol(0, hcA);
+  load_next(0, hcA, xn, ln, mn0, mn1);
+  if (nch > 1) load_hcol(1, hcB);
+  store_next(0, xn, ln, mn0, mn1);
+  __syncthreads();
+  for (int c = 0; c < nch; c += 2) {
+    step(c, hcB, hcA, xn, ln, mn0, mn1);
+    if (c + 1 < nch) step(c + 1, hcA, hcB, xn, ln, mn0, mn1);
+  }
+  epi.finish(sh, part1, part2, it.wl);
 }
 
 // Row loop of the vector passes: every row of the block's work item, LPR
@@ -833,18 +1128,22 @@ __global__ __launch_bounds__(kScalThreads) void k_sc_beta(BState s, const double
 // bounded while the rest of its group iterates.
 
 // After the start pass u = G r (part1 = u.u): rho = u.u, beta = 0.
+// part2 (optional): |u|^2, which then becomes the reference norm of the
+// column's relative residual (the u-recurrence form tracks u = G r only).
 __global__ __launch_bounds__(kScalThreads) void k_cg_start(BState s, const double* __restrict__ part1,
-                                                           int nsp, Work W) {
+                                                           const double* __restrict__ part2, int nsp, Work W) {
   const int g = blockIdx.x, c = threadIdx.x;
-  double ur = 0, ui = 0;
+  double ur = 0, ui = 0, n2 = 0, dummy = 0;
   block_sum_part(part1, W, g, nsp, ur, ui);
+  if (part2) block_sum_part(part2, W, g, nsp, n2, dummy);
   bool active = false;
   if (c < nsp) {
     const long long i = (long long)g * nsp + c;
     if (s.status[i] == ST_ACTIVE) {
-      if ((ur == 0.0 && ui == 0.0) || !isfinite(ur) || !isfinite(ui)) {
+      if ((ur == 0.0 && ui == 0.0) || !isfinite(ur) || !isfinite(ui) || (part2 && !(n2 > 0.0))) {
         s.status[i] = DOGPUP_RHS_BREAKDOWN;
       } else {
+        if (part2) s.bnorm[i] = sqrt(n2);
         s.rho[2 * i] = ur;
         s.rho[2 * i + 1] = ui;
         active = true;
@@ -1099,11 +1398,20 @@ struct dogpup_solver {
   const int *rA = nullptr, *cA = nullptr, *rG = nullptr, *cG = nullptr;
   const int *rGT = nullptr, *cGT = nullptr;
   dogpup::Buf pAr, pAc, pGr, pGc, pGTr, pGTc, amap, gmap, gtmap, order;
+  // x-halo lists of the three solver-order patterns (A, G, G^T) per 64-row
+  // chunk of each mesh (k_spmv_halo); halo_ok[m]: every chunk of pattern m
+  // fits the LDS halo; use_halo: the complex64 SpMVs take the halo form
+  dogpup::Buf h_ptr[3], h_col[3], h_loc[3], t_gok[3], t_gchb;
+  bool halo_ok[3] = {false, false, false};   // some mesh of the handle fits
+  std::vector<unsigned char> mesh_ok_tmp[3];  // per mesh (while building)
+  bool use_halo = true;
+  bool cg_rfree = true;   // the refinement form's inner COCG updates u = G r directly
+  std::vector<int> mesh_chb_tmp;   // first halo chunk of each mesh (while building)
   // values: a64re/a64im and g64 in reference order; the rest in solver order
   dogpup::Buf a64re, a64im, a64pre, a64pim, a32re, a32im, fscale, g64, gp64, gtp64, g32, gt32;
   dogpup::Buf ws, part1, part2, state, rstate, bperm, xperm;
   int tile_rows = 512, lds_pad = 0;
-  int unroll_a = 8, unroll_g = 8;  // gathers in flight per row (k_spmv: 4, 8 or 16)
+  int unroll_a = 8, unroll_g = 4;  // gathers in flight per row (k_spmv: 4, 8 or 16)
   double ws_budget = 0;             // workspace budget, fixed when the handle is made
   bool have_matrix = false;
   double stats[4] = {0, 0, 0, 0};
@@ -1262,10 +1570,29 @@ struct Chunk {
       default: return launch<MAT, EPI, 16>(rp, cp, m0, m1, x, y, w, ga, coef, coef2, xacc);
     }
   }
+  // the x-halo form (complex64, NSP <= 32) when the pattern's chunks fit it
+  static constexpr bool kHaloType = std::is_same<T, float>::value && NSP <= 32;
+  bool halo(int m) const { return kHaloType && S->use_halo && S->halo_ok[m]; }
+  template <int MAT, int EPI>
+  int launch_halo(int m, const int* rp, const T* m0, const T* m1, const T* x, T* y, const T* w, const int* ga,
+                  const double* coef, const double* coef2, T* xacc) {
+    if constexpr (kHaloType) {
+      Halo H{S->h_ptr[m].template as<int>(), S->h_col[m].template as<int>(),
+             S->h_loc[m].template as<unsigned char>(), S->t_gchb.template as<int>(),
+             S->t_gok[m].template as<unsigned char>(), m == 0 ? S->cA : (m == 1 ? S->cG : S->cGT)};
+      k_spmv_halo<NSP, MAT, EPI><<<vgrid(), 256, 0, st>>>(rp, m0, m1, x, y, w, coef, coef2, xacc, p1, p2, W, H,
+                                                          ga);
+      DP_LAUNCH_CHECK();
+    }
+    return DOGPUP_OK;
+  }
   template <int EPI>
   int spmvA(const T* x, T* y, const T* w, const int* ga) {
     pbegin();
-    DP_TRY((launch_u<MAT_A, EPI>(S->unroll_a, S->rA, S->cA, a0(), a1(), x, y, w, ga, nullptr, nullptr, nullptr)));
+    if (halo(0))
+      DP_TRY((launch_halo<MAT_A, EPI>(0, S->rA, a0(), a1(), x, y, w, ga, nullptr, nullptr, nullptr)));
+    else
+      DP_TRY((launch_u<MAT_A, EPI>(S->unroll_a, S->rA, S->cA, a0(), a1(), x, y, w, ga, nullptr, nullptr, nullptr)));
     pend(1, rows(ga != nullptr) * rowbytes() * ((EPI == EPI_STORE || w == x) ? 2 : 3) +
                 (double)S->nnzA * (2 * sizeof(T) + 4) + 4.0 * (S->nn + 1));
     return DOGPUP_OK;
@@ -1274,9 +1601,13 @@ struct Chunk {
   int spmvG(bool tr, const T* x, T* y, const T* w, const int* ga, const double* coef = nullptr,
             const double* coef2 = nullptr, T* xacc = nullptr) {
     pbegin();
-    DP_TRY((launch_u<MAT_G, EPI>(S->unroll_g, tr ? S->rGT : S->rG, tr ? S->cGT : S->cG, gvals(tr), nullptr, x, y,
-                                 w, ga, coef, coef2, xacc)));
-    const int streams = (EPI == EPI_STORE || EPI == EPI_DOT2 || EPI == EPI_STORE_SELF) ? 2
+    if (halo(tr ? 2 : 1))
+      DP_TRY((launch_halo<MAT_G, EPI>(tr ? 2 : 1, tr ? S->rGT : S->rG, gvals(tr), nullptr, x, y, w, ga, coef, coef2,
+                                      xacc)));
+    else
+      DP_TRY((launch_u<MAT_G, EPI>(S->unroll_g, tr ? S->rGT : S->rG, tr ? S->cGT : S->cG, gvals(tr), nullptr, x, y,
+                                   w, ga, coef, coef2, xacc)));
+    const int streams = (EPI == EPI_STORE || EPI == EPI_DOT2 || EPI == EPI_STORE_SELF || EPI == EPI_STORE_SELF2) ? 2
                         : (EPI == EPI_AXPBY_X ? 5 : 3);
     // G values: one copy per group, nnzG/nn entries per row on average
     pend(0, rows(ga != nullptr) * (rowbytes() * streams + (double)S->nnzG / S->nn * (2 * sizeof(T) + 4)) +
@@ -1363,7 +1694,28 @@ struct Chunk {
   // seven and 30).  The G^T pass runs for the groups that were active when
   // the iteration began (grp_prev), so a group that converges in this
   // iteration still receives its last x update.
+  // Inner solve of the refinement form: COCG on the preconditioned
+  // residual u = G r, updated as u -= alpha G q in the G pass itself, so r
+  // is neither updated nor read again (3 passes and 10 vector streams per
+  // iteration).  The column stops on |u|/|u_0|; the outer FP64 residual
+  // check decides convergence.
+  bool rfree() const { return std::is_same<T, float>::value && S->mode == DOGPUP_MODE_C64_REFINE && S->cg_rfree; }
+  int body_cocg_rfree(Vecs<T> V, int maxit) {
+    const int* ga = bs.grp_active;
+    const int nsp = NSP;
+    DP_TRY(spmvA<EPI_STORE_DOT>(V.p, V.v, V.p, ga));                     // q = A p, p.q
+    k_cg_alpha<<<gcount, kScalThreads, 0, st>>>(bs, p1, nsp, W);
+    DP_LAUNCH_CHECK();
+    DP_TRY(spmvG<EPI_AXPY_SELF>(false, V.v, V.u, nullptr, ga, bs.alpha));  // u -= alpha G q; u.u, |u|^2
+    DP_TRY(reset_count());
+    k_cg_beta<<<gcount, kScalThreads, 0, st>>>(bs, p1, p2, nsp, W, maxit);
+    DP_LAUNCH_CHECK();
+    DP_TRY(spmvG<EPI_AXPBY_X>(true, V.u, V.p, nullptr, bs.grp_prev, bs.beta, bs.alpha, V.x));
+    return DOGPUP_OK;
+  }
+
   int body_cocg(Vecs<T> V, int maxit) {
+    if (rfree()) return body_cocg_rfree(V, maxit);
     const int* ga = bs.grp_active;
     const int nsp = NSP;
     DP_TRY(spmvA<EPI_STORE_DOT>(V.p, V.v, V.p, ga));             // q = A p, p.q
@@ -1386,9 +1738,12 @@ struct Chunk {
   // p = G^T u (p holds finite values, beta = 0).
   int cocg_start(Vecs<T> V) {
     const int* ga = bs.grp_active;
-    DP_TRY(spmvG<EPI_STORE_SELF>(false, V.r, V.u, nullptr, ga));
+    if (rfree())
+      DP_TRY(spmvG<EPI_STORE_SELF2>(false, V.r, V.u, nullptr, ga));
+    else
+      DP_TRY(spmvG<EPI_STORE_SELF>(false, V.r, V.u, nullptr, ga));
     DP_TRY(reset_count());
-    k_cg_start<<<gcount, kScalThreads, 0, st>>>(bs, p1, NSP, W);
+    k_cg_start<<<gcount, kScalThreads, 0, st>>>(bs, p1, rfree() ? p2 : nullptr, NSP, W);
     DP_LAUNCH_CHECK();
     DP_TRY(spmvG<EPI_AXPBY>(true, V.u, V.p, nullptr, ga, bs.beta));
     return DOGPUP_OK;
@@ -2051,6 +2406,57 @@ static int create_batch(dogpup_solver_t* out, int nmesh, const int* h_row0, cons
   S->rA = S->pAr.as<int>(); S->cA = S->pAc.as<int>();
   S->rG = S->pGr.as<int>(); S->cG = S->pGc.as<int>();
   S->rGT = S->pGTr.as<int>(); S->cGT = S->pGTc.as<int>();
+  // x-halo lists per 64-row chunk of every mesh
+  {
+    std::vector<int> chunk_row0;   // first row of every chunk, all meshes
+    std::vector<int> mesh_chb(nmesh);
+    for (int i = 0; i < nmesh; ++i) {
+      mesh_chb[i] = (int)chunk_row0.size();
+      for (int r = h_row0[i]; r < h_row0[i + 1]; r += kChunkRows) chunk_row0.push_back(r);
+    }
+    const int nch = (int)chunk_row0.size();
+    auto chunk_end = [&](int c) {
+      const int r = chunk_row0[c];
+      return std::min(r + kChunkRows, h_row0[mesh_of[r] + 1]);
+    };
+    const std::vector<int>* pats[3][2] = {{&rAp, &cAp}, {&rGp, &cGp}, {&rGT, &cGT}};
+    for (int m = 0; m < 3; ++m) {
+      const std::vector<int>& rp = *pats[m][0];
+      const std::vector<int>& cp = *pats[m][1];
+      std::vector<int> hp(nch + 1, 0), hc;
+      std::vector<unsigned char> hl(cp.size(), 0), mok(nmesh, 1);
+      std::vector<int> cols;
+      // a mesh fits when every chunk's halo and nonzeros fit the LDS
+      for (int c = 0; c < nch; ++c) {
+        const int e0 = rp[chunk_row0[c]], e1 = rp[chunk_end(c)];
+        cols.assign(cp.begin() + e0, cp.begin() + e1);
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+        if (cols.empty() || (int)cols.size() > kHaloCap || e1 - e0 > kChunkCap) mok[mesh_of[chunk_row0[c]]] = 0;
+      }
+      for (int c = 0; c < nch; ++c) {
+        if (mok[mesh_of[chunk_row0[c]]]) {
+          const int e0 = rp[chunk_row0[c]], e1 = rp[chunk_end(c)];
+          cols.assign(cp.begin() + e0, cp.begin() + e1);
+          std::sort(cols.begin(), cols.end());
+          cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+          for (int q = e0; q < e1; ++q)
+            hl[q] = (unsigned char)(std::lower_bound(cols.begin(), cols.end(), cp[q]) - cols.begin());
+          hc.insert(hc.end(), cols.begin(), cols.end());
+        }
+        hp[c + 1] = (int)hc.size();
+      }
+      S->halo_ok[m] = std::find(mok.begin(), mok.end(), 1) != mok.end();
+      S->mesh_ok_tmp[m] = mok;
+      if (S->halo_ok[m]) {
+        if (hc.empty()) hc.push_back(0);
+        DP_TRY(up(S->h_ptr[m], hp));
+        DP_TRY(up(S->h_col[m], hc));
+        DP_TRY(upload(S->h_loc[m], hl.data(), hl.size()));
+      }
+    }
+    S->mesh_chb_tmp = mesh_chb;
+  }
 
   // meshes and groups
   S->m_row0.assign(h_row0, h_row0 + nmesh + 1);
@@ -2095,6 +2501,20 @@ static int create_batch(dogpup_solver_t* out, int nmesh, const int* h_row0, cons
   DP_TRY(upload(S->t_gvb, S->h_gvb.data(), g * 8));
   DP_TRY(upload(S->t_ggv, S->h_ggv.data(), g * 8));
   DP_TRY(upload(S->t_gcols, S->h_gcols.data(), g * 4));
+  {
+    std::vector<int> gchb(g);
+    for (int q = 0; q < g; ++q) gchb[q] = S->mesh_chb_tmp[S->h_gmesh[q]];
+    DP_TRY(upload(S->t_gchb, gchb.data(), g * 4));
+    S->mesh_chb_tmp.clear();
+    for (int m = 0; m < 3; ++m) {
+      std::vector<unsigned char> gok(g);
+      for (int q = 0; q < g; ++q) gok[q] = S->mesh_ok_tmp[m][S->h_gmesh[q]];
+      DP_TRY(upload(S->t_gok[m], gok.data(), g));
+      S->mesh_ok_tmp[m].clear();
+    }
+  }
+  if (const char* e = getenv("DOGPUP_HALO")) S->use_halo = atoi(e) != 0;
+  if (const char* e = getenv("DOGPUP_CG_RFREE")) S->cg_rfree = atoi(e) != 0;
   DP_TRY(S->t_gfsc.ensure((size_t)g * 8));
   DP_TRY(S->fscale.ensure((size_t)fs * 8));
   DP_TRY(build_work(S.get()));

@@ -35,12 +35,13 @@ def _f64(a):
 
 
 def group_width(ncols: int) -> int:
-    """Columns per solver group: the smallest of 8/16/32/64 holding ncols,
-    64 for more (several column blocks per frequency)."""
-    for w in (8, 16, 32, 64):
-        if ncols <= w:
-            return w
-    return 64
+    """Default columns per solver group: 8 for up to 8 columns, else 16
+    (several column blocks per frequency).  At 16 columns a chunk's x halo
+    (64 rows' stencil, 128 B per node) fits the LDS three times per CU, and
+    on the 269k-node, 32-source workload 16-wide groups beat 32-wide ones in
+    every form (mixed COCG 602 vs 633 ms, reference-structure FP64 10.3 vs
+    10.8 s).  Pass nsp explicitly for 32 or 64."""
+    return 8 if ncols <= 8 else 16
 
 
 class DOptode:

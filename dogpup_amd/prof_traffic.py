@@ -55,16 +55,20 @@ def algorithmic(name: str, method: str, nodes: int, groups: int, nsp: int, nnz_a
     V = float(nodes) * nsp * 8 * groups              # one complex64 vector stream, all groups
     MA = nnz_a * 12.0 + 4.0 * (nodes + 1)            # A: re, im (float) and column, once per launch
     MG = groups * nnz_g * 12.0 + 4.0 * (nodes + 1)   # G: complex64 per frequency and column
-    m = re.match(r"dogpup::k_spmv<float, (\d+), (\d), (\d)(?:, \d+)?>", name)
+    m = (re.match(r"dogpup::k_spmv<float, (\d+), (\d), (\d)(?:, \d+)?>", name)
+         or re.match(r"dogpup::k_spmv_halo<(\d+), (\d), (\d)>", name))
     if m:
         mat, epi = int(m.group(2)), int(m.group(3))
-        streams = {0: 2, 1: 2 if method == "cocg" else 3, 2: 2, 3: 2, 4: 3, 5: 5}[epi]
+        # vector streams per epilogue (STORE_DOT in COCG reads w = x, the
+        # gathered vector); AXPY_SELF: x in, y in and out; STORE_SELF2: x, y
+        streams = {0: 2, 1: 2 if method == "cocg" else 3, 2: 2, 3: 2, 4: 3, 5: 5, 6: 3, 7: 2}[epi]
         return (1, streams * V + MA) if mat == 0 else (0, streams * V + MG)
-    table = {"dogpup::k_cg_update<float, 32>": (2, 3 * V), "dogpup::k_update_xr<float, 32>": (2, 8 * V),
-             "dogpup::k_update_p<float, 32>": (2, 4 * V), "dogpup::k_axpy_s<float, 32>": (2, 3 * V),
+    table = {f"dogpup::k_cg_update<float, {nsp}>": (2, 3 * V), f"dogpup::k_update_xr<float, {nsp}>": (2, 8 * V),
+             f"dogpup::k_update_p<float, {nsp}>": (2, 4 * V), f"dogpup::k_axpy_s<float, {nsp}>": (2, 3 * V),
              # x64 and b in complex128 (2 V each), r, p, x out in complex64 (+ rhat for BiCGSTAB),
              # A in FP64
-             "dogpup::k_residual64<32>": (3, (7 if method == "cocg" else 8) * V + nnz_a * 20.0 + 4.0 * (nodes + 1))}
+             f"dogpup::k_residual64<{nsp}>": (3, (7 if method == "cocg" else 8) * V + nnz_a * 20.0
+                                              + 4.0 * (nodes + 1))}
     return table.get(name)
 
 
@@ -74,8 +78,8 @@ def main(argv=None):
     ap.add_argument("write_csv")
     ap.add_argument("--method", default="cocg", choices=["cocg", "bicgstab"])
     ap.add_argument("--nodes", type=int, default=269001)
-    ap.add_argument("--groups", type=int, default=64)
-    ap.add_argument("--nsp", type=int, default=32)
+    ap.add_argument("--groups", type=int, default=128, help="frequency x column-block groups")
+    ap.add_argument("--nsp", type=int, default=16)
     ap.add_argument("--nnz-a", type=int, default=3930201)
     ap.add_argument("--nnz-g", type=int, default=2099601)
     ap.add_argument("--copy-bytes", type=float, default=float(2 << 30))
