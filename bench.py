@@ -4,21 +4,27 @@ Workload: 80x80x40 mm slab, 1 mm Kuhn grid (269,001 nodes, 1,536,000 tets),
 mua 0.01/mm (6 mm sphere of 0.03/mm), musp 1.0/mm, n 1.4; 32 sources, 64
 frequencies 0-3.15 GHz in 50 MHz steps = 2048 complex right-hand sides.
 
-One step = FSAI factor setup for all 64 frequencies + the batched
-complex64 FSAI-BiCGSTAB with FP64 iterative refinement to a relative
-residual of 1e-10, from device-resident A and Q to converged fluence.
-value = 2048 * steps / timed seconds (RHS solves/s).
+One step = FSAI factor setup for all 64 frequencies + the batched complex64
+FSAI-preconditioned COCG inside FP64 iterative refinement to a relative
+residual of 1e-8, from device-resident A and Q to converged fluence, + fc2td
+of the nodal fluence to 256 time bins.  value = 2048 * steps / timed seconds
+(RHS solves/s).
 
 Also measured in the same run (reported beside the metric):
+  * the device copy rate, the fc2td DFT alone and the mixed solve with
+    BiCGSTAB (the reference's method),
   * the FP64 fused solve (tol 1e-12) and the relative fluence error of the
     mixed solve against it,
   * the reference-structure FP64 solve (separate passes, lockstep, host
     checks, tol 1e-12) and the speed-up of the mixed form over it,
   * the adjoint Jacobian: forward+adjoint (4096 RHS) mixed solve plus the
-    fused J-TG kernel (1024 pairs x 8 gates x 269,001 nodes),
+    fused J-TG kernel (1024 pairs x 8 gates x 269,001 nodes), and the voxel
+    basis (mesh2grid at 1 and 2 mm, J-TG over the 2 mm voxels),
+  * the 48-mesh uneven batch (autotuned shapes, FP64 comparison),
   * the roofline of the dominant kernel from HIP events around every launch
     in the timed region,
   * the oracle's FP64 FSAI-BiCGSTAB on the host cores for a bounded sample.
+  * with --lm, the weighted LM reconstruction of two inclusions.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
