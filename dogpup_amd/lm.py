@@ -90,7 +90,7 @@ def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed
     opt = mesh.optode
     data_m = np.asarray(data_m, float)
     incl = id_thresh(torch.as_tensor(data_m), 0.8).cpu().numpy().ravel(order="F")   # :25-26
-    ym = np.log(data_m.ravel(order="F"))[incl]                                     # :28-30
+    ym = np.log(data_m.ravel(order="F")[incl])                                     # :28-30
     gate_fc = td2fc(torch.as_tensor(np.asarray(gates, float)), opt.fAxis, opt.tAxis, 1)  # :32
     basis = getattr(mesh, "basis", None)
     spots = None
@@ -126,7 +126,7 @@ def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed
         torch.cuda.synchronize()
         res.t_forward.append(time.perf_counter() - t0)
         yc = yc.cpu().numpy()
-        dc = np.log(yc.ravel(order="F"))[incl]                                     # :62-63
+        dc = np.log(yc.ravel(order="F")[incl])                                     # :62-63
         J = J.index_select(0, rows)                                                # :64-65
         t1 = time.perf_counter()
         y = ym - dc                                                                # :67
