@@ -1572,7 +1572,11 @@ struct Chunk {
   }
   // the x-halo form (complex64, NSP <= 32) when the pattern's chunks fit it
   static constexpr bool kHaloType = std::is_same<T, float>::value && NSP <= 32;
-  bool halo(int m) const { return kHaloType && S->use_halo && S->halo_ok[m]; }
+  // (the halo kernel holds its tile's row pointers in LDS: tiles of at most
+  // kHaloTileRows rows)
+  bool halo(int m) const {
+    return kHaloType && S->use_halo && S->halo_ok[m] && S->tile_rows <= kHaloTileRows;
+  }
   template <int MAT, int EPI>
   int launch_halo(int m, const int* rp, const T* m0, const T* m1, const T* x, T* y, const T* w, const int* ga,
                   const double* coef, const double* coef2, T* xacc) {
