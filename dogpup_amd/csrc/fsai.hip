@@ -16,6 +16,8 @@
 #include "common.hpp"
 #include "fsai.hpp"
 
+#include <cstdlib>
+
 namespace dogpup {
 
 template <int MMAX, int BS>
@@ -124,6 +126,145 @@ __global__ __launch_bounds__(BS) void k_fsai_rows(
 #undef AI
 }
 
+// Rows with m <= 8 (every row of a Kuhn mesh): the local system in
+// registers.  A_k[P, P] is complex symmetric with a positive definite real
+// part (stiffness + Robin + k*i*mass), so its LDL^T factorisation needs no
+// pivoting and gives the same solution as the pivoted elimination above to
+// roundoff.  The system is padded to 8 x 8 with the identity (which leaves
+// the first m unknowns unchanged), every loop is unrolled over compile-time
+// indices, and the 36 lower-triangle entries and the unknowns stay in VGPRs:
+// 2 waves per SIMD instead of the LDS form's 73.7 KB per 64 lanes (half a
+// wave per SIMD).  Only e_m's solution is needed: with L unit lower
+// triangular, L y = e_m gives y = e_m, so x_m = 1/d_m and
+// x_i = -sum_{j>i} L_ji x_j.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_fsai_ldl8(const double* __restrict__ are, const double* __restrict__ aim,
+                                                  int a_il, const int* __restrict__ rptrA,
+                                                  const int* __restrict__ cptrA, const int* __restrict__ rptrG,
+                                                  const int* __restrict__ cptrG, int nrows, int nfreqs,
+                                                  const double* __restrict__ fscale, FsaiOut out) {
+  constexpr int M = 8;
+  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
+  if (t >= (long long)nrows * nfreqs) return;
+  const int row = (int)(t / nfreqs);
+  const int k = (int)(t % nfreqs);
+  const int gs = rptrG[row];
+  const int m = rptrG[row + 1] - gs;
+  if (m > M) return;
+  const double fk = fscale ? fscale[k] : (double)k;
+  // packed lower triangle: entry (a, b), b <= a, at a(a+1)/2 + b
+  double lr[M * (M + 1) / 2], li[M * (M + 1) / 2];
+#pragma unroll
+  for (int a = 0; a < M; ++a)
+#pragma unroll
+    for (int b = 0; b <= a; ++b) {
+      lr[a * (a + 1) / 2 + b] = (a == b && a >= m) ? 1.0 : 0.0;
+      li[a * (a + 1) / 2 + b] = 0.0;
+    }
+  // gather A_k[P, P] from the lower-triangle entries (P ascending)
+#pragma unroll
+  for (int a = 0; a < M; ++a) {
+    if (a < m) {
+      const int ra = cptrG[gs + a];
+      const int s1 = rptrA[ra + 1];
+      int p = rptrA[ra];
+#pragma unroll
+      for (int b = 0; b <= a; ++b) {
+        const int cb = cptrG[gs + b];
+        while (p < s1 && cptrA[p] < cb) ++p;
+        if (p < s1 && cptrA[p] == cb) {
+          double vr, vi;
+          if (a_il) { vr = are[2LL * p]; vi = are[2LL * p + 1]; }
+          else { vr = are[p]; vi = aim[p]; }
+          lr[a * (a + 1) / 2 + b] = vr;
+          li[a * (a + 1) / 2 + b] = vi * fk;
+        }
+      }
+    }
+  }
+  // in-place LDL^T: diagonal -> d_j, strict lower -> L_ij (unconjugated)
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    cpx<double> wj[M];   // w_jk = L_jk d_k
+#pragma unroll
+    for (int q = 0; q < j; ++q)
+      wj[q] = cmul(cpx<double>{lr[j * (j + 1) / 2 + q], li[j * (j + 1) / 2 + q]},
+                   cpx<double>{lr[q * (q + 1) / 2 + q], li[q * (q + 1) / 2 + q]});
+    cpx<double> d{lr[j * (j + 1) / 2 + j], li[j * (j + 1) / 2 + j]};
+#pragma unroll
+    for (int q = 0; q < j; ++q) {
+      const cpx<double> pr = cmul(cpx<double>{lr[j * (j + 1) / 2 + q], li[j * (j + 1) / 2 + q]}, wj[q]);
+      d.re -= pr.re;
+      d.im -= pr.im;
+    }
+    lr[j * (j + 1) / 2 + j] = d.re;
+    li[j * (j + 1) / 2 + j] = d.im;
+#pragma unroll
+    for (int i = j + 1; i < M; ++i) {
+      cpx<double> v{lr[i * (i + 1) / 2 + j], li[i * (i + 1) / 2 + j]};
+#pragma unroll
+      for (int q = 0; q < j; ++q) {
+        const cpx<double> pr = cmul(cpx<double>{lr[i * (i + 1) / 2 + q], li[i * (i + 1) / 2 + q]}, wj[q]);
+        v.re -= pr.re;
+        v.im -= pr.im;
+      }
+      const cpx<double> l = cdiv(v, d);
+      lr[i * (i + 1) / 2 + j] = l.re;
+      li[i * (i + 1) / 2 + j] = l.im;
+    }
+  }
+  // x_{m-1} = 1 / d_{m-1}; x_i = -sum_{j=i+1}^{m-1} L_ji x_j
+  cpx<double> x[M];
+#pragma unroll
+  for (int i = 0; i < M; ++i) x[i] = cpx<double>{0.0, 0.0};
+#pragma unroll
+  for (int i = M - 1; i >= 0; --i) {
+    if (i == m - 1) {
+      x[i] = cdiv(cpx<double>{1.0, 0.0}, cpx<double>{lr[i * (i + 1) / 2 + i], li[i * (i + 1) / 2 + i]});
+    } else if (i < m - 1) {
+      cpx<double> sacc{0.0, 0.0};
+#pragma unroll
+      for (int j = i + 1; j < M; ++j) {
+        if (j < m) {
+          const cpx<double> pr = cmul(cpx<double>{lr[j * (j + 1) / 2 + i], li[j * (j + 1) / 2 + i]}, x[j]);
+          sacc.re += pr.re;
+          sacc.im += pr.im;
+        }
+      }
+      x[i] = cpx<double>{-sacc.re, -sacc.im};
+    }
+  }
+  // principal square root of x_m, as the pivoted kernel
+  cpx<double> xm{0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+    if (i == m - 1) xm = x[i];
+  const double mod = sqrt(xm.re * xm.re + xm.im * xm.im);
+  cpx<double> q;
+  q.re = sqrt((mod + xm.re) / 2);
+  if (xm.im == 0.0) {
+    if (xm.re < 0.0) { q.re = 0.0; q.im = sqrt(-xm.re); }
+    else q.im = 0.0;
+  } else {
+    q.im = (xm.im > 0 ? 1.0 : -1.0) * sqrt((mod - xm.re) / 2);
+  }
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    if (i < m) {
+      const cpx<double> g = cdiv(x[i], q);
+      const long long j = gs + i;
+      if (out.layout == FSAI_OUT_REF) {
+        out.re[k + (long long)nfreqs * j] = g.re;
+        out.im[k + (long long)nfreqs * j] = g.im;
+      } else {
+        double* o = out.re + 2 * ((long long)k * out.nnzG + j);
+        o[0] = g.re;
+        o[1] = g.im;
+      }
+    }
+  }
+}
+
 __global__ void k_max_row_len(const int* __restrict__ rptr, int n, int* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int m = 0;
@@ -148,8 +289,16 @@ int fsai_launch(const double* are, const double* aim, int a_il, const int* rptrA
   DP_HIP(hipFree(dmax));
   if (hmax > 32) return fail(DOGPUP_ERR_UNSUPPORTED, "FSAI pattern row with %d entries (max 32)", hmax);
   const long long total = (long long)nrows * nfreqs;
-  k_fsai_rows<8, 64><<<div_up(total, 64), 64, 0, st>>>(are, aim, a_il, rptrA, cptrA, rptrG, cptrG,
-                                                        nrows, nfreqs, fscale, 0, out);
+  static const bool pivoted = [] {
+    const char* e = getenv("DOGPUP_FSAI_PIVOT");
+    return e && atoi(e) != 0;
+  }();
+  if (pivoted)
+    k_fsai_rows<8, 64><<<div_up(total, 64), 64, 0, st>>>(are, aim, a_il, rptrA, cptrA, rptrG, cptrG,
+                                                          nrows, nfreqs, fscale, 0, out);
+  else
+    k_fsai_ldl8<128><<<div_up(total, 128), 128, 0, st>>>(are, aim, a_il, rptrA, cptrA, rptrG, cptrG,
+                                                           nrows, nfreqs, fscale, out);
   DP_LAUNCH_CHECK();
   if (hmax > 8) {
     k_fsai_rows<16, 16><<<div_up(total, 16), 16, 0, st>>>(are, aim, a_il, rptrA, cptrA, rptrG,
