@@ -30,8 +30,7 @@ class MeshBatch:
 
     ``meshes``: DMesh instances with optodes.  ``which``: 's', 'd' or 'sd' as
     in DMesh.solve_point_sources.  ``nsp``: columns per group (8, 16, 32 or
-    64; default: the smallest holding the widest mesh's columns, capped at
-    32).  ``tuning``: (tile_rows, unroll_a, unroll_g, graph_k), e.g. from
+    64; default: mesh.group_width of the widest mesh's columns).  ``tuning``: (tile_rows, unroll_a, unroll_g, graph_k), e.g. from
     dogpup_amd.autotune.
     """
 
@@ -47,7 +46,7 @@ class MeshBatch:
         self.ncols = [sum(len((m.optode.s_bary if w == "s" else m.optode.d_bary)[0]) for w in which)
                       for m in self.meshes]
         self.nf = [m.optode.Nf for m in self.meshes]
-        self.nsp = int(nsp) if nsp else min(group_width(max(self.ncols)), 32)
+        self.nsp = int(nsp) if nsp else group_width(max(self.ncols), mode)
         self.ncb = [-(-c // self.nsp) for c in self.ncols]
         nn = [m.nn for m in self.meshes]
         self.row0 = np.concatenate([[0], np.cumsum(nn)]).astype(np.int32)

@@ -585,6 +585,248 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   epi.finish(sh, part1, part2, it.wl);
 }
 
+// Two-half x-halo form for 32-column groups (complex64).  A block walks its
+// tile chunk by chunk and each chunk in two halves of 16 columns: the
+// chunk's matrix slice is staged in LDS once and serves both halves, while
+// the x halo in LDS holds one half at a time (kHaloCap rows x 128 B), so
+// three blocks share a CU as with 16-wide groups.  Against 16-wide groups
+// the shared A values and slots and the per-frequency G values are read
+// once per 32 columns instead of once per 16.  The pipeline runs over
+// (chunk, half) sub-steps: while (c, 0) is computed, the half-1 rows of
+// chunk c and the halo list of chunk c+1 are in flight; while (c, 1) is
+// computed, chunk c+1's matrix slice and half-0 rows are.
+template <int MAT, int EPI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_spmv_halo2(
+    const int* __restrict__ rptr, const float* __restrict__ v0, const float* __restrict__ v1,
+    const float* __restrict__ x, float* __restrict__ y, const float* __restrict__ w,
+    const double* __restrict__ coef, const double* __restrict__ coef2, float* __restrict__ xacc,
+    double* __restrict__ part1, double* __restrict__ part2, Work W, Halo H, const int* __restrict__ grp_active) {
+  constexpr int NSP = 32, NH = 16, CPL = 2;
+  constexpr int LPR = NSP / CPL;        // 16-byte pieces per node row
+  constexpr int LPRH = NH / CPL;        // pieces (and lanes) per half row
+  constexpr int RPW = 64 / LPRH, RPB = 4 * RPW;
+  constexpr int NST = kChunkRows / RPB;
+  constexpr int PPT = (kHaloCap * LPRH + 255) / 256;
+  constexpr int EPT = kChunkCap / 256;
+  constexpr int MAXCH = kHaloTileRows / kChunkRows;
+  Item it;
+  if (!work_item(W, it)) return;
+  if (grp_active && !grp_active[it.gl]) return;
+  __shared__ double sh[2 * 4 * NSP];
+  __shared__ int srp[kHaloTileRows + 1];
+  __shared__ int shp[MAXCH + 1];
+  __shared__ unsigned char scol[kChunkCap];
+  __shared__ float sval[2 * kChunkCap];
+  __shared__ f4 xs[kHaloCap * LPRH];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int rsub = lane / LPRH, cl = lane % LPRH;
+  float fk = 0.f;
+  const float* gv = v0;
+  if (MAT == MAT_A) fk = (float)W.gfsc[it.g];
+  else gv = v0 + 2 * W.ggv[it.g];
+  SpmvEpi<float, NSP, EPI> ep0(y, w, xacc, coef, coef2, it.vb, cl * CPL, (long long)it.gl * NSP + cl * CPL);
+  SpmvEpi<float, NSP, EPI> ep1(y, w, xacc, coef, coef2, it.vb, NH + cl * CPL,
+                               (long long)it.gl * NSP + NH + cl * CPL);
+  auto finish = [&]() __attribute__((always_inline)) {
+    // both halves' column sums into the group's partial slots
+    constexpr bool AX = EPI == EPI_AXPBY || EPI == EPI_AXPBY_X;
+    if (EPI == EPI_STORE || AX) return;
+    auto reduce = [&](Acc<CPL>& a0, Acc<CPL>& a1, double* part) __attribute__((always_inline)) {
+#pragma unroll
+      for (int q = 0; q < CPL; ++q)
+#pragma unroll
+        for (int o = LPRH; o < 64; o <<= 1) {
+          a0.re[q] += __shfl_xor(a0.re[q], o);
+          a0.im[q] += __shfl_xor(a0.im[q], o);
+          a1.re[q] += __shfl_xor(a1.re[q], o);
+          a1.im[q] += __shfl_xor(a1.im[q], o);
+        }
+      if (rsub == 0) {
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) {
+          sh[2 * (wv * NSP + cl * CPL + q)] = a0.re[q];
+          sh[2 * (wv * NSP + cl * CPL + q) + 1] = a0.im[q];
+          sh[2 * (wv * NSP + NH + cl * CPL + q)] = a1.re[q];
+          sh[2 * (wv * NSP + NH + cl * CPL + q) + 1] = a1.im[q];
+        }
+      }
+      __syncthreads();
+      if (tid < NSP) {
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < 4; ++ww) {
+          sr += sh[2 * (ww * NSP + tid)];
+          si += sh[2 * (ww * NSP + tid) + 1];
+        }
+        part[2 * ((long long)it.wl * NSP + tid)] = sr;
+        part[2 * ((long long)it.wl * NSP + tid) + 1] = si;
+      }
+    };
+    reduce(ep0.d1, ep1.d1, part1);
+    if (EPI == EPI_DOT2 || EPI == EPI_AXPY_SELF || EPI == EPI_STORE_SELF2) {
+      __syncthreads();
+      reduce(ep0.d2, ep1.d2, part2);
+    }
+  };
+  if (!H.gok[it.g]) {
+    // a mesh with a chunk whose halo exceeds the LDS: the gather row loop
+    SpmvEpi<float, NSP, EPI> ep(y, w, xacc, coef, coef2, it.vb, (lane % LPR) * CPL,
+                                (long long)it.gl * NSP + (lane % LPR) * CPL);
+    spmv_rows<float, NSP, MAT, 4>(rptr, H.cptr, gv, v1, fk, x + (lane % LPR) * CPL * 2, it.vb, it.row0, it.row1,
+                                  srp, reinterpret_cast<int*>(xs), sval, ep);
+    ep.finish(sh, part1, part2, it.wl);
+    return;
+  }
+  const f4* xrow = reinterpret_cast<const f4*>(x);   // node n, piece l at n * LPR + l
+  const int nrow = it.row1 - it.row0;
+  const int nch = (nrow + kChunkRows - 1) / kChunkRows;
+  const int ch0 = H.gchb[it.g] + (it.row0 - W.grow0[it.g]) / kChunkRows;
+  for (int q = tid; q <= nrow; q += 256) srp[q] = rptr[it.row0 + q];
+  for (int q = tid; q <= nch; q += 256) shp[q] = H.hptr[ch0 + q];
+  __syncthreads();
+
+  unsigned char ln[EPT];
+  float mn0[EPT], mn1[EPT];
+  f4 xn[PPT];
+  int hcA[PPT], hcB[PPT];
+  f4 eo1[NST], eo2[NST];
+  using XN = f4[PPT];
+  using HC = int[PPT];
+  auto load_hcol = [&](int c, HC& hc) __attribute__((always_inline)) {
+    const int hs = shp[c], npc = (shp[c + 1] - hs) * LPRH;
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) hc[i] = H.hcol[hs + min(tid + 256 * i, npc - 1) / LPRH];
+  };
+  auto load_mat = [&](int c) __attribute__((always_inline)) {
+    const int es = srp[c * kChunkRows], ne = srp[min((c + 1) * kChunkRows, nrow)] - es;
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+      const int q = min(tid + 256 * i, ne - 1);
+      ln[i] = H.hloc[es + q];
+      if (MAT == MAT_A) {
+        mn0[i] = v0[es + q];
+        mn1[i] = v1[es + q];
+      } else {
+        mn0[i] = gv[2 * (long long)(es + q)];
+        mn1[i] = gv[2 * (long long)(es + q) + 1];
+      }
+    }
+  };
+  auto store_mat = [&](int c) __attribute__((always_inline)) {
+    const int es = srp[c * kChunkRows], ne = srp[min((c + 1) * kChunkRows, nrow)] - es;
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+      const int q = min(tid + 256 * i, ne - 1);
+      scol[q] = ln[i];
+      sval[2 * q] = mn0[i];
+      sval[2 * q + 1] = MAT == MAT_A ? fk * mn1[i] : mn1[i];
+    }
+  };
+  auto load_x = [&](int c, int h, const HC& hc, XN& xv) __attribute__((always_inline)) {
+    const int npc = (shp[c + 1] - shp[c]) * LPRH;
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+      const int q = min(tid + 256 * i, npc - 1);
+      xv[i] = xrow[(it.vb + hc[i]) * LPR + h * LPRH + q % LPRH];
+    }
+  };
+  auto store_x = [&](int c, const XN& xv) __attribute__((always_inline)) {
+    const int npc = (shp[c + 1] - shp[c]) * LPRH;
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) xs[min(tid + 256 * i, npc - 1)] = xv[i];
+  };
+  auto load_ops = [&](int c, SpmvEpi<float, NSP, EPI>& ep) __attribute__((always_inline)) {
+    const int rb = c * kChunkRows;
+#pragma unroll
+    for (int st = 0; st < NST; ++st) {
+      const int lr = min(rb + st * RPB + wv * RPW + rsub, nrow - 1);
+      ep.load_ops(it.row0 + lr, eo1[st], eo2[st]);
+    }
+  };
+  auto compute = [&](int c, SpmvEpi<float, NSP, EPI>& ep) __attribute__((always_inline)) {
+    const int rb = c * kChunkRows, re = min(rb + kChunkRows, nrow), es = srp[rb];
+#pragma unroll
+    for (int st = 0; st < NST; ++st) {
+      const int lr = rb + st * RPB + wv * RPW + rsub;
+      if (lr >= re) continue;
+      float ar[CPL], ai[CPL];
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) { ar[e] = 0.f; ai[e] = 0.f; }
+      const int s0 = srp[lr] - es, s1 = srp[lr + 1] - es;
+      for (int p = s0; p < s1; p += 4) {
+        int li[4];
+        float xr[4][CPL], xi[4][CPL], mr[4], mi[4];
+        f4 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) li[q] = scol[min(p + q, s1 - 1)];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int pq = min(p + q, s1 - 1);
+          v[q] = xs[li[q] * LPRH + cl];
+          mr[q] = sval[2 * pq];
+          mi[q] = sval[2 * pq + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          Pack<float>::load(reinterpret_cast<const float*>(&v[q]), xr[q], xi[q]);
+          if (p + q >= s1) { mr[q] = 0.f; mi[q] = 0.f; }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int e = 0; e < CPL; ++e) {
+            ar[e] += mr[q] * xr[q][e] - mi[q] * xi[q][e];
+            ai[e] += mr[q] * xi[q][e] + mi[q] * xr[q][e];
+          }
+      }
+      ep.apply(it.row0 + lr, ar, ai, eo1[st], eo2[st]);
+    }
+  };
+  // sub-step (c, 0): LDS holds chunk c's matrix and half-0 rows; hc = c's
+  // halo list, hn receives c+1's
+  auto half0 = [&](int c, const HC& hc, HC& hn) __attribute__((always_inline)) {
+    load_ops(c, ep0);
+    load_x(c, 1, hc, xn);
+    if (c + 1 < nch) load_hcol(c + 1, hn);
+    compute(c, ep0);
+    __syncthreads();
+    store_x(c, xn);
+    __syncthreads();
+  };
+  // sub-step (c, 1): LDS holds chunk c's matrix and half-1 rows; hn = c+1's
+  // halo list
+  auto half1 = [&](int c, const HC& hn) __attribute__((always_inline)) {
+    const bool has1 = c + 1 < nch;
+    load_ops(c, ep1);
+    if (has1) {
+      load_mat(c + 1);
+      load_x(c + 1, 0, hn, xn);
+    }
+    compute(c, ep1);
+    __syncthreads();
+    if (has1) {
+      store_mat(c + 1);
+      store_x(c + 1, xn);
+    }
+    __syncthreads();
+  };
+  load_hcol(0, hcA);
+  load_mat(0);
+  load_x(0, 0, hcA, xn);
+  store_mat(0);
+  store_x(0, xn);
+  __syncthreads();
+  for (int c = 0; c < nch; c += 2) {
+    half0(c, hcA, hcB);
+    half1(c, hcB);
+    if (c + 1 < nch) {
+      half0(c + 1, hcB, hcA);
+      half1(c + 1, hcA);
+    }
+  }
+  finish();
+}
+
 // Row loop of the vector passes: every row of the block's work item, LPR
 // lanes per row, CPL columns per lane; `off` indexes the row's first column
 // of this lane and rhs0 its first RHS in the chunk's per-RHS arrays.
@@ -1405,6 +1647,7 @@ struct dogpup_solver {
   bool halo_ok[3] = {false, false, false};   // some mesh of the handle fits
   std::vector<unsigned char> mesh_ok_tmp[3];  // per mesh (while building)
   bool use_halo = true;
+  bool halo_halves = true;   // 32-wide groups: the two-half halo kernel
   bool cg_rfree = true;   // the refinement form's inner COCG updates u = G r directly
   std::vector<int> mesh_chb_tmp;   // first halo chunk of each mesh (while building)
   // values: a64re/a64im and g64 in reference order; the rest in solver order
@@ -1584,6 +1827,13 @@ struct Chunk {
       Halo H{S->h_ptr[m].template as<int>(), S->h_col[m].template as<int>(),
              S->h_loc[m].template as<unsigned char>(), S->t_gchb.template as<int>(),
              S->t_gok[m].template as<unsigned char>(), m == 0 ? S->cA : (m == 1 ? S->cG : S->cGT)};
+      if constexpr (NSP == 32) {
+        if (S->halo_halves) {
+          k_spmv_halo2<MAT, EPI><<<vgrid(), 256, 0, st>>>(rp, m0, m1, x, y, w, coef, coef2, xacc, p1, p2, W, H, ga);
+          DP_LAUNCH_CHECK();
+          return DOGPUP_OK;
+        }
+      }
       k_spmv_halo<NSP, MAT, EPI><<<vgrid(), 256, 0, st>>>(rp, m0, m1, x, y, w, coef, coef2, xacc, p1, p2, W, H,
                                                           ga);
       DP_LAUNCH_CHECK();
@@ -2518,6 +2768,7 @@ static int create_batch(dogpup_solver_t* out, int nmesh, const int* h_row0, cons
     }
   }
   if (const char* e = getenv("DOGPUP_HALO")) S->use_halo = atoi(e) != 0;
+  if (const char* e = getenv("DOGPUP_HALO_HALVES")) S->halo_halves = atoi(e) != 0;
   if (const char* e = getenv("DOGPUP_CG_RFREE")) S->cg_rfree = atoi(e) != 0;
   DP_TRY(S->t_gfsc.ensure((size_t)g * 8));
   DP_TRY(S->fscale.ensure((size_t)fs * 8));

@@ -34,14 +34,22 @@ def _f64(a):
     return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=DEV)
 
 
-def group_width(ncols: int) -> int:
-    """Default columns per solver group: 8 for up to 8 columns, else 16
-    (several column blocks per frequency).  At 16 columns a chunk's x halo
-    (64 rows' stencil, 128 B per node) fits the LDS three times per CU, and
-    on the 269k-node, 32-source workload 16-wide groups beat 32-wide ones in
-    every form (mixed COCG 602 vs 633 ms, reference-structure FP64 10.3 vs
-    10.8 s).  Pass nsp explicitly for 32 or 64."""
-    return 8 if ncols <= 8 else 16
+def group_width(ncols: int, mode: str = "mixed") -> int:
+    """Default columns per solver group, measured on the 269k-node,
+    32-source workload:
+    - up to 8 columns: 8;
+    - the complex64 refinement form ('mixed') with more than 16 columns: 32,
+      whose SpMVs walk each chunk in two 16-column halves (the x halo of one
+      half fits the LDS three times per CU; the matrix slice serves both):
+      579 ms per solve against 601 ms with 16-wide groups;
+    - otherwise 16: the FP64 forms' gather SpMVs are faster on 16-wide groups
+      (reference-structure form 10.3 vs 10.8 s).
+    Pass nsp explicitly for other widths."""
+    if ncols <= 8:
+        return 8
+    if mode == "mixed" and ncols > 16:
+        return 32
+    return 16
 
 
 class DOptode:
@@ -249,7 +257,7 @@ class DMesh:
         key = (ncols, mode)
         s = self._solvers.get(key)
         if s is None:
-            s = FluenceSolver(self, ncols, MODES[mode])
+            s = FluenceSolver(self, ncols, MODES[mode], nsp=group_width(ncols, mode))
             self._solvers[key] = s
         s.set_method(method)
         return s
@@ -304,7 +312,7 @@ class FluenceSolver:
         self.mesh = mesh
         self.nf = mesh.optode.Nf
         self.ncols = ncols
-        self.nsp = int(nsp) if nsp else group_width(ncols)
+        self.nsp = int(nsp) if nsp else group_width(ncols, "mixed" if mode == _lib.MODE_C64_REFINE else "fp64")
         self.ncb = -(-ncols // self.nsp)
         self.mode = mode
         d = mesh._dev

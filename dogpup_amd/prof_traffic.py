@@ -56,7 +56,8 @@ def algorithmic(name: str, method: str, nodes: int, groups: int, nsp: int, nnz_a
     MA = nnz_a * 12.0 + 4.0 * (nodes + 1)            # A: re, im (float) and column, once per launch
     MG = groups * nnz_g * 12.0 + 4.0 * (nodes + 1)   # G: complex64 per frequency and column
     m = (re.match(r"dogpup::k_spmv<float, (\d+), (\d), (\d)(?:, \d+)?>", name)
-         or re.match(r"dogpup::k_spmv_halo<(\d+), (\d), (\d)>", name))
+         or re.match(r"dogpup::k_spmv_halo<(\d+), (\d), (\d)>", name)
+         or re.match(r"dogpup::k_spmv_halo2<()(\d), (\d)>", name))
     if m:
         mat, epi = int(m.group(2)), int(m.group(3))
         # vector streams per epilogue (STORE_DOT in COCG reads w = x, the
@@ -78,8 +79,8 @@ def main(argv=None):
     ap.add_argument("write_csv")
     ap.add_argument("--method", default="cocg", choices=["cocg", "bicgstab"])
     ap.add_argument("--nodes", type=int, default=269001)
-    ap.add_argument("--groups", type=int, default=128, help="frequency x column-block groups")
-    ap.add_argument("--nsp", type=int, default=16)
+    ap.add_argument("--groups", type=int, default=64, help="frequency x column-block groups")
+    ap.add_argument("--nsp", type=int, default=32)
     ap.add_argument("--nnz-a", type=int, default=3930201)
     ap.add_argument("--nnz-g", type=int, default=2099601)
     ap.add_argument("--copy-bytes", type=float, default=float(2 << 30))
