@@ -253,7 +253,8 @@ def batch_leg(tol):
         torch.cuda.synchronize()
         return time.perf_counter() - t2, f, b
 
-    default_nsp = dm.group_width(max(len(sp["src"]) for sp in specs), "mixed")
+    default_nsp = batch.MeshBatch.default_width(meshes, [len(sp["src"]) for sp in specs],
+                                                [sp["nf"] for sp in specs], "mixed")
     log(f"batch: library default shapes (nsp {default_nsp})")
     B = batch.MeshBatch(meshes, "s", "mixed", nsp=default_nsp, method="cocg")
     dt_def, _, b = timed(B)

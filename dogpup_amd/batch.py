@@ -30,7 +30,7 @@ class MeshBatch:
 
     ``meshes``: DMesh instances with optodes.  ``which``: 's', 'd' or 'sd' as
     in DMesh.solve_point_sources.  ``nsp``: columns per group (8, 16, 32 or
-    64; default: mesh.group_width of the widest mesh's columns).  ``tuning``: (tile_rows, unroll_a, unroll_g, graph_k), e.g. from
+    64; default: default_width, the width with the least padded work).  ``tuning``: (tile_rows, unroll_a, unroll_g, graph_k), e.g. from
     dogpup_amd.autotune.
     """
 
@@ -46,7 +46,7 @@ class MeshBatch:
         self.ncols = [sum(len((m.optode.s_bary if w == "s" else m.optode.d_bary)[0]) for w in which)
                       for m in self.meshes]
         self.nf = [m.optode.Nf for m in self.meshes]
-        self.nsp = int(nsp) if nsp else group_width(max(self.ncols), mode)
+        self.nsp = int(nsp) if nsp else self.default_width(self.meshes, self.ncols, self.nf, mode)
         self.ncb = [-(-c // self.nsp) for c in self.ncols]
         nn = [m.nn for m in self.meshes]
         self.row0 = np.concatenate([[0], np.cumsum(nn)]).astype(np.int32)
@@ -81,6 +81,23 @@ class MeshBatch:
         # block of each mesh in the batch vectors: nf*ncb groups of nn rows
         sizes = [f * b * n * self.nsp for f, b, n in zip(self.nf, self.ncb, nn)]
         self.voff = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+
+    @staticmethod
+    def default_width(meshes, ncols, nf, mode):
+        """The group width (8, 16 or 32; at most mesh.group_width of the
+        widest mesh) with the least padded work sum_i nf_i nn_i
+        ceil(c_i/w) w; ties go to the wider group.  On the 48-mesh uneven
+        batch (8-32 columns per mesh) this is 8, the width the autotuner
+        picks."""
+        cap = group_width(max(ncols), mode)
+        best = None
+        for w in (8, 16, 32):
+            if w > cap:
+                break
+            work = sum(f * m.nn * (-(-c // w)) * w for m, c, f in zip(meshes, ncols, nf))
+            if best is None or work <= best[0]:
+                best = (work, w)
+        return best[1]
 
     def __del__(self):
         try:

@@ -42,3 +42,16 @@ def test_uneven_batch_sizes_and_determinism():
         # sources lie on the top face, inside the mesh's footprint
         assert np.allclose(x["src"][:, 2], m.extent[2])
         assert (x["src"][:, :2] >= 0).all() and (x["src"][:, :2] <= np.array(m.extent[:2])).all()
+
+
+def test_batch_default_width_minimises_padding():
+    from types import SimpleNamespace as NS
+    from dogpup_amd.batch import MeshBatch
+    from dogpup_amd.mesh import group_width
+    big = [NS(nn=10000)]
+    assert MeshBatch.default_width(big, [32], [8], "mixed") == 32          # no padding at 32: widest wins
+    assert MeshBatch.default_width(big, [24], [8], "mixed") == 8           # 24 = 3 x 8
+    assert MeshBatch.default_width(big, [12], [8], "mixed") == 16          # 16 caps it (group_width)
+    two = [NS(nn=1000), NS(nn=50000)]
+    assert MeshBatch.default_width(two, [9, 31], [4, 4], "mixed") == 32    # the big mesh decides
+    assert group_width(20, "fp64") == 16 and group_width(20, "mixed") == 32 and group_width(5) == 8
