@@ -53,5 +53,6 @@ def test_batch_default_width_minimises_padding():
     assert MeshBatch.default_width(big, [24], [8], "mixed") == 8           # 24 = 3 x 8
     assert MeshBatch.default_width(big, [12], [8], "mixed") == 16          # 16 caps it (group_width)
     two = [NS(nn=1000), NS(nn=50000)]
-    assert MeshBatch.default_width(two, [9, 31], [4, 4], "mixed") == 32    # the big mesh decides
+    assert MeshBatch.default_width(two, [9, 31], [4, 4], "mixed") == 16    # 16+32 columns pad less than 32+32
+    assert MeshBatch.default_width(two, [17, 32], [4, 4], "mixed") == 32   # a tie goes to the wider group
     assert group_width(20, "fp64") == 16 and group_width(20, "mixed") == 32 and group_width(5) == 8
