@@ -54,5 +54,5 @@ def test_batch_default_width_minimises_padding():
     assert MeshBatch.default_width(big, [12], [8], "mixed") == 16          # 16 caps it (group_width)
     two = [NS(nn=1000), NS(nn=50000)]
     assert MeshBatch.default_width(two, [9, 31], [4, 4], "mixed") == 16    # 16+32 columns pad less than 32+32
-    assert MeshBatch.default_width(two, [17, 32], [4, 4], "mixed") == 32   # a tie goes to the wider group
+    assert MeshBatch.default_width(two, [16, 32], [4, 4], "mixed") == 16   # 8 and 16 tie: the wider group
     assert group_width(20, "fp64") == 16 and group_width(20, "mixed") == 32 and group_width(5) == 8
