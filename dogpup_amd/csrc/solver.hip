@@ -1829,7 +1829,11 @@ struct Chunk {
              S->t_gok[m].template as<unsigned char>(), m == 0 ? S->cA : (m == 1 ? S->cG : S->cGT)};
       if constexpr (NSP == 32) {
         if (S->halo_halves) {
-          k_spmv_halo2<MAT, EPI><<<vgrid(), 256, 0, st>>>(rp, m0, m1, x, y, w, coef, coef2, xacc, p1, p2, W, H, ga);
+          if (S->lds_pad > 0)
+            DP_HIP(hipFuncSetAttribute((const void*)k_spmv_halo2<MAT, EPI>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, S->lds_pad));
+          k_spmv_halo2<MAT, EPI><<<vgrid(), 256, S->lds_pad, st>>>(rp, m0, m1, x, y, w, coef, coef2, xacc, p1, p2, W,
+                                                                    H, ga);
           DP_LAUNCH_CHECK();
           return DOGPUP_OK;
         }
