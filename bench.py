@@ -290,6 +290,64 @@ def batch_leg(tol):
     return out
 
 
+def fd_leg(n_nodes=50, seed=0, delta=1e-5):
+    """Config 3 derivative check at full size: on the 269,001-node mesh, the
+    exact-derivative vector-Jacobian product g = J_TG^T ybar (kappa fixed,
+    dogpup_jacobian_vjp) against central differences of ybar . y(mua) at 50
+    seeded nodes under the optodes, and the reference's nodal-product J at the
+    same nodes.  A reduced optode set (4 sources x 4 detectors, 8
+    frequencies, 8 gates) keeps the 100 FP64 forward solves to about a
+    minute."""
+    import torch
+    from dogpup_amd import jacobian as jac, meshgen, mesh as dm
+    from dogpup_amd.transforms import td2fc
+    t = np.arange(0, 1001) * 20e-12
+    m, src, det, _, = meshgen.slab_80x80x40_h1(seed=0)
+    src = src[[0, 9, 18, 27]]
+    det = det[[4, 13, 22, 31]]
+    D = dm.DMesh(m.node, m.elem, m.nr, np.stack([m.mua, m.musp], 1))
+    D.add_optode(dm.DOptode(src, det, meshgen.all_pairs(4, 4), 1e3, meshgen.gaussian_tpsf(t, 250e-12), t, 8))
+    gates = meshgen.boxcar_gates(t, 8, 500e-12, 200e-12, start=500e-12)
+    gfc = td2fc(torch.as_tensor(gates), D.optode.fAxis, t, 1)
+    rng = np.random.default_rng(seed)
+    ybar = rng.normal(size=(16, 8))
+    kappa0 = D.kappa.copy()
+    mua0 = D.mua.copy()
+    t0 = time.perf_counter()
+    f = D.solve_point_sources("sd", mode="fp64", tol=1e-13)
+    g_exact = jac.vjp(D, f, gfc, torch.as_tensor(ybar, device="cuda"), exact=True).cpu().numpy()
+    g_nodal = jac.vjp(D, f, gfc, torch.as_tensor(ybar, device="cuda"), exact=False, dxyz=1.0).cpu().numpy()
+    del f
+    lo = np.array([15.0, 15.0, 40.0 - 15.0])
+    hi = np.array([65.0, 65.0, 40.0 - 2.0])
+    cand = np.flatnonzero(np.all((D.node >= lo) & (D.node <= hi), axis=1))
+    nodes = np.sort(rng.choice(cand, n_nodes, replace=False))
+
+    def objective(mu):
+        D.update_properties(np.stack([mu, D.musp], 1), kappa=kappa0)
+        fs = D.solve_point_sources("s", mode="fp64", tol=1e-13)
+        return float((jac.tg_data(D, fs, gfc).cpu().numpy() * ybar).sum())
+
+    fd = np.zeros(n_nodes)
+    for q, v in enumerate(nodes):
+        vals = []
+        for sgn in (1.0, -1.0):
+            mu = mua0.copy()
+            mu[v] += sgn * delta
+            vals.append(objective(mu))
+        fd[q] = (vals[0] - vals[1]) / (2.0 * delta)
+    D.update_properties(np.stack([mua0, D.musp], 1), kappa=kappa0)
+    dt = time.perf_counter() - t0
+    scale = np.abs(g_exact[nodes]).max()
+    err = np.abs(fd - g_exact[nodes])
+    rel_nodal = np.abs(g_nodal[nodes] - g_exact[nodes]) / np.maximum(np.abs(g_exact[nodes]), 1e-300)
+    return {"nodes": int(n_nodes), "mesh_nodes": int(D.nn), "delta": delta, "seconds": dt,
+            "max_abs_err_over_max_g": float(err.max() / scale),
+            "median_rel_err": float(np.median(err / np.maximum(np.abs(g_exact[nodes]), 1e-300))),
+            "nodal_product_vs_exact_median_rel": float(np.median(rel_nodal)),
+            "nodal_product_vs_exact_max_rel": float(rel_nodal.max())}
+
+
 def lm_leg(max_iter=10):
     """Config 5: weighted LM absorption reconstruction.  Data: FP64 solve on
     a 0.75 mm mesh of the slab with two inclusions, 32 x 32 pairs x 8 gates,
@@ -358,6 +416,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-mesh uneven batch")
     ap.add_argument("--lm", action="store_true", help="also run the LM reconstruction (config 5; minutes)")
+    ap.add_argument("--fd", action="store_true",
+                    help="also check the exact-derivative J_TG^T ybar against central differences at 50 nodes "
+                         "of the 269k mesh (config 3; about a minute)")
     ap.add_argument("--method", default="cocg", choices=["bicgstab", "cocg"],
                     help="Krylov method of the headline mixed solve")
     ap.add_argument("--no-alt", action="store_true", help="skip the mixed solve with the other method")
@@ -510,6 +571,10 @@ def main(argv=None):
         release(D)
         log("48-mesh uneven batch")
         extra["batch48"] = batch_leg(a.tol)
+    if a.fd and not a.small:
+        release(D)
+        log("FD check of the Jacobian at 50 nodes (config 3)")
+        extra["fd_check"] = fd_leg()
     if a.lm and not a.small:
         release(D)
         log("LM reconstruction (config 5)")
