@@ -585,6 +585,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   epi.finish(sh, part1, part2, it.wl);
 }
 
+// Chunks of a work item in the x-halo kernels.  Contiguous (ilv = 0): the
+// item's row tile, chunk after chunk.  Interleaved (ilv = 1): the XCD's
+// items of a group (tiles [x*per, x*per + n) of the group, all on XCD x) share
+// that tile range's chunks round-robin, item j taking chunks j, j + n,
+// j + 2n, ... of the range.  The blocks resident on an XCD then work on
+// neighbouring chunks at the same time, so a halo row fetched by one block is
+// in L2 for the others; each item still walks up to tile_rows / 64 chunks,
+// which keeps the prefetch pipeline.  Chunk s of the item is mesh chunk
+// cb + cs * s; nch chunks.
+__device__ inline void chunk_map(const Work& W, const Item& it, int ilv, int& cb, int& cs, int& nch) {
+  const int grows = W.grows[it.g], T = W.tile_rows, tch = T / kChunkRows;
+  const int total = (grows + kChunkRows - 1) / kChunkRows;
+  const int t = (it.row0 - W.grow0[it.g]) / T;
+  if (!ilv) {
+    cb = t * tch;
+    cs = 1;
+    nch = min(tch, total - cb);
+    return;
+  }
+  // the work list puts tile t on XCD t / per (per = ceil(nt / 8))
+  const int nt = (grows + T - 1) / T, per = (nt + 7) / 8;
+  const int x = t / per, j = t - x * per;
+  const int nreal = min(per, nt - x * per);
+  const int cend = min((x * per + nreal) * tch, total);
+  cb = x * per * tch + j;
+  cs = nreal;
+  nch = cb < cend ? (cend - cb + cs - 1) / cs : 0;
+}
+
 // Two-half x-halo form for 32-column groups (complex64).  A block walks its
 // tile chunk by chunk and each chunk in two halves of 16 columns: the
 // chunk's matrix slice is staged in LDS once and serves both halves, while
@@ -600,7 +629,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const int* __restrict__ rptr, const float* __restrict__ v0, const float* __restrict__ v1,
     const float* __restrict__ x, float* __restrict__ y, const float* __restrict__ w,
     const double* __restrict__ coef, const double* __restrict__ coef2, float* __restrict__ xacc,
-    double* __restrict__ part1, double* __restrict__ part2, Work W, Halo H, const int* __restrict__ grp_active) {
+    double* __restrict__ part1, double* __restrict__ part2, Work W, Halo H, const int* __restrict__ grp_active,
+    int ilv) {
   constexpr int NSP = 32, NH = 16, CPL = 2;
   constexpr int LPR = NSP / CPL;        // 16-byte pieces per node row
   constexpr int LPRH = NH / CPL;        // pieces (and lanes) per half row
@@ -612,14 +642,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   Item it;
   if (!work_item(W, it)) return;
   if (grp_active && !grp_active[it.gl]) return;
+  constexpr int SRS = kChunkRows + 1;   // row pointers per chunk in srp
   __shared__ double sh[2 * 4 * NSP];
-  __shared__ int srp[kHaloTileRows + 1];
-  __shared__ int shp[MAXCH + 1];
+  __shared__ int srp[MAXCH * SRS];
+  __shared__ int shp[2 * MAXCH];
   __shared__ unsigned char scol[kChunkCap];
   __shared__ float sval[2 * kChunkCap];
   __shared__ f4 xs[kHaloCap * LPRH];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int rsub = lane / LPRH, cl = lane % LPRH;
+  const int mr0 = W.grow0[it.g], grows = W.grows[it.g];
+  int cb, cs, nch;
+  chunk_map(W, it, ilv, cb, cs, nch);
+  // first row (mesh-relative) and row count of the item's chunk c
+  auto crow = [&](int c) __attribute__((always_inline)) { return (cb + cs * c) * kChunkRows; };
+  auto cnr = [&](int c) __attribute__((always_inline)) { return min(kChunkRows, grows - crow(c)); };
   float fk = 0.f;
   const float* gv = v0;
   if (MAT == MAT_A) fk = (float)W.gfsc[it.g];
@@ -672,17 +709,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // a mesh with a chunk whose halo exceeds the LDS: the gather row loop
     SpmvEpi<float, NSP, EPI> ep(y, w, xacc, coef, coef2, it.vb, (lane % LPR) * CPL,
                                 (long long)it.gl * NSP + (lane % LPR) * CPL);
-    spmv_rows<float, NSP, MAT, 4>(rptr, H.cptr, gv, v1, fk, x + (lane % LPR) * CPL * 2, it.vb, it.row0, it.row1,
-                                  srp, reinterpret_cast<int*>(xs), sval, ep);
+    for (int c = 0; c < nch; ++c)
+      spmv_rows<float, NSP, MAT, 4>(rptr, H.cptr, gv, v1, fk, x + (lane % LPR) * CPL * 2, it.vb, mr0 + crow(c),
+                                    mr0 + crow(c) + cnr(c), srp, reinterpret_cast<int*>(xs), sval, ep);
     ep.finish(sh, part1, part2, it.wl);
     return;
   }
   const f4* xrow = reinterpret_cast<const f4*>(x);   // node n, piece l at n * LPR + l
-  const int nrow = it.row1 - it.row0;
-  const int nch = (nrow + kChunkRows - 1) / kChunkRows;
-  const int ch0 = H.gchb[it.g] + (it.row0 - W.grow0[it.g]) / kChunkRows;
-  for (int q = tid; q <= nrow; q += 256) srp[q] = rptr[it.row0 + q];
-  for (int q = tid; q <= nch; q += 256) shp[q] = H.hptr[ch0 + q];
+  for (int q = tid; q < nch * SRS; q += 256) {
+    const int c = q / SRS, i = q - c * SRS;
+    srp[q] = rptr[mr0 + min(crow(c) + i, grows)];
+  }
+  for (int c = tid; c < nch; c += 256) {
+    const int gc = H.gchb[it.g] + cb + cs * c;
+    shp[2 * c] = H.hptr[gc];
+    shp[2 * c + 1] = H.hptr[gc + 1];
+  }
   __syncthreads();
 
   unsigned char ln[EPT];
@@ -693,12 +735,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   using XN = f4[PPT];
   using HC = int[PPT];
   auto load_hcol = [&](int c, HC& hc) __attribute__((always_inline)) {
-    const int hs = shp[c], npc = (shp[c + 1] - hs) * LPRH;
+    const int hs = shp[2 * c], npc = (shp[2 * c + 1] - hs) * LPRH;
 #pragma unroll
     for (int i = 0; i < PPT; ++i) hc[i] = H.hcol[hs + min(tid + 256 * i, npc - 1) / LPRH];
   };
   auto load_mat = [&](int c) __attribute__((always_inline)) {
-    const int es = srp[c * kChunkRows], ne = srp[min((c + 1) * kChunkRows, nrow)] - es;
+    const int es = srp[c * SRS], ne = srp[c * SRS + cnr(c)] - es;
 #pragma unroll
     for (int i = 0; i < EPT; ++i) {
       const int q = min(tid + 256 * i, ne - 1);
@@ -713,7 +755,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
   };
   auto store_mat = [&](int c) __attribute__((always_inline)) {
-    const int es = srp[c * kChunkRows], ne = srp[min((c + 1) * kChunkRows, nrow)] - es;
+    const int es = srp[c * SRS], ne = srp[c * SRS + cnr(c)] - es;
 #pragma unroll
     for (int i = 0; i < EPT; ++i) {
       const int q = min(tid + 256 * i, ne - 1);
@@ -723,7 +765,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
   };
   auto load_x = [&](int c, int h, const HC& hc, XN& xv) __attribute__((always_inline)) {
-    const int npc = (shp[c + 1] - shp[c]) * LPRH;
+    const int npc = (shp[2 * c + 1] - shp[2 * c]) * LPRH;
 #pragma unroll
     for (int i = 0; i < PPT; ++i) {
       const int q = min(tid + 256 * i, npc - 1);
@@ -731,28 +773,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
   };
   auto store_x = [&](int c, const XN& xv) __attribute__((always_inline)) {
-    const int npc = (shp[c + 1] - shp[c]) * LPRH;
+    const int npc = (shp[2 * c + 1] - shp[2 * c]) * LPRH;
 #pragma unroll
     for (int i = 0; i < PPT; ++i) xs[min(tid + 256 * i, npc - 1)] = xv[i];
   };
   auto load_ops = [&](int c, SpmvEpi<float, NSP, EPI>& ep) __attribute__((always_inline)) {
-    const int rb = c * kChunkRows;
+    const int rb = mr0 + crow(c), nr = cnr(c);
 #pragma unroll
     for (int st = 0; st < NST; ++st) {
-      const int lr = min(rb + st * RPB + wv * RPW + rsub, nrow - 1);
-      ep.load_ops(it.row0 + lr, eo1[st], eo2[st]);
+      const int lr = min(st * RPB + wv * RPW + rsub, nr - 1);
+      ep.load_ops(rb + lr, eo1[st], eo2[st]);
     }
   };
   auto compute = [&](int c, SpmvEpi<float, NSP, EPI>& ep) __attribute__((always_inline)) {
-    const int rb = c * kChunkRows, re = min(rb + kChunkRows, nrow), es = srp[rb];
+    const int rb = mr0 + crow(c), nr = cnr(c);
+    const int* cp = srp + c * SRS;
+    const int es = cp[0];
 #pragma unroll
     for (int st = 0; st < NST; ++st) {
-      const int lr = rb + st * RPB + wv * RPW + rsub;
-      if (lr >= re) continue;
+      const int lr = st * RPB + wv * RPW + rsub;
+      if (lr >= nr) continue;
       float ar[CPL], ai[CPL];
 #pragma unroll
       for (int e = 0; e < CPL; ++e) { ar[e] = 0.f; ai[e] = 0.f; }
-      const int s0 = srp[lr] - es, s1 = srp[lr + 1] - es;
+      const int s0 = cp[lr] - es, s1 = cp[lr + 1] - es;
       for (int p = s0; p < s1; p += 4) {
         int li[4];
         float xr[4][CPL], xi[4][CPL], mr[4], mi[4];
@@ -779,7 +823,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             ai[e] += mr[q] * xi[q][e] + mi[q] * xr[q][e];
           }
       }
-      ep.apply(it.row0 + lr, ar, ai, eo1[st], eo2[st]);
+      ep.apply(rb + lr, ar, ai, eo1[st], eo2[st]);
     }
   };
   // sub-step (c, 0): LDS holds chunk c's matrix and half-0 rows; hc = c's
@@ -1008,15 +1052,18 @@ __global__ __launch_bounds__(256) void k_init(const T* __restrict__ b, T* __rest
 
 // FP64 refinement residual: r64 = b - A x64 (complex128), with A x64 from
 // the staged row products of k_spmv (x64 == nullptr: x64 = 0, r64 = b).
-// When `write`, stores complex64 r, p (and rhat, if given) = r64 and x32 = 0
-// for columns whose outer status is active (zeros elsewhere);
+// b is read in the caller's node order when `order` is given (solver row i
+// is caller row order[i]), so the right-hand side needs no permuted copy.
+// When `write`, stores complex64 r (and p and rhat, if given) = r64 and
+// x32 = 0 for columns whose outer status is active (zeros elsewhere);
 // part1 = sum |r64|^2 (FP64), part2 = sum r32 r32.
 template <int NSP>
 __global__ __launch_bounds__(256) void k_residual64(
     const int* __restrict__ rptr, const int* __restrict__ cptr, const double* __restrict__ are,
-    const double* __restrict__ aim, const double* __restrict__ b, const double* __restrict__ x64,
-    float* __restrict__ r32, float* __restrict__ rhat32, float* __restrict__ p32, float* __restrict__ x32,
-    const int* __restrict__ ostatus, int write, double* __restrict__ part1, double* __restrict__ part2, Work W) {
+    const double* __restrict__ aim, const double* __restrict__ b, const int* __restrict__ order,
+    const double* __restrict__ x64, float* __restrict__ r32, float* __restrict__ rhat32,
+    float* __restrict__ p32, float* __restrict__ x32, const int* __restrict__ ostatus, int write,
+    double* __restrict__ part1, double* __restrict__ part2, Work W) {
   using G = Geo<double, NSP>;  // one complex128 per lane
   Item it;
   if (!work_item(W, it)) return;
@@ -1034,7 +1081,8 @@ __global__ __launch_bounds__(256) void k_residual64(
   d2.zero();
   auto epi = [&](int row, const double* yr, const double* yi) {
     const long long off = (vb + row) * NSP + c;
-    const double2 bv = *reinterpret_cast<const double2*>(b + 2 * off);
+    const long long boff = order ? (vb + order[row]) * NSP + c : off;
+    const double2 bv = *reinterpret_cast<const double2*>(b + 2 * boff);
     double rr = bv.x - yr[0], ri = bv.y - yi[0];
     if (!act) { rr = 0.0; ri = 0.0; }
     d1.re[0] += rr * rr + ri * ri;
@@ -1045,7 +1093,7 @@ __global__ __launch_bounds__(256) void k_residual64(
       const float2 v = make_float2(fr, fi);
       reinterpret_cast<float2*>(r32)[off] = v;
       if (rhat32) reinterpret_cast<float2*>(rhat32)[off] = v;
-      reinterpret_cast<float2*>(p32)[off] = v;
+      if (p32) reinterpret_cast<float2*>(p32)[off] = v;
       reinterpret_cast<float2*>(x32)[off] = make_float2(0.f, 0.f);
     }
   };
@@ -1063,21 +1111,27 @@ __global__ __launch_bounds__(256) void k_residual64(
   }
 }
 
-// x64 += x32 for columns that took part in the sweep
+// x64 += x32 for columns that took part in the sweep.  On the first sweep
+// (`first`) x64 holds no values yet: it is written as x32 for those columns
+// and 0 for the others, without being read.
 template <int NSP>
 __global__ __launch_bounds__(256) void k_add_correction(double* __restrict__ x64, const float* __restrict__ x32,
-                                                        const int* __restrict__ istatus, Work W) {
+                                                        const int* __restrict__ istatus, int first, Work W) {
   Item it;
   if (!work_item(W, it)) return;
   const int nr = it.row1 - it.row0;
   for (int i = threadIdx.x; i < nr * NSP; i += 256) {
     const int c = i % NSP;
-    if (istatus[(long long)it.gl * NSP + c] == ST_SKIP) continue;
     const long long o = (it.vb + it.row0 + i / NSP) * NSP + c;
-    const float2 d = reinterpret_cast<const float2*>(x32)[o];
-    double2 v = reinterpret_cast<double2*>(x64)[o];
-    v.x += d.x;
-    v.y += d.y;
+    const bool skip = istatus[(long long)it.gl * NSP + c] == ST_SKIP;
+    if (skip && !first) continue;
+    double2 v = make_double2(0.0, 0.0);
+    if (!first) v = reinterpret_cast<double2*>(x64)[o];
+    if (!skip) {
+      const float2 d = reinterpret_cast<const float2*>(x32)[o];
+      v.x += d.x;
+      v.y += d.y;
+    }
     reinterpret_cast<double2*>(x64)[o] = v;
   }
 }
@@ -1648,6 +1702,7 @@ struct dogpup_solver {
   std::vector<unsigned char> mesh_ok_tmp[3];  // per mesh (while building)
   bool use_halo = true;
   bool halo_halves = true;   // 32-wide groups: the two-half halo kernel
+  bool halo_ilv = true;      // halo kernels: chunks interleaved over an XCD's items
   bool cg_rfree = true;   // the refinement form's inner COCG updates u = G r directly
   std::vector<int> mesh_chb_tmp;   // first halo chunk of each mesh (while building)
   // values: a64re/a64im and g64 in reference order; the rest in solver order
@@ -1833,7 +1888,7 @@ struct Chunk {
             DP_HIP(hipFuncSetAttribute((const void*)k_spmv_halo2<MAT, EPI>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, S->lds_pad));
           k_spmv_halo2<MAT, EPI><<<vgrid(), 256, S->lds_pad, st>>>(rp, m0, m1, x, y, w, coef, coef2, xacc, p1, p2, W,
-                                                                    H, ga);
+                                                                    H, ga, S->halo_ilv ? 1 : 0);
           DP_LAUNCH_CHECK();
           return DOGPUP_OK;
         }
@@ -1993,7 +2048,7 @@ struct Chunk {
   }
 
   // COCG start from r (state initialised, x = 0): u = G r, rho = u.u,
-  // p = G^T u (p holds finite values, beta = 0).
+  // p = G^T u (stored: p's previous contents are not read).
   int cocg_start(Vecs<T> V) {
     const int* ga = bs.grp_active;
     if (rfree())
@@ -2003,7 +2058,7 @@ struct Chunk {
     DP_TRY(reset_count());
     k_cg_start<<<gcount, kScalThreads, 0, st>>>(bs, p1, rfree() ? p2 : nullptr, NSP, W);
     DP_LAUNCH_CHECK();
-    DP_TRY(spmvG<EPI_AXPBY>(true, V.u, V.p, nullptr, ga, bs.beta));
+    DP_TRY(spmvG<EPI_STORE>(true, V.u, V.p, nullptr, ga));    // p = G^T u
     return DOGPUP_OK;
   }
 
@@ -2197,8 +2252,8 @@ static int solve_c128(dogpup_solver* S, hipStream_t st, const double* b, double*
     S->stats[0] = std::max(S->stats[0], (double)its);
     // true residual for reporting
     k_residual64<NSP><<<C.vgrid(), 256, 0, st>>>(S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(),
-                                                 bc, V.x, nullptr, nullptr, nullptr, nullptr, nullptr, 0, C.p1,
-                                                 C.p2, C.W);
+                                                 bc, nullptr, V.x, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+                                                 C.p1, C.p2, C.W);
     DP_LAUNCH_CHECK();
     const int g1 = S->chunk_g[c + 1];
     const int nparts = (g1 < S->ngroups ? S->h_gwbeg[g1] : S->nparts) - C.W.pw0;
@@ -2238,25 +2293,33 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
     RState R = carve_rstate(S->rstate, gc * NSP);
     Vecs<float> V = carve<float>(S->ws, (size_t)C.nodes * NSP * 8, true, false);
     const size_t goff = (size_t)C.W.vb0 * NSP * 2;
-    DP_TRY(permute_block<NSP>(S, st, C.W, b + goff, S->bperm.as<double>(), 0));
+    // b is read in the caller's order by every residual pass; x64 (solver
+    // order) is first written by the first sweep's correction
+    const double* bc = b + goff;
     double* xc = S->xperm.as<double>();
-    const double* bc = S->bperm.as<double>();
-    DP_HIP(hipMemsetAsync(xc, 0, (size_t)C.nodes * NSP * 16, st));
+    const bool cocg = S->method == DOGPUP_METHOD_COCG;
     const double e64 = (double)C.nodes * NSP;  // complex elements of the chunk
     for (int sweep = 0;; ++sweep) {
       C.pbegin();
+      // COCG starts from r alone (p = G^T G r is written by its start), so
+      // only BiCGSTAB needs p and rhat
       k_residual64<NSP><<<C.vgrid(), 256, 0, st>>>(
-          S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), bc, sweep == 0 ? nullptr : xc, V.r,
-          S->method == DOGPUP_METHOD_COCG ? nullptr : V.rhat, V.p, V.x, sweep == 0 ? nullptr : R.status, 1, C.p1,
-          C.p2, C.W);
+          S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), bc, S->order.as<int>(),
+          sweep == 0 ? nullptr : xc, V.r, cocg ? nullptr : V.rhat, cocg ? nullptr : V.p, V.x,
+          sweep == 0 ? nullptr : R.status, 1, C.p1, C.p2, C.W);
       DP_LAUNCH_CHECK();
-      C.pend(3, e64 * (16 + 16 + 4 * 8) + (double)S->nnzA * 20 + 4.0 * (S->nn + 1));
+      C.pend(3, e64 * (16 + (sweep == 0 ? 0 : 16) + (cocg ? 2 : 4) * 8) + (sweep == 0 ? 0.0 : (double)S->nnzA * 20) +
+                    4.0 * (S->nn + 1));
       DP_TRY(C.reset_count());
       k_rf_check<<<gc, kScalThreads, 0, st>>>(R, C.bs, C.p1, NSP, C.W, sweep, tol, tol_in, maxit);
       DP_LAUNCH_CHECK();
       int active = 0;
       DP_TRY(C.wait_count(&active));
-      if (active == 0) break;
+      if (active == 0) {
+        // every column met the tolerance at x = 0 (zero right-hand sides)
+        if (sweep == 0) DP_HIP(hipMemsetAsync(xc, 0, (size_t)C.nodes * NSP * 16, st));
+        break;
+      }
       const double sweep_rows = (double)S->active_rows;
       // inner state from the residual pass: bnorm = |r64|, rho = r32.r32
       k_sc_init<<<gc, kScalThreads, 0, st>>>(C.bs, C.p1, C.p2, R.status, NSP, C.W);
@@ -2264,9 +2327,9 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
       int its = 0;
       DP_TRY(C.run_fused(V, nullptr, maxit, R.status, &its));
       C.pbegin();
-      k_add_correction<NSP><<<C.vgrid(), 256, 0, st>>>(xc, V.x, C.bs.status, C.W);
+      k_add_correction<NSP><<<C.vgrid(), 256, 0, st>>>(xc, V.x, C.bs.status, sweep == 0, C.W);
       DP_LAUNCH_CHECK();
-      C.pend(3, sweep_rows * NSP * (32 + 8));
+      C.pend(3, (sweep == 0 ? e64 * 16 : 0.0) + sweep_rows * NSP * (sweep == 0 ? 8 : 32 + 8));
       k_rf_accum<<<gc, kScalThreads, 0, st>>>(R, C.bs, NSP);
       DP_LAUNCH_CHECK();
       S->stats[0] = std::max(S->stats[0], (double)its);
@@ -2773,6 +2836,7 @@ static int create_batch(dogpup_solver_t* out, int nmesh, const int* h_row0, cons
   }
   if (const char* e = getenv("DOGPUP_HALO")) S->use_halo = atoi(e) != 0;
   if (const char* e = getenv("DOGPUP_HALO_HALVES")) S->halo_halves = atoi(e) != 0;
+  if (const char* e = getenv("DOGPUP_HALO_INTERLEAVE")) S->halo_ilv = atoi(e) != 0;
   if (const char* e = getenv("DOGPUP_CG_RFREE")) S->cg_rfree = atoi(e) != 0;
   DP_TRY(S->t_gfsc.ensure((size_t)g * 8));
   DP_TRY(S->fscale.ensure((size_t)fs * 8));
