@@ -407,6 +407,71 @@ constexpr int kHaloTileRows = 1024;
 // is copied with memcpy, which kept the staged rows in scratch memory)
 typedef float f4 __attribute__((ext_vector_type(4)));
 
+// a += m * v for the two complex values of a 16-byte x piece, v = (re0,
+// im0, re1, im1): two fused multiply-adds per real output.
+__device__ inline void cmac2(const f4& v, float mr, float mi, float* ar, float* ai) {
+  ar[0] = fmaf(mr, v[0], ar[0]);
+  ar[0] = fmaf(-mi, v[1], ar[0]);
+  ai[0] = fmaf(mr, v[1], ai[0]);
+  ai[0] = fmaf(mi, v[0], ai[0]);
+  ar[1] = fmaf(mr, v[2], ar[1]);
+  ar[1] = fmaf(-mi, v[3], ar[1]);
+  ai[1] = fmaf(mr, v[3], ai[1]);
+  ai[1] = fmaf(mi, v[2], ai[1]);
+}
+
+// Row product of the x-halo kernels: ar + i ai = sum over the staged entries
+// [s0, s1) of value * x, x read from the LDS halo at slot * LPX + cl.  Four
+// entries per step.  Rows of 15 entries (A) take whole steps and a masked
+// tail step: the whole steps read their slots at constant offsets from p;
+// the empty asm keeps each slot read a byte read (merged into one dword at a
+// row start that is not 4-aligned, the read measured 13 % slower on the
+// A pass).  Shorter rows of varying length (the FSAI factors, at most 8
+// entries) measured faster with every step masked.
+template <bool WHOLE_STEPS, int LPX>
+__device__ inline void halo_row(const unsigned char* __restrict__ scol, const float* __restrict__ sval,
+                                const f4* __restrict__ xs, int cl, int s0, int s1, float* ar, float* ai) {
+  int p = s0;
+  if (WHOLE_STEPS) {
+    for (; p + 4 <= s1; p += 4) {
+      int li[4];
+      float mr[4], mi[4];
+      f4 v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int iq = p + q;
+        asm volatile("" : "+v"(iq));
+        li[q] = scol[iq];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        v[q] = xs[li[q] * LPX + cl];
+        mr[q] = sval[2 * (p + q)];
+        mi[q] = sval[2 * (p + q) + 1];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) cmac2(v[q], mr[q], mi[q], ar, ai);
+    }
+  }
+  for (; p < s1; p += 4) {
+    int li[4];
+    float mr[4], mi[4];
+    f4 v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) li[q] = scol[min(p + q, s1 - 1)];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int pq = min(p + q, s1 - 1);
+      v[q] = xs[li[q] * LPX + cl];
+      mr[q] = sval[2 * pq];
+      mi[q] = sval[2 * pq + 1];
+      if (p + q >= s1) { mr[q] = 0.f; mi[q] = 0.f; }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cmac2(v[q], mr[q], mi[q], ar, ai);
+  }
+}
+
 struct Halo {
   const int* hptr;             // halo chunk -> first entry of its list
   const int* hcol;             // halo lists (columns, mesh-global numbering)
@@ -530,33 +595,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       float ar[CPL], ai[CPL];
 #pragma unroll
       for (int e = 0; e < CPL; ++e) { ar[e] = 0.f; ai[e] = 0.f; }
-      const int s0 = srp[lr] - es, s1 = srp[lr + 1] - es;
-      for (int p = s0; p < s1; p += 4) {
-        int li[4];
-        float xr[4][CPL], xi[4][CPL], mr[4], mi[4];
-        f4 v[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) li[q] = scol[min(p + q, s1 - 1)];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int pq = min(p + q, s1 - 1);
-          v[q] = xs[li[q] * LPR + cl];
-          mr[q] = sval[2 * pq];
-          mi[q] = sval[2 * pq + 1];
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          Pack<float>::load(reinterpret_cast<const float*>(&v[q]), xr[q], xi[q]);
-          if (p + q >= s1) { mr[q] = 0.f; mi[q] = 0.f; }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int e = 0; e < CPL; ++e) {
-            ar[e] += mr[q] * xr[q][e] - mi[q] * xi[q][e];
-            ai[e] += mr[q] * xi[q][e] + mi[q] * xr[q][e];
-          }
-      }
+      halo_row<MAT == MAT_A, LPR>(scol, sval, xs, cl, srp[lr] - es, srp[lr + 1] - es, ar, ai);
       epi.apply(it.row0 + lr, ar, ai, eo1[st], eo2[st]);
     }
   };
@@ -796,33 +835,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       float ar[CPL], ai[CPL];
 #pragma unroll
       for (int e = 0; e < CPL; ++e) { ar[e] = 0.f; ai[e] = 0.f; }
-      const int s0 = cp[lr] - es, s1 = cp[lr + 1] - es;
-      for (int p = s0; p < s1; p += 4) {
-        int li[4];
-        float xr[4][CPL], xi[4][CPL], mr[4], mi[4];
-        f4 v[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) li[q] = scol[min(p + q, s1 - 1)];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int pq = min(p + q, s1 - 1);
-          v[q] = xs[li[q] * LPRH + cl];
-          mr[q] = sval[2 * pq];
-          mi[q] = sval[2 * pq + 1];
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          Pack<float>::load(reinterpret_cast<const float*>(&v[q]), xr[q], xi[q]);
-          if (p + q >= s1) { mr[q] = 0.f; mi[q] = 0.f; }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int e = 0; e < CPL; ++e) {
-            ar[e] += mr[q] * xr[q][e] - mi[q] * xi[q][e];
-            ai[e] += mr[q] * xi[q][e] + mi[q] * xr[q][e];
-          }
-      }
+      halo_row<MAT == MAT_A, LPRH>(scol, sval, xs, cl, cp[lr] - es, cp[lr + 1] - es, ar, ai);
       ep.apply(rb + lr, ar, ai, eo1[st], eo2[st]);
     }
   };
