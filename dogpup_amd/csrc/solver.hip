@@ -1076,7 +1076,7 @@ __global__ __launch_bounds__(256) void k_residual64(
     const double* __restrict__ aim, const double* __restrict__ b, const int* __restrict__ order,
     const double* __restrict__ x64, float* __restrict__ r32, float* __restrict__ rhat32,
     float* __restrict__ p32, float* __restrict__ x32, const int* __restrict__ ostatus, int write,
-    double* __restrict__ part1, double* __restrict__ part2, Work W) {
+    double* __restrict__ part1, double* __restrict__ part2, Work W, int ilv) {
   using G = Geo<double, NSP>;  // one complex128 per lane
   Item it;
   if (!work_item(W, it)) return;
@@ -1111,8 +1111,16 @@ __global__ __launch_bounds__(256) void k_residual64(
     }
   };
   if (x64) {
-    spmv_rows<double, NSP, MAT_A, 8>(rptr, cptr, are, aim, W.gfsc[it.g], x64 + c * 2, vb, it.row0, it.row1, srp,
-                                     scol, sval, epi);
+    // the item's chunks as in the x-halo kernels (interleaved: neighbouring
+    // chunks on one XCD at a time share their x rows in L2)
+    int cb, cs, nch;
+    chunk_map(W, it, ilv, cb, cs, nch);
+    const int mr0 = W.grow0[it.g], mr1 = mr0 + W.grows[it.g];
+    for (int k = 0; k < nch; ++k) {
+      const int r0 = mr0 + (cb + cs * k) * kChunkRows;
+      spmv_rows<double, NSP, MAT_A, 8>(rptr, cptr, are, aim, W.gfsc[it.g], x64 + c * 2, vb, r0,
+                                       min(r0 + kChunkRows, mr1), srp, scol, sval, epi);
+    }
   } else {
     const double z[1] = {0.0};
     for (int row = it.row0 + wv * G::RPW + lane / G::LPR; row < it.row1; row += G::RPB) epi(row, z, z);
@@ -1715,7 +1723,7 @@ struct dogpup_solver {
   std::vector<unsigned char> mesh_ok_tmp[3];  // per mesh (while building)
   bool use_halo = true;
   bool halo_halves = true;   // 32-wide groups: the two-half halo kernel
-  bool halo_ilv = true;      // halo kernels: chunks interleaved over an XCD's items
+  bool halo_ilv = true;      // halo SpMVs, FP64 residual: chunks interleaved over an XCD's items
   bool cg_rfree = true;   // the refinement form's inner COCG updates u = G r directly
   std::vector<int> mesh_chb_tmp;   // first halo chunk of each mesh (while building)
   // values: a64re/a64im and g64 in reference order; the rest in solver order
@@ -2266,7 +2274,7 @@ static int solve_c128(dogpup_solver* S, hipStream_t st, const double* b, double*
     // true residual for reporting
     k_residual64<NSP><<<C.vgrid(), 256, 0, st>>>(S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(),
                                                  bc, nullptr, V.x, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
-                                                 C.p1, C.p2, C.W);
+                                                 C.p1, C.p2, C.W, S->halo_ilv ? 1 : 0);
     DP_LAUNCH_CHECK();
     const int g1 = S->chunk_g[c + 1];
     const int nparts = (g1 < S->ngroups ? S->h_gwbeg[g1] : S->nparts) - C.W.pw0;
@@ -2319,7 +2327,7 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
       k_residual64<NSP><<<C.vgrid(), 256, 0, st>>>(
           S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), bc, S->order.as<int>(),
           sweep == 0 ? nullptr : xc, V.r, cocg ? nullptr : V.rhat, cocg ? nullptr : V.p, V.x,
-          sweep == 0 ? nullptr : R.status, 1, C.p1, C.p2, C.W);
+          sweep == 0 ? nullptr : R.status, 1, C.p1, C.p2, C.W, S->halo_ilv ? 1 : 0);
       DP_LAUNCH_CHECK();
       C.pend(3, e64 * (16 + (sweep == 0 ? 0 : 16) + (cocg ? 2 : 4) * 8) + (sweep == 0 ? 0.0 : (double)S->nnzA * 20) +
                     4.0 * (S->nn + 1));
