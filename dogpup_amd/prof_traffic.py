@@ -66,9 +66,9 @@ def algorithmic(name: str, method: str, nodes: int, groups: int, nsp: int, nnz_a
         return (1, streams * V + MA) if mat == 0 else (0, streams * V + MG)
     table = {f"dogpup::k_cg_update<float, {nsp}>": (2, 3 * V), f"dogpup::k_update_xr<float, {nsp}>": (2, 8 * V),
              f"dogpup::k_update_p<float, {nsp}>": (2, 4 * V), f"dogpup::k_axpy_s<float, {nsp}>": (2, 3 * V),
-             # x64 and b in complex128 (2 V each), r, p, x out in complex64 (+ rhat for BiCGSTAB),
-             # A in FP64
-             f"dogpup::k_residual64<{nsp}>": (3, (7 if method == "cocg" else 8) * V + nnz_a * 20.0
+             # x64 and b in complex128 (2 V each), r and x out in complex64 (+ p and rhat for
+             # BiCGSTAB), A in FP64
+             f"dogpup::k_residual64<{nsp}>": (3, (6 if method == "cocg" else 8) * V + nnz_a * 20.0
                                               + 4.0 * (nodes + 1))}
     return table.get(name)
 
