@@ -481,149 +481,6 @@ struct Halo {
   const int* cptr;             // CSR columns (the gather path of meshes that do not fit)
 };
 
-template <int NSP, int MAT, int EPI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_spmv_halo(
-    const int* __restrict__ rptr, const float* __restrict__ v0, const float* __restrict__ v1,
-    const float* __restrict__ x, float* __restrict__ y, const float* __restrict__ w,
-    const double* __restrict__ coef, const double* __restrict__ coef2, float* __restrict__ xacc,
-    double* __restrict__ part1, double* __restrict__ part2, Work W, Halo H, const int* __restrict__ grp_active) {
-  using G = Geo<float, NSP>;
-  constexpr int CPL = G::CPL, LPR = G::LPR;
-  constexpr int PPT = (kHaloCap * LPR + 255) / 256;   // 16-byte halo pieces per thread
-  constexpr int EPT = kChunkCap / 256;                // chunk nonzeros per thread
-  constexpr int MAXCH = kHaloTileRows / kChunkRows;
-  Item it;
-  if (!work_item(W, it)) return;
-  if (grp_active && !grp_active[it.gl]) return;
-  __shared__ double sh[2 * 4 * NSP];
-  __shared__ int srp[kHaloTileRows + 1];
-  __shared__ int shp[MAXCH + 1];
-  __shared__ unsigned char scol[kChunkCap];
-  __shared__ float sval[2 * kChunkCap];
-  __shared__ f4 xs[kHaloCap * LPR];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int rsub = lane / LPR, cl = lane % LPR, c0 = cl * CPL;
-  float fk = 0.f;
-  const float* gv = v0;
-  if (MAT == MAT_A) fk = (float)W.gfsc[it.g];
-  else gv = v0 + 2 * W.ggv[it.g];
-  SpmvEpi<float, NSP, EPI> epi(y, w, xacc, coef, coef2, it.vb, c0, (long long)it.gl * NSP + c0);
-  if (!H.gok[it.g]) {
-    // a mesh with a chunk whose halo exceeds the LDS: the gather row loop,
-    // so that a mesh takes the same path alone and in a batch
-    spmv_rows<float, NSP, MAT, 4>(rptr, H.cptr, gv, v1, fk, x + c0 * 2, it.vb, it.row0, it.row1, srp,
-                                  reinterpret_cast<int*>(xs), sval, epi);
-    epi.finish(sh, part1, part2, it.wl);
-    return;
-  }
-  const f4* xrow = reinterpret_cast<const f4*>(x);   // node n, piece l at n * LPR + l
-  const int nrow = it.row1 - it.row0;
-  const int nch = (nrow + kChunkRows - 1) / kChunkRows;
-  const int ch0 = H.gchb[it.g] + (it.row0 - W.grow0[it.g]) / kChunkRows;
-  for (int q = tid; q <= nrow; q += 256) srp[q] = rptr[it.row0 + q];
-  for (int q = tid; q <= nch; q += 256) shp[q] = H.hptr[ch0 + q];
-  __syncthreads();
-
-  // staged state of the next chunk (registers; passed to the stages by
-  // reference, which keeps them out of the lambdas' captures and in VGPRs)
-  unsigned char ln[EPT];
-  float mn0[EPT], mn1[EPT];
-  f4 xn[PPT];
-  int hcA[PPT], hcB[PPT];
-  using XN = f4[PPT];
-  using LN = unsigned char[EPT];
-  using MN = float[EPT];
-  auto load_hcol = [&](int c, int* hc) __attribute__((always_inline)) {
-    const int hs = shp[c], npc = (shp[c + 1] - hs) * LPR;
-#pragma unroll
-    for (int i = 0; i < PPT; ++i) hc[i] = H.hcol[hs + min(tid + 256 * i, npc - 1) / LPR];
-  };
-  auto load_next = [&](int c, const int* hc, XN& xv, LN& lv, MN& m0v, MN& m1v) __attribute__((always_inline)) {
-    const int es = srp[c * kChunkRows], ne = srp[min((c + 1) * kChunkRows, nrow)] - es;
-#pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-      const int q = min(tid + 256 * i, ne - 1);
-      lv[i] = H.hloc[es + q];
-      if (MAT == MAT_A) {
-        m0v[i] = v0[es + q];
-        m1v[i] = v1[es + q];
-      } else {
-        m0v[i] = gv[2 * (long long)(es + q)];
-        m1v[i] = gv[2 * (long long)(es + q) + 1];
-      }
-    }
-    const int npc = (shp[c + 1] - shp[c]) * LPR;
-#pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-      const int q = min(tid + 256 * i, npc - 1);
-      xv[i] = xrow[(it.vb + hc[i]) * LPR + q % LPR];
-    }
-  };
-  auto store_next = [&](int c, const XN& xv, const LN& lv, const MN& m0v, const MN& m1v)
-                        __attribute__((always_inline)) {
-    const int es = srp[c * kChunkRows], ne = srp[min((c + 1) * kChunkRows, nrow)] - es;
-#pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-      const int q = min(tid + 256 * i, ne - 1);
-      scol[q] = lv[i];
-      sval[2 * q] = m0v[i];
-      sval[2 * q + 1] = MAT == MAT_A ? fk * m1v[i] : m1v[i];
-    }
-    const int npc = (shp[c + 1] - shp[c]) * LPR;
-#pragma unroll
-    for (int i = 0; i < PPT; ++i) xs[min(tid + 256 * i, npc - 1)] = xv[i];
-  };
-  // epilogue operands of this thread's rows of the current chunk, loaded
-  // before the next chunk's prefetch so that waiting for them leaves the
-  // prefetch in flight (the vector-memory counter retires in order)
-  constexpr int NST = kChunkRows / G::RPB;
-  f4 eo1[NST], eo2[NST];
-  auto load_ops = [&](int c) __attribute__((always_inline)) {
-    const int rb = c * kChunkRows;
-#pragma unroll
-    for (int st = 0; st < NST; ++st) {
-      const int lr = min(rb + st * G::RPB + wv * G::RPW + rsub, nrow - 1);
-      epi.load_ops(it.row0 + lr, eo1[st], eo2[st]);
-    }
-  };
-  auto compute = [&](int c) __attribute__((always_inline)) {
-    const int rb = c * kChunkRows, re = min(rb + kChunkRows, nrow), es = srp[rb];
-#pragma unroll
-    for (int st = 0; st < NST; ++st) {
-      const int lr = rb + st * G::RPB + wv * G::RPW + rsub;
-      if (lr >= re) continue;
-      float ar[CPL], ai[CPL];
-#pragma unroll
-      for (int e = 0; e < CPL; ++e) { ar[e] = 0.f; ai[e] = 0.f; }
-      halo_row<MAT == MAT_A, LPR>(scol, sval, xs, cl, srp[lr] - es, srp[lr + 1] - es, ar, ai);
-      epi.apply(it.row0 + lr, ar, ai, eo1[st], eo2[st]);
-    }
-  };
-  // one pipeline step: LDS holds chunk c; hc1 = halo list of c+1, hc2
-  // receives that of c+2
-  auto step = [&](int c, const int* hc1, int* hc2, XN& xv, LN& lv, MN& m0v, MN& m1v)
-                  __attribute__((always_inline)) {
-    const bool has1 = c + 1 < nch;
-    load_ops(c);
-    if (has1) load_next(c + 1, hc1, xv, lv, m0v, m1v);
-    if (c + 2 < nch) load_hcol(c + 2, hc2);
-    compute(c);
-    __syncthreads();
-    if (has1) store_next(c + 1, xv, lv, m0v, m1v);
-    __syncthreads();
-  };
-  load_hcol(0, hcA);
-  load_next(0, hcA, xn, ln, mn0, mn1);
-  if (nch > 1) load_hcol(1, hcB);
-  store_next(0, xn, ln, mn0, mn1);
-  __syncthreads();
-  for (int c = 0; c < nch; c += 2) {
-    step(c, hcB, hcA, xn, ln, mn0, mn1);
-    if (c + 1 < nch) step(c + 1, hcA, hcB, xn, ln, mn0, mn1);
-  }
-  epi.finish(sh, part1, part2, it.wl);
-}
-
 // Chunks of a work item in the x-halo kernels.  Contiguous (ilv = 0): the
 // item's row tile, chunk after chunk.  Interleaved (ilv = 1): the XCD's
 // items of a group (tiles [x*per, x*per + n) of the group, all on XCD x) share
@@ -652,6 +509,165 @@ __device__ inline void chunk_map(const Work& W, const Item& it, int ilv, int& cb
   cs = nreal;
   nch = cb < cend ? (cend - cb + cs - 1) / cs : 0;
 }
+
+template <int NSP, int MAT, int EPI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_spmv_halo(
+    const int* __restrict__ rptr, const float* __restrict__ v0, const float* __restrict__ v1,
+    const float* __restrict__ x, float* __restrict__ y, const float* __restrict__ w,
+    const double* __restrict__ coef, const double* __restrict__ coef2, float* __restrict__ xacc,
+    double* __restrict__ part1, double* __restrict__ part2, Work W, Halo H, const int* __restrict__ grp_active,
+    int ilv) {
+  using G = Geo<float, NSP>;
+  constexpr int CPL = G::CPL, LPR = G::LPR;
+  constexpr int PPT = (kHaloCap * LPR + 255) / 256;   // 16-byte halo pieces per thread
+  constexpr int EPT = kChunkCap / 256;                // chunk nonzeros per thread
+  constexpr int MAXCH = kHaloTileRows / kChunkRows;
+  Item it;
+  if (!work_item(W, it)) return;
+  if (grp_active && !grp_active[it.gl]) return;
+  constexpr int SRS = kChunkRows + 1;   // row pointers per chunk in srp
+  __shared__ double sh[2 * 4 * NSP];
+  __shared__ int srp[MAXCH * SRS];
+  __shared__ int shp[2 * MAXCH];
+  __shared__ unsigned char scol[kChunkCap];
+  __shared__ float sval[2 * kChunkCap];
+  __shared__ f4 xs[kHaloCap * LPR];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int rsub = lane / LPR, cl = lane % LPR, c0 = cl * CPL;
+  const int mr0 = W.grow0[it.g], grows = W.grows[it.g];
+  int cb, cs, nch;
+  chunk_map(W, it, ilv, cb, cs, nch);
+  // first row (mesh-relative) and row count of the item's chunk c
+  auto crow = [&](int c) __attribute__((always_inline)) { return (cb + cs * c) * kChunkRows; };
+  auto cnr = [&](int c) __attribute__((always_inline)) { return min(kChunkRows, grows - crow(c)); };
+  float fk = 0.f;
+  const float* gv = v0;
+  if (MAT == MAT_A) fk = (float)W.gfsc[it.g];
+  else gv = v0 + 2 * W.ggv[it.g];
+  SpmvEpi<float, NSP, EPI> epi(y, w, xacc, coef, coef2, it.vb, c0, (long long)it.gl * NSP + c0);
+  if (!H.gok[it.g]) {
+    // a mesh with a chunk whose halo exceeds the LDS: the gather row loop,
+    // so that a mesh takes the same path alone and in a batch
+    for (int c = 0; c < nch; ++c)
+      spmv_rows<float, NSP, MAT, 4>(rptr, H.cptr, gv, v1, fk, x + c0 * 2, it.vb, mr0 + crow(c),
+                                    mr0 + crow(c) + cnr(c), srp, reinterpret_cast<int*>(xs), sval, epi);
+    epi.finish(sh, part1, part2, it.wl);
+    return;
+  }
+  const f4* xrow = reinterpret_cast<const f4*>(x);   // node n, piece l at n * LPR + l
+  for (int q = tid; q < nch * SRS; q += 256) {
+    const int c = q / SRS, i = q - c * SRS;
+    srp[q] = rptr[mr0 + min(crow(c) + i, grows)];
+  }
+  for (int c = tid; c < nch; c += 256) {
+    const int gc = H.gchb[it.g] + cb + cs * c;
+    shp[2 * c] = H.hptr[gc];
+    shp[2 * c + 1] = H.hptr[gc + 1];
+  }
+  __syncthreads();
+
+  // staged state of the next chunk (registers; passed to the stages by
+  // reference, which keeps them out of the lambdas' captures and in VGPRs)
+  unsigned char ln[EPT];
+  float mn0[EPT], mn1[EPT];
+  f4 xn[PPT];
+  int hcA[PPT], hcB[PPT];
+  using XN = f4[PPT];
+  using LN = unsigned char[EPT];
+  using MN = float[EPT];
+  auto load_hcol = [&](int c, int* hc) __attribute__((always_inline)) {
+    const int hs = shp[2 * c], npc = (shp[2 * c + 1] - hs) * LPR;
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) hc[i] = H.hcol[hs + min(tid + 256 * i, npc - 1) / LPR];
+  };
+  auto load_next = [&](int c, const int* hc, XN& xv, LN& lv, MN& m0v, MN& m1v) __attribute__((always_inline)) {
+    const int es = srp[c * SRS], ne = srp[c * SRS + cnr(c)] - es;
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+      const int q = min(tid + 256 * i, ne - 1);
+      lv[i] = H.hloc[es + q];
+      if (MAT == MAT_A) {
+        m0v[i] = v0[es + q];
+        m1v[i] = v1[es + q];
+      } else {
+        m0v[i] = gv[2 * (long long)(es + q)];
+        m1v[i] = gv[2 * (long long)(es + q) + 1];
+      }
+    }
+    const int npc = (shp[2 * c + 1] - shp[2 * c]) * LPR;
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+      const int q = min(tid + 256 * i, npc - 1);
+      xv[i] = xrow[(it.vb + hc[i]) * LPR + q % LPR];
+    }
+  };
+  auto store_next = [&](int c, const XN& xv, const LN& lv, const MN& m0v, const MN& m1v)
+                        __attribute__((always_inline)) {
+    const int es = srp[c * SRS], ne = srp[c * SRS + cnr(c)] - es;
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+      const int q = min(tid + 256 * i, ne - 1);
+      scol[q] = lv[i];
+      sval[2 * q] = m0v[i];
+      sval[2 * q + 1] = MAT == MAT_A ? fk * m1v[i] : m1v[i];
+    }
+    const int npc = (shp[2 * c + 1] - shp[2 * c]) * LPR;
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) xs[min(tid + 256 * i, npc - 1)] = xv[i];
+  };
+  // epilogue operands of this thread's rows of the current chunk, loaded
+  // before the next chunk's prefetch so that waiting for them leaves the
+  // prefetch in flight (the vector-memory counter retires in order)
+  constexpr int NST = kChunkRows / G::RPB;
+  f4 eo1[NST], eo2[NST];
+  auto load_ops = [&](int c) __attribute__((always_inline)) {
+    const int rb = mr0 + crow(c), nr = cnr(c);
+#pragma unroll
+    for (int st = 0; st < NST; ++st) {
+      const int lr = min(st * G::RPB + wv * G::RPW + rsub, nr - 1);
+      epi.load_ops(rb + lr, eo1[st], eo2[st]);
+    }
+  };
+  auto compute = [&](int c) __attribute__((always_inline)) {
+    const int rb = mr0 + crow(c), nr = cnr(c);
+    const int* cp = srp + c * SRS;
+    const int es = cp[0];
+#pragma unroll
+    for (int st = 0; st < NST; ++st) {
+      const int lr = st * G::RPB + wv * G::RPW + rsub;
+      if (lr >= nr) continue;
+      float ar[CPL], ai[CPL];
+#pragma unroll
+      for (int e = 0; e < CPL; ++e) { ar[e] = 0.f; ai[e] = 0.f; }
+      halo_row<MAT == MAT_A, LPR>(scol, sval, xs, cl, cp[lr] - es, cp[lr + 1] - es, ar, ai);
+      epi.apply(rb + lr, ar, ai, eo1[st], eo2[st]);
+    }
+  };
+  // one pipeline step: LDS holds chunk c; hc1 = halo list of c+1, hc2
+  // receives that of c+2
+  auto step = [&](int c, const int* hc1, int* hc2, XN& xv, LN& lv, MN& m0v, MN& m1v)
+                  __attribute__((always_inline)) {
+    const bool has1 = c + 1 < nch;
+    load_ops(c);
+    if (has1) load_next(c + 1, hc1, xv, lv, m0v, m1v);
+    if (c + 2 < nch) load_hcol(c + 2, hc2);
+    compute(c);
+    __syncthreads();
+    if (has1) store_next(c + 1, xv, lv, m0v, m1v);
+    __syncthreads();
+  };
+  load_hcol(0, hcA);
+  load_next(0, hcA, xn, ln, mn0, mn1);
+  if (nch > 1) load_hcol(1, hcB);
+  store_next(0, xn, ln, mn0, mn1);
+  __syncthreads();
+  for (int c = 0; c < nch; c += 2) {
+    step(c, hcB, hcA, xn, ln, mn0, mn1);
+    if (c + 1 < nch) step(c + 1, hcA, hcB, xn, ln, mn0, mn1);
+  }
+  epi.finish(sh, part1, part2, it.wl);
+}
+
 
 // Two-half x-halo form for 32-column groups (complex64).  A block walks its
 // tile chunk by chunk and each chunk in two halves of 16 columns: the
@@ -1915,7 +1931,7 @@ struct Chunk {
         }
       }
       k_spmv_halo<NSP, MAT, EPI><<<vgrid(), 256, 0, st>>>(rp, m0, m1, x, y, w, coef, coef2, xacc, p1, p2, W, H,
-                                                          ga);
+                                                          ga, S->halo_ilv ? 1 : 0);
       DP_LAUNCH_CHECK();
     }
     return DOGPUP_OK;

@@ -32,10 +32,10 @@ def _mesh_and_ref(kind):
     return D, opt, ref
 
 
-def _solve(D, opt, tile=0):
+def _solve(D, opt, tile=0, nsp=None):
     from dogpup_amd import mesh as dm, _lib
     sid, _ = opt.s_bary
-    S = dm.FluenceSolver(D, len(sid), dm.MODES["mixed"])
+    S = dm.FluenceSolver(D, len(sid), dm.MODES["mixed"], nsp=nsp)
     S.set_method("cocg")
     if tile:
         _lib.call("dogpup_solver_set_tuning", S.h, tile, 0, 0, 0)
@@ -78,3 +78,24 @@ def test_tiles_above_the_halo_limit_take_the_gather_form():
     b, ib = _solve(D, opt, tile=2048)
     assert np.abs(ia - ib).max() <= 3
     assert rel(a, b) < 1e-8 and rel(b, ref) < 1e-8
+
+
+@pytest.mark.parametrize("nsp", [8, 16, 32])
+def test_interleaved_chunk_walk_matches_contiguous(nsp, monkeypatch):
+    """With 128-row tiles on the 1,989-node slab every XCD holds two items of
+    a group, which walk the range's four chunks interleaved (0, 2 and 1, 3)
+    or contiguously (0, 1 and 2, 3); the halo SpMVs and the FP64 residual
+    give the same solve either way (the per-item partial sums group the rows
+    differently, so iteration counts may differ by a step or two)."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    D, opt, ref = _mesh_and_ref("slab")
+    assert D.nn >= 9 * 128            # at least two tiles per XCD
+    out = {}
+    for ilv in ("1", "0"):
+        monkeypatch.setenv("DOGPUP_HALO_INTERLEAVE", ilv)
+        print(f"[case] chunk walk interleave {ilv}, nsp {nsp}, tile 128, {D.nn} nodes", flush=True)
+        out[ilv] = _solve(D, opt, tile=128, nsp=nsp)
+    assert np.abs(out["1"][1] - out["0"][1]).max() <= 3
+    assert rel(out["1"][0], out["0"][0]) < 1e-8
+    assert rel(out["1"][0], ref) < 1e-8 and rel(out["0"][0], ref) < 1e-8

@@ -370,3 +370,39 @@ def test_halo_spmv_matches_gather_spmv(setup, monkeypatch, nsp):
     assert np.abs(out["1"][1] - out["0"][1]).max() <= 3
     assert rel(out["1"][0], out["0"][0]) < 1e-8
     assert rel(out["1"][0], ref) < 1e-8 and rel(out["0"][0], ref) < 1e-8
+
+
+@pytest.mark.parametrize("method", ["cocg", "bicgstab"])
+def test_refinement_zero_rhs(setup, method):
+    """Refinement form with zero right-hand sides (the reference's relative
+    residual divides by their zero initial residual, dMesh.m:1285; here they
+    are answered x = 0 with DOGPUP_RHS_ZERO): a zero column comes back
+    exactly 0 while the others match the direct solve; an all-zero block
+    ends the solve before any sweep and still writes x = 0 over the caller's
+    buffer (filled with NaN beforehand)."""
+    from dogpup_amd import mesh as dm, _lib
+    from oracle import optode as oopt, solver as osol
+    D, A, om, opt = setup["D"], setup["A"], setup["om"], setup["opt"]
+    sid, sw = opt.s_bary
+    ref = osol.direct(A, oopt.source_matrix(om, sid, sw, opt.s_fpsf))
+    print(f"[case] zero right-hand sides, mixed {method}", flush=True)
+    S = dm.FluenceSolver(D, len(sid), dm.MODES["mixed"])
+    S.set_method(method)
+    nodes, w = D._bary_dev("s")
+    spec = torch.as_tensor(opt.s_fpsf, dtype=torch.complex128, device="cuda")
+    b = S.empty_block()
+    _lib.call("dogpup_point_sources", _lib.ptr(torch.view_as_real(b)), D.nn, S.nf, len(sid), S.nsp,
+              _lib.ptr(nodes), _lib.ptr(w), _lib.ptr(torch.view_as_real(spec)), _lib.stream_ptr())
+    assert S.ncb == 1                               # column s of every group is source s
+    b[:, :, 1] = 0
+    x = torch.full_like(b, float("nan"))
+    f = S.solve(b, tol=1e-10, x=x)
+    got = f.to_reference_layout().cpu().numpy()
+    assert (f.status[:, 1] == _lib.RHS_ZERO).all() and (f.status[:, [0, 2, 3]] == 0).all(), f.status
+    assert np.all(got[:, :, 1] == 0)
+    for s_ in (0, 2, 3):
+        assert rel(got[:, :, s_], ref[:, :, s_]) < 1e-7, s_
+    x0 = torch.full_like(b, float("nan"))
+    f0 = S.solve(torch.zeros_like(b), tol=1e-10, x=x0)
+    assert (f0.status == _lib.RHS_ZERO).all()
+    assert torch.count_nonzero(x0).item() == 0
