@@ -1276,12 +1276,28 @@ __device__ inline void block_sum_part(const double* __restrict__ part, const Wor
   __shared__ double sr[kScalThreads], si[kScalThreads];
   const int t = threadIdx.x, c = t % nsp, q = t / nsp, nq = kScalThreads / nsp;
   const int g = W.g0 + gl, wb = W.gwbeg[g] - W.pw0, wc = W.gwcnt[g];
-  double a = 0.0, b = 0.0;
-  for (int k = q; k < wc; k += nq) {
-    const long long i = 2 * ((long long)(wb + k) * nsp + c);
-    a += part[i];
-    b += part[i + 1];
+  // eight loads in flight per thread (one running sum would wait for each)
+  constexpr int U = 8;
+  double au[U], bu[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) { au[u] = 0.0; bu[u] = 0.0; }
+  int k = q;
+  for (; k + (U - 1) * nq < wc; k += U * nq) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long i = 2 * ((long long)(wb + k + u * nq) * nsp + c);
+      au[u] += part[i];
+      bu[u] += part[i + 1];
+    }
   }
+  for (; k < wc; k += nq) {
+    const long long i = 2 * ((long long)(wb + k) * nsp + c);
+    au[0] += part[i];
+    bu[0] += part[i + 1];
+  }
+  double a = 0.0, b = 0.0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) { a += au[u]; b += bu[u]; }
   sr[t] = a;
   si[t] = b;
   __syncthreads();
