@@ -274,7 +274,7 @@ class DMesh:
         ncols = nodes.shape[0]
         S = self.solver(ncols, mode, method)
         spec = torch.as_tensor(self.optode.s_fpsf, dtype=torch.complex128, device=DEV)
-        b = S.empty_block()
+        b = S.empty_block(zero=False)               # dogpup_point_sources clears it
         _lib.call("dogpup_point_sources", _lib.ptr(torch.view_as_real(b)), self.nn, S.nf, ncols, S.nsp,
                   _lib.ptr(nodes), _lib.ptr(w), _lib.ptr(torch.view_as_real(spec)), _lib.stream_ptr())
         if tol is None:
@@ -340,9 +340,11 @@ class FluenceSolver:
         except Exception:
             pass
 
-    def empty_block(self):
-        return torch.zeros((self.nf * self.ncb, self.mesh.nn, self.nsp), dtype=torch.complex128,
-                           device=DEV)
+    def empty_block(self, zero=True):
+        """A right-hand-side block in the solver layout, [NF*NCB][NN][NSP]
+        complex128 (zeros, or uninitialised with zero=False)."""
+        make = torch.zeros if zero else torch.empty
+        return make((self.nf * self.ncb, self.mesh.nn, self.nsp), dtype=torch.complex128, device=DEV)
 
     def ensure_matrix(self):
         m = self.mesh
