@@ -20,6 +20,9 @@ Also measured in the same run (reported beside the metric):
   * the adjoint Jacobian: forward+adjoint (4096 RHS) mixed solve plus the
     fused J-TG kernel (1024 pairs x 8 gates x 269,001 nodes), and the voxel
     basis (mesh2grid at 1 and 2 mm, J-TG over the 2 mm voxels),
+  * forward + adjoint + J-TG warm in the mixed form against the
+    reference-structure FP64 form, and the mixed J's error against the FP64
+    pipeline's J at sampled nodes,
   * the 48-mesh uneven batch (autotuned shapes, FP64 comparison),
   * the roofline of the dominant kernel from HIP events around every launch
     in the timed region,
@@ -288,6 +291,70 @@ def batch_leg(tol):
     gc.collect()
     torch.cuda.empty_cache()
     return out
+
+
+def target_leg(D, gfc, tol, method, nsample=4000, seed=1):
+    """Forward + adjoint solve of every source and detector at every
+    frequency (4096 RHS) and J-TG (1024 pairs x 8 gates x 269k nodes): the
+    mixed COCG form on a warm handle (FSAI setup included, as in the headline
+    step) against the reference-structure FP64 form (cold handle, the same
+    J-TG kernel).  The mixed J is compared with J of the FP64 field,
+    J64[v, g, m] = Re sum_k C_gk phi(v, k, s_m) psi(v, k, d_m) in complex128
+    at nsample seeded nodes, over every (pair, gate) column: the Frobenius
+    relative error, and per gate the largest error of a column relative to
+    that column's largest entry and to the gate's largest entry."""
+    import torch
+    from dogpup_amd.jacobian import jacobian_tg, tg_coefficients
+    log("forward + Jacobian: warm mixed run")
+    D.solve_point_sources("sd", mode="mixed", tol=tol, method=method)      # handle, graph capture
+    S = D.solver(sum(len(b[0]) for b in (D.optode.s_bary, D.optode.d_bary)), "mixed", method)
+    S._version = None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fm = D.solve_point_sources("sd", mode="mixed", tol=tol, method=method)
+    Jm, im = jacobian_tg(D, gfc, field=fm)
+    torch.cuda.synchronize()
+    t_mixed = time.perf_counter() - t0
+    rng = np.random.default_rng(seed)
+    vs = torch.as_tensor(np.sort(rng.choice(D.nn, nsample, replace=False)), device="cuda")
+    Jm_s = Jm[vs].double()
+    its_m = int(fm.iters.max())
+    del fm, Jm
+    release(D)
+    log("forward + Jacobian: reference-structure FP64 run")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fp = D.solve_point_sources("sd", mode="plain", tol=1e-12)
+    Jp, _ = jacobian_tg(D, gfc, field=fp)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    its_p = int(fp.iters.max())
+    opt = D.optode
+    ns = len(opt.s_bary[0])
+    R = fp.to_reference_layout()[vs]                                   # V x NF x NCOLS
+    ls = torch.as_tensor(opt.link[:, 0].astype(np.int64), device="cuda")
+    ld = torch.as_tensor(opt.link[:, 1].astype(np.int64) + ns, device="cuda")
+    coef = tg_coefficients(opt, gfc, D.dxyz)
+    J64 = torch.einsum("gk,vkm->vgm", coef, R[:, :, ls] * R[:, :, ld]).real
+    Jp_s = Jp[vs].double()
+    del Jp, fp, R
+
+    def errs(J):
+        d = (J - J64).abs()
+        col = (d.amax(0) / J64.abs().amax(0).clamp_min(1e-300)).amax(1)      # per gate
+        gate = d.amax(0).amax(1) / J64.abs().amax(0).amax(1)
+        return (float(torch.linalg.vector_norm(J - J64) / torch.linalg.vector_norm(J64)),
+                col.tolist(), gate.tolist())
+    fro_m, col_m, gate_m = errs(Jm_s)
+    fro_p, col_p, gate_p = errs(Jp_s)
+    return {"rhs": 2 * 2048, "mixed_s": t_mixed, "mixed_iters": its_m, "plain_s": t_plain, "plain_iters": its_p,
+            "speedup": t_plain / t_mixed, "jtg_kernel_s": im["kernel_s"], "sampled_nodes": nsample,
+            "mixed_J_rel_err_fro": fro_m, "mixed_J_col_rel_err_max_per_gate": col_m,
+            "mixed_J_err_over_gate_max_per_gate": gate_m,
+            "fp64_field_J_rel_err_fro": fro_p, "fp64_field_J_col_rel_err_max_per_gate": col_p,
+            "note": "mixed = COCG c64 + FP64 refinement to tol, warm handle with FSAI setup; plain = "
+                    "reference-structure FP64 BiCGSTAB to 1e-12; J64 = complex128 J-TG of the plain field at "
+                    "sampled nodes (fp64_field_J_* is the J-TG kernel's own error on that field)"}
 
 
 def fd_leg(n_nodes=50, seed=0, delta=1e-5):
@@ -570,6 +637,9 @@ def main(argv=None):
         D.clear_grid()
         del Jv, iv, info
         release(D)
+        if not a.no_plain and not a.small:
+            extra["forward_plus_jacobian"] = target_leg(D, gfc, a.tol, a.method)
+            release(D)
     if not a.no_batch and not a.small:
         release(D)
         log("48-mesh uneven batch")

@@ -29,8 +29,8 @@
 //   J_g[d][s] = sum_kk A_g[d][kk] B[kk][s],  kk = (k, re/im),
 //   A_g[d][(k,0)] = Re(C_gk psi_dk), A_g[d][(k,1)] = -Im(C_gk psi_dk),
 //   B[(k,0)][s] = Re phi_sk,         B[(k,1)][s] = Im phi_sk,
-// i.e. (32 x 2NF) x (2NF x 32) per gate, on v_mfma_f32_16x16x4f32 with FP32
-// accumulation.  Two wavefronts own one node (16 detector rows each): they
+// i.e. (32 x 2NF) x (2NF x 32) per gate, on v_mfma_f64_16x16x4f64 (FP64
+// operands and accumulation).  Two wavefronts own one node (16 detector rows each): they
 // stream the node's fluence two frequencies (one K step of 4) at a time
 // straight into the MFMA operand registers and keep all gates' accumulator
 // tiles in registers.  Links
@@ -321,21 +321,24 @@ __global__ __launch_bounds__(1024) void k_fc2td_mfma(const double* __restrict__ 
 // dcol[i]) or -1; coef: complex128 NG x NF.  Gates [gb, gb + NGB) in this
 // launch.  Two wavefronts per node (detector rows 0-15 and 16-31), so a
 // wavefront keeps NGB x 2 accumulator tiles; 4 wavefronts per block.
+// FP64 throughout (v_mfma_f64_16x16x4f64): for early gates and distant
+// pairs J is a near-total cancellation of the frequency terms, which FP32
+// products and sums leave at up to 1e-2 of such a column's size (measured on
+// the 269k-node, 64-frequency case); only the stored J is rounded to FP32.
 template <int NGB>
 __global__ __launch_bounds__(256) void k_jtg_mfma(const double* __restrict__ field,
                                                   const double* __restrict__ coef,
                                                   const int* __restrict__ tab, const float* __restrict__ scale,
                                                   float* __restrict__ out, int nn, int nf, int ng, int gb,
                                                   int ncols, int nsp, int nm, int nv) {
-  __shared__ float2 sc[NGB * 128];   // C_gk (zero for k >= nf), row stride 128
+  __shared__ double2 sc[NGB * 128];   // C_gk (zero for k >= nf), row stride 128
   __shared__ int sd[32], ss[32];
   __shared__ int sm[32 * 32];
   const int nfp4 = (nf + 3) & ~3;     // K loop: four frequencies per iteration
   for (int i = threadIdx.x; i < NGB * 128; i += 256) {
     const int g = i / 128, k = i % 128;
-    float2 c = make_float2(0.f, 0.f);
-    if (gb + g < ng && k < nf)
-      c = make_float2((float)coef[2 * ((gb + g) * nf + k)], (float)coef[2 * ((gb + g) * nf + k) + 1]);
+    double2 c = make_double2(0.0, 0.0);
+    if (gb + g < ng && k < nf) c = make_double2(coef[2 * ((gb + g) * nf + k)], coef[2 * ((gb + g) * nf + k) + 1]);
     sc[i] = c;
   }
   if (threadIdx.x < 32) {
@@ -362,12 +365,12 @@ __global__ __launch_bounds__(256) void k_jtg_mfma(const double* __restrict__ fie
   }
   const long long kstride = (long long)ncb * nn * nsp;   // complex elements per frequency
   const double2* F = reinterpret_cast<const double2*>(field);
-  typedef float f4 __attribute__((ext_vector_type(4)));
-  f4 acc[NGB][2];
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  d4 acc[NGB][2];
 #pragma unroll
   for (int g = 0; g < NGB; ++g)
 #pragma unroll
-    for (int u = 0; u < 2; ++u) acc[g][u] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int u = 0; u < 2; ++u) acc[g][u] = d4{0.0, 0.0, 0.0, 0.0};
   // two K steps (four frequencies) per iteration, each operand register
   // reloaded for the next iteration right after its use; frequencies past
   // nf read frequency nf-1 and meet C = 0
@@ -387,9 +390,8 @@ __global__ __launch_bounds__(256) void k_jtg_mfma(const double* __restrict__ fie
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int k = k0 + 2 * j + kf;
-      const float b0 = (float)ps0[j];
-      const float b1 = (float)ps1[j];
-      const float dr = (float)pd[j].x, di = (float)pd[j].y;
+      const double b0 = ps0[j], b1 = ps1[j];
+      const double dr = pd[j].x, di = pd[j].y;
       const long long ko = (long long)min(k + 4, nf - 1) * kstride;
       pd[j] = F[ko + offd];
       ps0[j] = Fs0[2 * ko];
@@ -397,14 +399,15 @@ __global__ __launch_bounds__(256) void k_jtg_mfma(const double* __restrict__ fie
       __builtin_amdgcn_sched_barrier(0);   // issue the reloads before this step's MFMAs
 #pragma unroll
       for (int g = 0; g < NGB; ++g) {
-        const float2 c = sc[g * 128 + k];
-        const float a = im ? -(c.x * di + c.y * dr) : (c.x * dr - c.y * di);
-        acc[g][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc[g][0], 0, 0, 0);
-        acc[g][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc[g][1], 0, 0, 0);
+        const double2 c = sc[g * 128 + k];
+        const double a = im ? -(c.x * di + c.y * dr) : (c.x * dr - c.y * di);
+        acc[g][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[g][0], 0, 0, 0);
+        acc[g][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[g][1], 0, 0, 0);
       }
     }
   }
-  // accumulator register r of lane l holds D[4*(l/16) + r][l%16] of a tile
+  // accumulator register r of lane l holds D[l/16 + 4*r][l%16] of a tile
+  // (the f64 16x16x4 layout; the f32 one holds D[4*(l/16) + r])
 #pragma unroll
   for (int g = 0; g < NGB; ++g) {
     if (gb + g < ng) {
@@ -414,8 +417,8 @@ __global__ __launch_bounds__(256) void k_jtg_mfma(const double* __restrict__ fie
       for (int u = 0; u < 2; ++u)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int m = sm[(t * 16 + 4 * kk + r) * 32 + u * 16 + i16];
-          if (m >= 0) o[m] = scg ? acc[g][u][r] * scg[m] : acc[g][u][r];
+          const int m = sm[(t * 16 + kk + 4 * r) * 32 + u * 16 + i16];
+          if (m >= 0) o[m] = (float)(scg ? acc[g][u][r] * (double)scg[m] : acc[g][u][r]);
         }
     }
   }
@@ -530,8 +533,8 @@ int dogpup_jacobian_tg(const double* d_field, const double* d_coef, const int* d
       int* d_tab = nullptr;
       DP_HIP(hipMalloc((void**)&d_tab, tab.size() * 4));
       DP_HIP(hipMemcpyAsync(d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, st));
-      for (int gb = 0; gb < ng; gb += 8) {
-        k_jtg_mfma<8><<<div_up(nv, 2), 256, 0, st>>>(d_field, d_coef, d_tab, d_scale, d_out, nn, nf, ng, gb, ncols,
+      for (int gb = 0; gb < ng; gb += 4) {
+        k_jtg_mfma<4><<<div_up(nv, 2), 256, 0, st>>>(d_field, d_coef, d_tab, d_scale, d_out, nn, nf, ng, gb, ncols,
                                                      nsp, nm, nv);
         DP_LAUNCH_CHECK();
       }

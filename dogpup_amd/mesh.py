@@ -216,13 +216,18 @@ class DMesh:
         grid, gridSize, dxyz, m2g, g2m, gridinmesh (and ``basis``, which the
         Jacobian and the LM driver use)."""
         b = grid.mesh2grid(self._dev["node"], self._dev["elem"], x, y, z, tol, locator=self.locator())
+        if self.basis is None:
+            self._nodal_dxyz = getattr(self, "dxyz", None)      # restored by clear_grid
         self.basis = b
         self.grid, self.gridSize, self.dxyz = b.grid, b.gridSize, b.dxyz
         self.m2g, self.g2m, self.gridinmesh = b.m2g, b.g2m, b.gridinmesh
         return self
 
     def clear_grid(self):
-        """Back to the nodal basis (m2g = g2m = I)."""
+        """Back to the nodal basis (m2g = g2m = I), with the dxyz the mesh had
+        before mesh2grid."""
+        if self.basis is not None:
+            self.dxyz = self._nodal_dxyz
         self.basis = None
         return self
 

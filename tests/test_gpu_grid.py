@@ -192,6 +192,10 @@ def test_jacobian_tg_on_voxel_basis():
     J, _ = jacobian_tg(D, torch.as_tensor(gfc), mode="fp64", tol=1e-12)
     assert J.shape == (D.basis.nv_in, gfc.shape[0], link.shape[0])
     assert rel(np.transpose(J.cpu().numpy(), (2, 1, 0)), ref) < 1e-5
+    # back on the nodal basis, the mesh's own dxyz (unset here) is restored
+    assert D.dxyz == r["dxyz"]
+    D.clear_grid()
+    assert D.basis is None and D.dxyz is None
 
 
 def test_lm_on_voxel_basis():
