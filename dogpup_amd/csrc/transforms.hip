@@ -36,8 +36,8 @@
 // tiles in registers.  Links
 // are then picked out of the 32x32 block through an (d, s) -> m table.
 // Scalar form (k_jacobian_tg, any link list): one block per node stages that
-// node's fluence of every frequency in LDS (complex64) and every lane
-// produces whole (m, g) rows.
+// node's fluence of every frequency in LDS (complex128) and every lane
+// produces whole (m, g) rows, summed in FP64.
 #include <algorithm>
 #include <vector>
 
@@ -188,30 +188,29 @@ __global__ __launch_bounds__(256) void k_jacobian_tg(
     const double* __restrict__ field, const double* __restrict__ coef, const int* __restrict__ link,
     const float* __restrict__ scale, float* __restrict__ out, int nn, int nf, int ng, int ncols,
     int nsp, int nm) {
-  extern __shared__ float2 sm[];  // [nf][ncols] then coefficients [ng][nf]
-  float2* sc = sm + nf * ncols;
+  extern __shared__ double2 sm[];  // [nf][ncols] then coefficients [ng][nf]
+  double2* sc = sm + nf * ncols;
   const int v = blockIdx.x;
   const int ncb = (ncols + nsp - 1) / nsp;
   for (int j = threadIdx.x; j < nf * ncols; j += blockDim.x) {
     const int k = j / ncols, c = j % ncols;
     const long long o = (((long long)k * ncb + c / nsp) * nn + v) * nsp + c % nsp;
-    sm[j] = make_float2((float)field[2 * o], (float)field[2 * o + 1]);
+    sm[j] = make_double2(field[2 * o], field[2 * o + 1]);
   }
-  for (int j = threadIdx.x; j < ng * nf; j += blockDim.x)
-    sc[j] = make_float2((float)coef[2 * j], (float)coef[2 * j + 1]);
+  for (int j = threadIdx.x; j < ng * nf; j += blockDim.x) sc[j] = make_double2(coef[2 * j], coef[2 * j + 1]);
   __syncthreads();
   for (int m = threadIdx.x; m < nm; m += blockDim.x) {
     const int s = link[2 * m], d = link[2 * m + 1];
-    float acc[MAXG];
+    double acc[MAXG];
 #pragma unroll
-    for (int g = 0; g < MAXG; ++g) acc[g] = 0.f;
+    for (int g = 0; g < MAXG; ++g) acc[g] = 0.0;
     for (int k = 0; k < nf; ++k) {
-      const float2 a = sm[k * ncols + s], b = sm[k * ncols + d];
-      const float pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
+      const double2 a = sm[k * ncols + s], b = sm[k * ncols + d];
+      const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
 #pragma unroll
       for (int g = 0; g < MAXG; ++g) {
         if (g < ng) {
-          const float2 c = sc[g * nf + k];
+          const double2 c = sc[g * nf + k];
           acc[g] += c.x * pr - c.y * pi;
         }
       }
@@ -219,7 +218,7 @@ __global__ __launch_bounds__(256) void k_jacobian_tg(
     float* o = out + (long long)v * ng * nm;
 #pragma unroll
     for (int g = 0; g < MAXG; ++g)
-      if (g < ng) o[(long long)g * nm + m] = scale ? acc[g] * scale[(long long)g * nm + m] : acc[g];
+      if (g < ng) o[(long long)g * nm + m] = (float)(scale ? acc[g] * (double)scale[(long long)g * nm + m] : acc[g]);
   }
 }
 
@@ -508,7 +507,8 @@ int dogpup_jacobian_tg(const double* d_field, const double* d_coef, const int* d
   hipStream_t st = as_stream(stream);
   // MFMA form when the links span at most 32 source and 32 detector columns
   // without repeated pairs (the table is built from a host copy of link)
-  if (nf <= 128 && !getenv("DOGPUP_JTG_SCALAR")) {
+  const char* force_scalar = getenv("DOGPUP_JTG_SCALAR");
+  if (nf <= 128 && !(force_scalar && atoi(force_scalar) != 0)) {
     std::vector<int> hl((size_t)2 * nm);
     DP_HIP(hipMemcpyAsync(hl.data(), d_link, hl.size() * 4, hipMemcpyDeviceToHost, st));
     DP_HIP(hipStreamSynchronize(st));
@@ -544,14 +544,20 @@ int dogpup_jacobian_tg(const double* d_field, const double* d_coef, const int* d
     }
   }
   DP_ARG(ng <= 16, "jacobian_tg supports at most 16 gates");
-  const size_t shm = (size_t)(nf * ncols + ng * nf) * sizeof(float2);
-  DP_ARG(shm <= 60 * 1024, "nf*(ncols+ng) too large for one node's LDS stage (%zu bytes)", shm);
-  if (ng <= 8)
+  // one node's fluence and the gate coefficients in LDS as complex128 (up to
+  // the 160 KiB of a CU)
+  const size_t shm = (size_t)(nf * ncols + ng * nf) * sizeof(double2);
+  DP_ARG(shm <= 160 * 1024, "nf*(ncols+ng) too large for one node's LDS stage (%zu bytes)", shm);
+  if (ng <= 8) {
+    DP_HIP(hipFuncSetAttribute((const void*)k_jacobian_tg<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     k_jacobian_tg<8><<<nv, 256, shm, st>>>(d_field, d_coef, d_link, d_scale, d_out, nn, nf, ng, ncols,
                                            nsp, nm);
-  else
+  } else {
+    DP_HIP(hipFuncSetAttribute((const void*)k_jacobian_tg<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)shm));
     k_jacobian_tg<16><<<nv, 256, shm, st>>>(d_field, d_coef, d_link, d_scale, d_out, nn, nf, ng, ncols,
                                             nsp, nm);
+  }
   DP_LAUNCH_CHECK();
   return DOGPUP_OK;
 }

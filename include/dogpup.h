@@ -270,8 +270,10 @@ int dogpup_meas(const double* d_phi, double* d_out, const int* d_dnodes, const d
  * complex128 solver layout (ncols columns per frequency, group width nsp);
  * link: NM x 2 int32 = (source column, detector column) in that field;
  * scale: NG x NM float (m fastest) or NULL; out: float, NM x NG x NV
- * column-major (node v = 0..nv-1).  Limits: ng <= 16,
- * (nf*ncols + ng*nf)*8 <= 60 KiB. */
+ * column-major (node v = 0..nv-1).  Sums in FP64 (only out is FP32).
+ * Limits: links within 32 source and 32 detector columns without repeated
+ * pairs take the MFMA form (nf <= 128, any ng); otherwise ng <= 16 and
+ * (nf*ncols + ng*nf)*16 <= 160 KiB. */
 int dogpup_jacobian_tg(const double* d_field, const double* d_coef, const int* d_link,
                        const float* d_scale, float* d_out, int nn, int nf, int ng, int ncols, int nsp,
                        int nm, int nv, void* stream);

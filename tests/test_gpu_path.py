@@ -262,9 +262,43 @@ def test_jacobian_tg_mfma_vs_scalar_and_fp64(monkeypatch, ng, nf):
                   _lib.stream_ptr())
         outs[flag] = out.cpu().numpy()
     monkeypatch.delenv("DOGPUP_JTG_SCALAR")
-    assert rel(outs["0"], ref) < 1e-5
-    assert rel(outs["1"], ref) < 1e-5
-    assert rel(outs["0"], outs["1"]) < 1e-5
+    assert rel(outs["0"], ref) < 1e-6
+    assert rel(outs["1"], ref) < 1e-6
+    assert rel(outs["0"], outs["1"]) < 1e-6
+
+
+def test_jacobian_tg_cancellation(monkeypatch):
+    """A J entry that is a near-total cancellation of its frequency terms, as
+    for early gates and distant pairs: unit-modulus fields, coefficients
+    C_gk = w_gk conj(phi_s phi_d) with terms w_gk of order 1 whose sum is
+    1e-6 (times the gate number + 1).  Both kernels sum in FP64, so J comes
+    out to the FP32 rounding of the stored value; FP32 sums would leave an
+    error of order 1e-5 here, ten times the value."""
+    _need_gpu()
+    import math
+    from dogpup_amd import _lib
+    rng = np.random.default_rng(11)
+    nn, nsp, ncols, nf, ng = 64, 8, 2, 64, 3
+    a, b = rng.uniform(0, 2 * np.pi, nf), rng.uniform(0, 2 * np.pi, nf)
+    field = np.zeros((nf, nn, nsp), complex)
+    field[:, :, 0] = np.exp(1j * a)[:, None]
+    field[:, :, 1] = np.exp(1j * b)[:, None]
+    P = np.exp(1j * (a + b))
+    w = rng.standard_normal((ng, nf))
+    w -= w.mean(axis=1, keepdims=True)
+    w += (1e-6 * (np.arange(ng) + 1) / nf)[:, None]
+    truth = np.array([math.fsum(w[g] * np.abs(P) ** 2) for g in range(ng)])
+    coef = w * np.conj(P)[None, :]
+    link = np.array([[0, 1]], np.int32)
+    print("[case] jtg cancellation", flush=True)
+    for flag in ("0", "1"):
+        monkeypatch.setenv("DOGPUP_JTG_SCALAR", flag)
+        out = torch.empty((nn, ng, 1), dtype=torch.float32, device="cuda")
+        _lib.call("dogpup_jacobian_tg", _lib.ptr(torch.view_as_real(dev(field, torch.complex128))),
+                  _lib.ptr(torch.view_as_real(dev(coef, torch.complex128))), _lib.ptr(dev(link, torch.int32)),
+                  None, _lib.ptr(out), nn, nf, ng, ncols, nsp, 1, nn, _lib.stream_ptr())
+        got = out.cpu().numpy()[:, :, 0].astype(np.float64)
+        assert np.all(np.abs(got - truth[None, :]) <= 1e-6 * np.abs(truth)[None, :]), (flag, got[0], truth)
 
 
 def test_autograd_vjp_vs_fd_and_reference_jacobian(setup):
