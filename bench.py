@@ -293,6 +293,23 @@ def batch_leg(tol):
     return out
 
 
+def td_error(td_sample, vs, phi64, opt, t_td):
+    """TD curves of the timed step (FP32-MFMA fc2td of the mixed field) at
+    the nodes vs against the FP64 fc2td (dogpup_fc2td) of the FP64 field:
+    Frobenius relative error, and per curve (node x source) the largest
+    error relative to that curve's peak (max and median)."""
+    import torch
+    from dogpup_amd import transforms as tr
+    tm = td_sample.double()                                                    # V x NT x NS
+    t64 = tr.fc2td(phi64.to_reference_layout()[vs], opt.fAxis, t_td, 1)        # V x NT x NS
+    d = (tm - t64).abs()
+    curve = d.amax(1) / t64.abs().amax(1)
+    return {"sampled_nodes": int(vs.numel()), "rel_err_fro": float(torch.linalg.vector_norm(tm - t64)
+                                                          / torch.linalg.vector_norm(t64)),
+            "curve_err_over_peak_max": float(curve.max()), "curve_err_over_peak_median": float(curve.median()),
+            "err_over_global_max": float(d.max() / t64.abs().max())}
+
+
 def target_leg(D, gfc, tol, method, nsample=4000, seed=1):
     """Forward + adjoint solve of every source and detector at every
     frequency (4096 RHS) and J-TG (1024 pairs x 8 gates x 269k nodes): the
@@ -542,6 +559,9 @@ def main(argv=None):
                       "GBps": bytes_td / ms_td / 1e6, "columns": ncol_td, "bins": int(t_td.size),
                       "note": "fc2td of the mixed-solve nodal fluence (32 sources, 269,001 nodes) to 256 bins "
                               "of 20 ps; part of every timed step"}
+    # the step's TD curves at seeded nodes, kept for the FP64 comparison
+    td_vs = torch.as_tensor(np.sort(np.random.default_rng(2).choice(D.nn, 4000, replace=False)), device="cuda")
+    td_sample = tr.td_field_to_reference(td_out["td"], phi_mixed.ncols)[td_vs].clone()
     del td_out["td"]
     release(D)
     if not a.no_alt:
@@ -590,6 +610,7 @@ def main(argv=None):
         extra["fp64_iters_per_freq"] = phi64.iters.max(axis=1).tolist()
         extra["mixed_vs_fp64_rel_err_max"] = float(err.max())
         extra["speedup_mixed_vs_fp64_fused"] = dt64 / dt_mixed
+        extra["td_mixed_mfma_vs_fp64"] = td_error(td_sample, td_vs, phi64, opt, t_td)
         del phi64
         release(D)
     if not a.no_plain:
