@@ -61,6 +61,7 @@ SIGNATURES = {
     "dogpup_chol_solve": [vp, vp, i32, i32, vp],
     "dogpup_solver_set_profile": [vp, i32],
     "dogpup_solver_set_method": [vp, i32],
+    "dogpup_solver_set_rhs_tol": [vp, vp, i32],
     "dogpup_solver_profile": [vp, vp, i32],
     "dogpup_locator_create": [P(vp), vp, i32, vp, i32, vp],
     "dogpup_locator_destroy": [vp],

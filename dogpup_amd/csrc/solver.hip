@@ -1580,7 +1580,8 @@ struct RState {
 // tolerance.  part1 = |r64|^2.
 __global__ __launch_bounds__(kScalThreads) void k_rf_check(RState o, BState in,
                                                            const double* __restrict__ part1, int nsp, Work W,
-                                                           int sweep, double tol, double tol_in, int maxit) {
+                                                           int sweep, double tol, double tol_in, int maxit,
+                                                           const double* __restrict__ ctol) {
   const int g = blockIdx.x, c = threadIdx.x;
   double n2 = 0, dummy = 0;
   block_sum_part(part1, W, g, nsp, n2, dummy);
@@ -1596,15 +1597,16 @@ __global__ __launch_bounds__(kScalThreads) void k_rf_check(RState o, BState in,
       o.relres[i] = (o.status[i] == ST_ACTIVE) ? 1.0 : 0.0;
     }
     if (o.status[i] == ST_ACTIVE) {
+      const double tl = (ctol && ctol[i] > 0.0) ? ctol[i] : tol;
       const double rel = sqrt(n2) / o.bnorm[i];
       o.relres[i] = rel;
-      if (rel < tol) o.status[i] = DOGPUP_RHS_CONVERGED;
+      if (rel < tl) o.status[i] = DOGPUP_RHS_CONVERGED;
       else if (!isfinite(rel)) o.status[i] = DOGPUP_RHS_BREAKDOWN;
       else if (o.iters[i] >= maxit) o.status[i] = DOGPUP_RHS_MAXIT;
       else if (rel > 0.5 * o.prev[i]) o.status[i] = DOGPUP_RHS_BREAKDOWN;  // stagnation
       else {
         o.prev[i] = rel;
-        double t = 0.5 * tol / rel;
+        double t = 0.5 * tl / rel;
         if (t < tol_in) t = tol_in;
         if (t > 0.5) t = 0.5;
         in.tol[i] = t;
@@ -1764,6 +1766,9 @@ struct dogpup_solver {
   int tile_rows = 512, lds_pad = 0;
   int unroll_a = 8, unroll_g = 4;  // gathers in flight per row (k_spmv: 4, 8 or 16)
   double ws_budget = 0;             // workspace budget, fixed when the handle is made
+  // per-RHS tolerances in group-slot order (g * nsp + c); 0 = the solve's tol
+  dogpup::Buf rhs_tol;
+  bool have_rhs_tol = false;
   bool have_matrix = false;
   double stats[4] = {0, 0, 0, 0};
   int* h_counter = nullptr;  // pinned: [0] active RHS, [1] active groups, [2] their rows
@@ -2231,6 +2236,17 @@ __global__ void k_fill(double* __restrict__ a, double v, int n) {
   if (i < n) a[i] = v;
 }
 
+// a[i] = t[i] where the per-RHS tolerance t is set (> 0), else v
+__global__ void k_fill_tol(double* __restrict__ a, const double* __restrict__ t, double v, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = (t && t[i] > 0.0) ? t[i] : v;
+}
+
+// Per-RHS tolerances of a chunk starting at group g0 (null: none set).
+static const double* rhs_tol_at(const dogpup_solver* S, int g0) {
+  return S->have_rhs_tol ? S->rhs_tol.as<double>() + (size_t)g0 * S->nsp : nullptr;
+}
+
 // Output index (freq*ncols + col within its mesh, meshes one after the
 // other) of column c of group g, or -1 for padding.
 static int out_index(const dogpup_solver* S, int g, int c) {
@@ -2298,7 +2314,7 @@ static int solve_c128(dogpup_solver* S, hipStream_t st, const double* b, double*
     DP_TRY(permute_block<NSP>(S, st, C.W, b + goff, S->bperm.as<double>(), 0));
     V.x = S->xperm.as<double>();
     const double* bc = S->bperm.as<double>();
-    k_fill<<<div_up(gc * NSP, 256), 256, 0, st>>>(C.bs.tol, tol, gc * NSP);
+    k_fill_tol<<<div_up(gc * NSP, 256), 256, 0, st>>>(C.bs.tol, rhs_tol_at(S, g0), tol, gc * NSP);
     DP_LAUNCH_CHECK();
     int its = 0;
     DP_TRY(C.run_fused(V, bc, maxit, nullptr, &its));
@@ -2364,7 +2380,8 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
       C.pend(3, e64 * (16 + (sweep == 0 ? 0 : 16) + (cocg ? 2 : 4) * 8) + (sweep == 0 ? 0.0 : (double)S->nnzA * 20) +
                     4.0 * (S->nn + 1));
       DP_TRY(C.reset_count());
-      k_rf_check<<<gc, kScalThreads, 0, st>>>(R, C.bs, C.p1, NSP, C.W, sweep, tol, tol_in, maxit);
+      k_rf_check<<<gc, kScalThreads, 0, st>>>(R, C.bs, C.p1, NSP, C.W, sweep, tol, tol_in, maxit,
+                                               rhs_tol_at(S, g0));
       DP_LAUNCH_CHECK();
       int active = 0;
       DP_TRY(C.wait_count(&active));
@@ -3042,6 +3059,27 @@ int dogpup_solver_set_method(dogpup_solver_t S, int method) {
          "the reference-structure form is BiCGSTAB only");
   if (method != S->method) S->clear_graphs();
   S->method = method;
+  return DOGPUP_OK;
+}
+
+int dogpup_solver_set_rhs_tol(dogpup_solver_t S, const double* h_tol, int n) {
+  DP_ARG(S != nullptr, "NULL solver");
+  if (h_tol == nullptr || n == 0) {
+    S->have_rhs_tol = false;
+    return DOGPUP_OK;
+  }
+  DP_ARG(S->mode != DOGPUP_MODE_C128_PLAIN, "the reference-structure form has one lockstep tolerance");
+  DP_ARG(n == S->nrhs_out, "h_tol must hold one tolerance per right-hand side (%d), got %d", S->nrhs_out, n);
+  std::vector<double> t((size_t)S->ngroups * S->nsp, 0.0);
+  for (int g = 0; g < S->ngroups; ++g)
+    for (int c = 0; c < S->nsp; ++c) {
+      const int o = out_index(S, g, c);
+      if (o < 0) continue;
+      DP_ARG(h_tol[o] >= 0.0 && std::isfinite(h_tol[o]), "tolerance %d is %g", o, h_tol[o]);
+      t[(size_t)g * S->nsp + c] = h_tol[o];
+    }
+  DP_TRY(upload(S->rhs_tol, t.data(), t.size() * 8));
+  S->have_rhs_tol = true;
   return DOGPUP_OK;
 }
 

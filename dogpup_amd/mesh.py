@@ -268,11 +268,14 @@ class DMesh:
         return s
 
     def solve_point_sources(self, which="s", mode="mixed", tol=None, maxit=1000, extra=None,
-                            method="bicgstab"):
+                            method="bicgstab", adj_tol=None):
         """Solve for the sources ('s'), detectors ('d') or both ('sd', forward
         and adjoint batched in one launch: columns 0..NS-1 sources,
         NS..NS+ND-1 detectors).  method: 'bicgstab' (the reference's) or
-        'cocg' (conjugate orthogonal CG, same solution)."""
+        'cocg' (conjugate orthogonal CG, same solution).  adj_tol: relative
+        residual of the detector (adjoint) columns when it differs from tol
+        (fused and mixed modes), as separate flu_solve / adj_flu_solve calls
+        with two tolerances would have."""
         parts = [self._bary_dev(w) for w in which]
         nodes = torch.cat([p[0] for p in parts])
         w = torch.cat([p[1] for p in parts])
@@ -284,7 +287,11 @@ class DMesh:
                   _lib.ptr(nodes), _lib.ptr(w), _lib.ptr(torch.view_as_real(spec)), _lib.stream_ptr())
         if tol is None:
             tol = 1e-12 if MODES[mode] != _lib.MODE_C64_REFINE else 1e-10
-        return S.solve(b, tol=tol, maxit=maxit)
+        col_tol = None
+        if adj_tol is not None and "d" in which:
+            col_tol = np.concatenate([np.full(len(p[0]), adj_tol if w_ == "d" else tol)
+                                      for w_, p in zip(which, parts)])
+        return S.solve(b, tol=tol, maxit=maxit, col_tol=col_tol)
 
     # -- forward / adjoint (dMesh.m:106-202) ------------------------------------
     def flu_solve(self, display=False, mode="mixed", tol=None):
@@ -329,6 +336,7 @@ class FluenceSolver:
                   _lib.ptr(d["cG"]), _lib.ptr(np.ascontiguousarray(order, np.int32)))
         self.h = h
         self._version = None
+        self._col_tol = False
         self.method = "bicgstab"
 
     def set_method(self, method: str):
@@ -361,8 +369,24 @@ class FluenceSolver:
         torch.cuda.current_stream().synchronize()
         self._version = m._matrix_version
 
-    def solve(self, b: torch.Tensor, tol=1e-10, maxit=1000, x: torch.Tensor | None = None):
+    def set_col_tol(self, col_tol):
+        """Per-column tolerances (NCOLS, the same at every frequency, or
+        NF x NCOLS); None restores the solve's single tol."""
+        if col_tol is None:
+            if self._col_tol:
+                _lib.call("dogpup_solver_set_rhs_tol", self.h, None, 0)
+                self._col_tol = False
+            return
+        t = np.asarray(col_tol, np.float64)
+        t = np.ascontiguousarray(np.broadcast_to(t, (self.nf, self.ncols)))
+        _lib.call("dogpup_solver_set_rhs_tol", self.h, _lib.ptr(t), t.size)
+        self._col_tol = True
+
+    def solve(self, b: torch.Tensor, tol=1e-10, maxit=1000, x: torch.Tensor | None = None, col_tol=None):
+        """col_tol: per-column tolerances (see set_col_tol); None = tol for
+        every column."""
         self.ensure_matrix()
+        self.set_col_tol(col_tol)
         if x is None:
             x = torch.empty_like(b)
         n = self.nf * self.ncols

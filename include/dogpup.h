@@ -191,6 +191,17 @@ int dogpup_solver_solve(dogpup_solver_t s, const double* d_b, double* d_x, doubl
  * (dMesh.m:1247-1348); the reference-structure mode is BiCGSTAB only. */
 int dogpup_solver_set_method(dogpup_solver_t s, int method);
 
+/* Per-RHS tolerances for the following solves: h_tol[o] is the relative
+ * residual at which right-hand side o (output order: per mesh,
+ * freq * ncols + col) counts as converged; 0 keeps the tol passed to
+ * dogpup_solver_solve.  n = 0 or h_tol = NULL clears them.  The fused and
+ * refinement forms only (the reference-structure form iterates in lockstep to
+ * one tolerance).  It lets a batched forward + adjoint solve run each half to
+ * the accuracy its result needs, as separate flu_solve and adj_flu_solve calls
+ * (dMesh.m:106-202) would with two tolerances; no counterpart in the
+ * reference, whose gpuBicstab_FSAIP hard-codes one (dMesh.m:1243). */
+int dogpup_solver_set_rhs_tol(dogpup_solver_t s, const double* h_tol, int n);
+
 /* Counters of the last solve: number of vector-kernel passes weighted by the
  * columns they moved, for roofline accounting.  h_out[0] = BiCGSTAB
  * iterations (max), [1] = sum over groups of active iterations (c64 or

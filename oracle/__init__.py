@@ -6,7 +6,7 @@ algorithms (``DOGPUP/dMesh.m``, ``dOptode.m``, ``td2fc.m``, ``fc2td.m``,
 CUDA kernels (``cuda_lib/mex_sources/*_CUDA.cu``) in NumPy/SciPy, in FP64, with
 the reference's own structure (quadrature loops, lockstep BiCGSTAB, per-row
 Gaussian elimination).  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker or
+``cpu_baseline`` leg of ``bench.py`` may import it, and only to check results or
 as the timed CPU baseline; the product (``dogpup_amd``) never imports it.
 
 Indices are 0-based throughout (the reference is 1-based in MATLAB and 0-based

@@ -1,6 +1,7 @@
 """GPU parity of the HIP path against the FP64 oracle (small meshes).
 
-Every call goes through libdogpup_hip.so's C ABI; the oracle is the checker.
+Every call goes through libdogpup_hip.so's C ABI; every result is compared
+with the oracle's.
 """
 import ctypes as C
 
@@ -440,3 +441,41 @@ def test_refinement_zero_rhs(setup, method):
     f0 = S.solve(torch.zeros_like(b), tol=1e-10, x=x0)
     assert (f0.status == _lib.RHS_ZERO).all()
     assert torch.count_nonzero(x0).item() == 0
+
+
+@pytest.mark.parametrize("mode,tol", [("mixed", 1e-10), ("fp64", 1e-12)])
+def test_adjoint_tolerance_per_column(setup, mode, tol):
+    """Forward and adjoint batched in one solve with a looser tolerance on the
+    detector columns (dogpup_solver_set_rhs_tol): each column meets its own
+    tolerance, the detector columns stop earlier, the source columns still
+    match the direct solve to the forward tolerance, and the setting is
+    cleared by the next solve without one.  The reference-structure form
+    (one lockstep tolerance) rejects it."""
+    from dogpup_amd import _lib
+    from oracle import optode as oopt, solver as osol
+    D, A, om, opt = setup["D"], setup["A"], setup["om"], setup["opt"]
+    sid, sw = opt.s_bary
+    did, dw = opt.d_bary
+    ns = len(sid)
+    ref_s = osol.direct(A, oopt.source_matrix(om, sid, sw, opt.s_fpsf))
+    ref_d = osol.direct(A, oopt.source_matrix(om, did, dw, opt.s_fpsf))
+    adj_tol = 1e-5
+    print(f"[case] per-column tolerance, {mode} cocg, tol {tol}, adjoint {adj_tol}", flush=True)
+    f = D.solve_point_sources("sd", mode=mode, tol=tol, method="cocg", adj_tol=adj_tol)
+    assert (f.status == 0).all(), f.status
+    assert (f.relres[:, :ns] < tol).all(), f.relres
+    assert (f.relres[:, ns:] < adj_tol).all(), f.relres
+    assert f.relres[:, ns:].max() > 10 * tol
+    assert f.iters[:, ns:].mean() < f.iters[:, :ns].mean(), f.iters
+    got = f.to_reference_layout().cpu().numpy()
+    bound_s = 1e-7 if mode == "mixed" else 1e-9
+    for k in range(setup["nf"]):
+        for s_ in range(ns):
+            assert rel(got[:, k, s_], ref_s[:, k, s_]) < bound_s, (k, s_)
+        for d_ in range(len(did)):
+            assert rel(got[:, k, ns + d_], ref_d[:, k, d_]) < 100 * adj_tol, (k, d_)
+    f2 = D.solve_point_sources("sd", mode=mode, tol=tol, method="cocg")
+    assert (f2.status == 0).all() and (f2.relres < tol).all(), f2.relres
+    S = D.solver(ns + len(did), "plain")
+    with pytest.raises(_lib.DogpupError):
+        S.set_col_tol(np.full(ns + len(did), 1e-6))
