@@ -6,9 +6,12 @@ frequencies 0-3.15 GHz in 50 MHz steps = 2048 complex right-hand sides.
 
 One step = FSAI factor setup for all 64 frequencies + the batched complex64
 FSAI-preconditioned COCG inside FP64 iterative refinement to a relative
-residual of 1e-8, from device-resident A and Q to converged fluence, + fc2td
+residual of 5e-8, from device-resident A and Q to converged fluence, + fc2td
 of the nodal fluence to 256 time bins.  value = 2048 * steps / timed seconds
-(RHS solves/s).
+(RHS solves/s).  5e-8 is the loosest residual whose fluence error against the
+FP64 solve (largest over the 2048 RHS, measured in the same run) stays a
+factor 3 below the 1e-6 target; the FP64 forms run the same step (solve +
+fc2td) to 1e-12.
 
 Also measured in the same run (reported beside the metric):
   * the device copy rate, the fc2td DFT alone and the mixed solve with
@@ -525,7 +528,7 @@ def main(argv=None):
     ap.add_argument("--adj-tol", type=float, default=1e-6,
                     help="relative FP64 residual of the adjoint columns in the forward + Jacobian leg "
                          "(the Jacobian's 1e-4 error target)")
-    ap.add_argument("--tol", type=float, default=1e-8,
+    ap.add_argument("--tol", type=float, default=5e-8,
                     help="relative FP64 residual of the mixed solve (chosen to meet 1e-6 fluence error)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
@@ -612,29 +615,34 @@ def main(argv=None):
     tfile = os.path.join(ROOT, "profiles", f"traffic_{a.method}_c64_slab269k_2048rhs.json")
     if not a.small and os.path.exists(tfile):
         with open(tfile) as fh:
-            tr = json.load(fh)["classes"].get(str(cls))
-        if tr:
+            tcl = json.load(fh)["classes"].get(str(cls))
+        if tcl:
             # traffic/algorithmic ratio of the class's full-activity launches (rocprofv3
             # PMC passes FETCH_SIZE and WRITE_SIZE, calibrated on a plain copy; made by
             # python -m dogpup_amd.prof_traffic), applied to this run's average launch
-            traffic = {"bytes_per_launch": tr["ratio"] * byts / max(nl, 1), "ratio": tr["ratio"],
+            traffic = {"bytes_per_launch": tcl["ratio"] * byts / max(nl, 1), "ratio": tcl["ratio"],
                        "source": os.path.relpath(tfile, ROOT)}
     if not a.no_fp64:
         log("FP64 fused solve")
-        dt64, phi64, _ = timed_solves(D, "s", "fp64", 1e-12, 1, 0)
+        dt64, phi64, _ = timed_solves(D, "s", "fp64", 1e-12, 1, 0, post=to_td)
+        td_out.pop("td", None)
         err = rel_err_per_rhs(x_mixed.reshape(x_mixed.shape[0], -1, x_mixed.shape[2]).permute(0, 2, 1).reshape(-1, D.nn),
                               phi64.data.permute(0, 2, 1).reshape(-1, D.nn))
         extra["fp64_fused_s_per_solve"] = dt64
         extra["fp64_fused_max_iters"] = int(phi64.iters.max())
         extra["fp64_iters_per_freq"] = phi64.iters.max(axis=1).tolist()
         extra["mixed_vs_fp64_rel_err_max"] = float(err.max())
+        extra["mixed_vs_fp64_rel_err_median"] = float(err.median())
+        extra["fluence_error_target"] = 1e-6
+        extra["meets_fluence_error_target"] = bool(float(err.max()) <= 1e-6)
         extra["speedup_mixed_vs_fp64_fused"] = dt64 / dt_mixed
         extra["td_mixed_mfma_vs_fp64"] = td_error(td_sample, td_vs, phi64, opt, t_td)
         del phi64
         release(D)
     if not a.no_plain:
         log("reference-structure FP64 solve")
-        dtp, php, _ = timed_solves(D, "s", "plain", 1e-12, 1, 0)
+        dtp, php, _ = timed_solves(D, "s", "plain", 1e-12, 1, 0, post=to_td)
+        td_out.pop("td", None)
         extra["fp64_plain_s_per_solve"] = dtp
         extra["fp64_plain_iters"] = int(php.iters.max())
         extra["speedup_mixed_vs_fp64_plain"] = dtp / dt_mixed
