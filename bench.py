@@ -654,7 +654,7 @@ def main(argv=None):
         gfc = boxcar_gates_fc(opt, t, 8, 500e-12, 200e-12)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        J, info = jacobian_tg(D, gfc, mode="mixed", tol=a.tol, method=a.method)
+        J, info = jacobian_tg(D, gfc, mode="mixed", tol=a.tol, method=a.method, adj_tol=a.adj_tol)
         torch.cuda.synchronize()
         extra["jacobian_s"] = time.perf_counter() - t0
         extra["jacobian_solve_s"] = info["solve_s"]

@@ -53,12 +53,13 @@ def tg_coefficients(optode, gate_fc, dxyz):
 
 
 def jacobian_tg(mesh, gate_fc, mode="mixed", tol=1e-10, dxyz=None, scale=None, field=None,
-                method="bicgstab"):
+                method="bicgstab", adj_tol=None):
     """J_TG as float32 [NV][NG][NM] (= the reference's NM x NG x NV,
     column-major); NV = nodes, or in-mesh voxels when ``mesh.basis`` is set.
     ``field``: a forward+adjoint FluenceField from
     ``mesh.solve_point_sources('sd')`` to reuse; ``scale``: optional NM x NG
-    factor (e.g. 1/data for the Rytov normalisation)."""
+    factor (e.g. 1/data for the Rytov normalisation); ``adj_tol``: tolerance
+    of the adjoint (detector) columns when it differs from ``tol``."""
     opt = mesh.optode
     ns, nd = len(opt.s_bary[0]), len(opt.d_bary[0])
     basis = getattr(mesh, "basis", None)
@@ -68,7 +69,7 @@ def jacobian_tg(mesh, gate_fc, mode="mixed", tol=1e-10, dxyz=None, scale=None, f
             raise ValueError("voxel side dxyz is needed (set mesh.dxyz or pass dxyz)")
     t0 = time.perf_counter()
     if field is None:
-        field = mesh.solve_point_sources("sd", mode=mode, tol=tol, method=method)
+        field = mesh.solve_point_sources("sd", mode=mode, tol=tol, method=method, adj_tol=adj_tol)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     vfield = basis.field_to_voxels(field) if basis is not None else field

@@ -479,3 +479,33 @@ def test_adjoint_tolerance_per_column(setup, mode, tol):
     S = D.solver(ns + len(did), "plain")
     with pytest.raises(_lib.DogpupError):
         S.set_col_tol(np.full(ns + len(did), 1e-6))
+
+
+def test_assembly_true_robin_faces_switch():
+    """The Robin-face switch (reference_faces=False: the true boundary faces
+    instead of the reference's selection, dMesh.m:1129-1140): on a jittered
+    slab, 1'K1 = sum_e vol_e * mean(mua) + R * (box surface) exactly, 1'M1 =
+    box volume, and the reference selection adds R times the area of its
+    extra faces."""
+    from dogpup_amd import mesh as dm, topology
+    m = small_slab(seed=4, inclusions=[((10.0, 10.0, 5.0), 3.0, 0.03)])
+    props = np.stack([m.mua, m.musp], 1)
+    print("[case] true Robin faces", flush=True)
+    Dt = dm.DMesh(m.node, m.elem, m.nr, props, reference_faces=False)
+    Dr = dm.DMesh(m.node, m.elem, m.nr, props, reference_faces=True)
+    ex, ey, ez = m.extent
+    e = m.elem
+    p = m.node[e]
+    vol = np.abs(np.einsum("ij,ij->i", np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), p[:, 3] - p[:, 0])) / 6
+    int_mua = float((vol * m.mua[e].mean(axis=1)).sum())
+    R = topology.robin_factor(m.nr)
+    surface = 2 * (ex * ey + ey * ez + ex * ez)
+    Kt = float(Dt.K.sum())
+    assert abs(Kt - (int_mua + R * surface)) <= 1e-12 * Kt, (Kt, int_mua + R * surface)
+    assert abs(float(Dt.M.sum()) - ex * ey * ez) <= 1e-12 * ex * ey * ez
+    bf = Dr.bface
+    pf = m.node[bf]
+    a = 0.5 * np.linalg.norm(np.cross(pf[:, 1] - pf[:, 0], pf[:, 2] - pf[:, 0]), axis=1).sum()
+    assert a > surface                                   # the reference selection has extra faces
+    Kr = float(Dr.K.sum())
+    assert abs(Kr - (int_mua + R * a)) <= 1e-12 * Kr, (Kr, int_mua + R * a)

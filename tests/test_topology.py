@@ -120,3 +120,20 @@ def test_locate_and_snap():
     assert (eid >= 0).all()
     rec = np.einsum("pa,pad->pd", w, m.node[m.elem[eid]])
     assert np.allclose(rec, pos - np.array([0, 0, 1.0]))
+
+
+def test_true_face_switch_covers_the_box_surface():
+    """robin_faces without the reference's selection returns the true
+    boundary faces: their area is the box surface; the reference's selection
+    (dMesh.m:1129-1140) is a superset with extra interior faces."""
+    m = small_slab(seed=6)
+    face, area = topology.true_faces(m.node, m.elem)
+    bnd = np.zeros(m.node.shape[0], np.int32)
+    bnd[np.unique(face.ravel())] = 1
+    tf = topology.robin_faces(m.elem, bnd, False, face)
+    assert np.array_equal(tf, face)
+    ex, ey, ez = m.extent
+    assert abs(area.sum() - 2 * (ex * ey + ey * ez + ex * ez)) < 1e-9
+    rf = topology.robin_faces(m.elem, bnd, True, face)
+    key = lambda f: set(map(tuple, np.sort(f, axis=1).tolist()))
+    assert key(face) < key(rf)
