@@ -42,3 +42,6 @@ def test_argument_errors_without_gpu():
     with pytest.raises(_lib.DogpupError) as e:
         _lib.call("dogpup_solver_create", C.byref(h), 10, 2, 4, 12, 0, 1, None, None, 1, None, None, None)
     assert e.value.code == _lib.ERR_ARG and "group width" in str(e.value)
+    with pytest.raises(_lib.DogpupError) as e:
+        _lib.call("dogpup_solver_set_rhs_tol", None, None, 0)
+    assert e.value.code == _lib.ERR_ARG and "NULL solver" in str(e.value)
