@@ -6,6 +6,10 @@
  *  - Every entry point returns an int status: DOGPUP_OK (0) or a negative
  *    DOGPUP_ERR_*; dogpup_last_error() gives the message of the last failure on
  *    the calling thread.  No C++ exception crosses this boundary.
+ *  - Threads: the library keeps no mutable global state.  A handle (solver,
+ *    locator) is not locked: use one handle per thread, or serialise the
+ *    calls on a handle; different handles may run on different threads at
+ *    once (graph capture is thread-local to the capturing thread).
  *  - CSR arrays are int32 and 0-based, as the reference's sparse_csr produces
  *    (DOGPUP/dMesh.m:1388-1414).
  *  - "dev" pointers are device (HBM) pointers, "host" pointers are host memory.
