@@ -455,8 +455,10 @@ def lm_leg(max_iter=10):
     """Config 5: weighted LM absorption reconstruction.  Data: FP64 solve on
     a 0.75 mm mesh of the slab with two inclusions, 32 x 32 pairs x 8 gates,
     Poisson noise at 1e6 peak counts; inversion on the 1 mm (269k-node) mesh
-    from the homogeneous background, in the mixed COCG form and in the
-    reference-structure FP64 form."""
+    from the homogeneous background, in the mixed COCG form (1e-8: with the
+    forward columns at 5e-8 and the adjoint columns at 1e-6 the reconstructed
+    mua moved 1.3 % from the FP64 form's, against 0.05 % at 1e-8) and in the
+    reference-structure FP64 form (1e-12)."""
     import gc
     import torch
     from dogpup_amd import lm, meshgen, mesh as dm
@@ -480,7 +482,7 @@ def lm_leg(max_iter=10):
     truth = m.mua.copy()
     xs = np.arange(8.0, 73.0, 8.0)
     pos_s = np.stack(np.meshgrid(xs, xs, [30.0, 25.0, 20.0], indexing="ij"), -1).reshape(-1, 3)
-    for mode, method, tol in (("mixed", "cocg", 1e-8), ("plain", "bicgstab", 1e-12)):
+    for mode, method, tol, atol in (("mixed", "cocg", 1e-8, None), ("plain", "bicgstab", 1e-12, None)):
         D = dm.DMesh(m.node, m.elem, m.nr, np.stack([np.full(m.node.shape[0], 0.01), m.musp], 1))
         D.add_optode(dm.DOptode(src, det, link, 1e3, tpsf, t, 64))
         D.dxyz = 1.0
@@ -489,7 +491,7 @@ def lm_leg(max_iter=10):
         # lam = (LM damping, spot reshaping); 1e-1 keeps reshape_J's inverse of the
         # 8192-row J J^T stable under the Poisson noise (1e-2 diverged)
         r = lm.wghtd_lm_recon(D, data, gates, pos_s, (1e2, 1e-1), mode=mode, tol=tol, max_iter=max_iter,
-                              method=method)
+                              method=method, adj_tol=atol)
         wall = time.time() - t2
         inside = [np.linalg.norm(m.node - np.asarray(c)[None, :], axis=1) <= rad for c, rad, _ in inc]
         out[mode] = {"s": wall, "iterations": r.count, "exit": r.exit, "errPerc": r.errPerc,

@@ -70,10 +70,10 @@ class LMResult:
     t_algebra: list = field(default_factory=list)
 
 
-def _forward(mesh, gate_fc, mode, tol, method):
+def _forward(mesh, gate_fc, mode, tol, method, adj_tol=None):
     """TG data (NM x NG, float64) and the Rytov-normalised TG Jacobian as
     rows x NV float32, rows ordered m + NM*g."""
-    f = mesh.solve_point_sources("sd", mode=mode, tol=tol, method=method)
+    f = mesh.solve_point_sources("sd", mode=mode, tol=tol, method=method, adj_tol=adj_tol)
     y = tg_data(mesh, f, gate_fc)                                  # NM x NG
     J, _ = jacobian_tg(mesh, gate_fc, field=f, scale=(1.0 / y).to(torch.float32))
     nv, ng, nm = J.shape
@@ -82,11 +82,13 @@ def _forward(mesh, gate_fc, mode, tol, method):
 
 
 def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed", tol=1e-8,
-                   max_iter=10, spot_fwhm=1.5, verbose=False, method="bicgstab"):
+                   max_iter=10, spot_fwhm=1.5, verbose=False, method="bicgstab", adj_tol=None):
     """wghtd_LM_recon(mesh, data_m, gates, pos_s, lam): returns LMResult with
     the reconstructed nodal mua.  data_m: NM x NG measured TG data; gates:
     NG x NT gating functions on the optode's time axis.  mode/method: the
-    solver form of the forward+adjoint solves ('plain' is BiCGSTAB only)."""
+    solver form of the forward+adjoint solves ('plain' is BiCGSTAB only);
+    adj_tol: tolerance of the adjoint solves, which feed only the Jacobian
+    (default tol; the fused and mixed forms)."""
     opt = mesh.optode
     data_m = np.asarray(data_m, float)
     incl = id_thresh(torch.as_tensor(data_m), 0.8).cpu().numpy().ravel(order="F")   # :25-26
@@ -122,7 +124,7 @@ def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed
     while True:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        yc, J = _forward(mesh, gate_fc, mode, tol, method)                         # :58-61
+        yc, J = _forward(mesh, gate_fc, mode, tol, method, adj_tol)                # :58-61
         torch.cuda.synchronize()
         res.t_forward.append(time.perf_counter() - t0)
         yc = yc.cpu().numpy()
