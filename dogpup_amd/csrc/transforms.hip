@@ -15,14 +15,15 @@
 // with C_gk = -2 Nt w_k conj(Gate_gk) dxyz^3 / s_fpsf(k) supplied by the host.
 // The 282 GB complex Jacobian of the reference is never materialised.
 //
-// fc2td of a whole solver-layout field (k_fc2td_mfma): every column e of the
-// block [k][e] (e = (group, node, column) in the solver order) is a real
-// (NT x 2NF) x (2NF x 1) product, y[t][e] = sum_kk Wr[t][kk] X[kk][e] with
-// Wr[t][(k,0)] = 2 w_k cos(f_k t), Wr[t][(k,1)] = -2 w_k sin(f_k t) and
+// fc2td of a whole solver-layout field (k_fc2td_mfma64): every column e of
+// the block [k][e] (e = (group, node, column) in the solver order) is a real
+// (NT x 2NF) x (2NF x 1) product, y[t][e] = sum_kk Wd[t][kk] X[kk][e] with
+// Wd[t][(k,0)] = 2 w_k cos(f_k t), Wd[t][(k,1)] = -2 w_k sin(f_k t) and
 // X[(k,0)][e] = Re F_k(e), X[(k,1)][e] = Im F_k(e): one GEMM over all columns
-// on v_mfma_f32_16x16x4f32 with FP32 accumulation.  A persistent 1024-thread
-// block per CU holds Wr (up to 256 time bins per pass) in LDS; a wavefront
-// keeps 64 bins of 16 columns in accumulator registers.
+// on v_mfma_f64_16x16x4f64, FP64 products and accumulation, FP32 output.  A
+// persistent 1024-thread block per CU holds Wd (128 time bins per pass up to
+// 64 frequencies, 64 beyond) in LDS; a wavefront keeps all of a pass's bins
+// of 16 columns in accumulator registers.
 //
 // MFMA form (k_jtg_mfma, up to 32 distinct source and 32 distinct detector
 // columns): per node and gate the link block is a real GEMM
@@ -241,47 +242,51 @@ __global__ __launch_bounds__(256) void k_jacobian_tg(
 __device__ inline double2 primed(double2 v) { return make_double2(v.x + 0.0, v.y + 0.0); }
 __device__ inline double primed(double v) { return v + 0.0; }
 
-// Wr (float, ntp x kks, zero padded) for time bins t[t0 .. t0 + ntp) of fc2td.
-__global__ void k_build_wr(float* __restrict__ Wr, const double* __restrict__ f, const double* __restrict__ t,
+// Wd (double, ntp x kks, zero padded) for time bins t[t0 .. t0 + ntp) of
+// fc2td: Wd[r][2k] = 2 w_k cos(f_k t), Wd[r][2k+1] = -2 w_k sin(f_k t).
+__global__ void k_build_wd(double* __restrict__ Wd, const double* __restrict__ f, const double* __restrict__ t,
                            int t0, int nt, int nf, int ntp, int kks) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ntp * kks) return;
   const int r = i / kks, kk = i % kks, k = kk >> 1;
-  float v = 0.f;
+  double v = 0.0;
   if (t0 + r < nt && k < nf) {
     double sn, cs;
     sincos(f[k] * t[t0 + r], &sn, &cs);
     const double w = (k == 0) ? 1.0 : 2.0;   // 2 w_k with the halved DC term
-    v = (float)((kk & 1) ? -w * sn : w * cs);
+    v = (kk & 1) ? -w * sn : w * cs;
   }
-  Wr[i] = v;
+  Wd[i] = v;
 }
 
-// y[t0 + t][e] = sum_kk Wr[t][kk] X[kk][e] for t < min(th * NTT * 16, nt - t0).
+// y[t0 + t][e] = sum_kk Wd[t][kk] X[kk][e] for t < min(th * NTT * 16, nt - t0).
 // F: complex128 [nf][E]; out: float [nt][E].  A wavefront computes NTT time
-// tiles (16 bins each) of 16 columns; th (1, 2 or 4) wavefronts share a
-// column tile, one per block of NTT*16 time bins (so a wavefront needs few
-// enough registers for four K steps of X loads in flight).
+// tiles (16 bins each) of a 16-column tile on v_mfma_f64_16x16x4f64 (FP64
+// products and sums; only the stored value is rounded to FP32); th (1 or 2)
+// wavefronts share a column tile, one per block of NTT*16 bins, with Wd for
+// the pass's th*NTT*16 bins in LDS.  FP64 because a time bin of a curve far
+// from its source is a near-total cancellation of the 2nf terms: in FP32
+// such curves came out wrong by up to their own peak.
 template <int NTT>
-__global__ __launch_bounds__(1024) void k_fc2td_mfma(const double* __restrict__ F, const float* __restrict__ Wr,
-                                                     float* __restrict__ out, long long E, int nf, int kks,
-                                                     int t0, int nt, int th) {
-  extern __shared__ float sW[];   // th*NTT*16 rows x kks
-  for (int i = threadIdx.x; i < th * NTT * 16 * kks; i += 1024) sW[i] = Wr[i];
+__global__ __launch_bounds__(1024) void k_fc2td_mfma64(const double* __restrict__ F, const double* __restrict__ Wd,
+                                                       float* __restrict__ out, long long E, int nf, int kks,
+                                                       int t0, int nt, int th) {
+  extern __shared__ double sWd[];   // th*NTT*16 rows x kks
+  for (int i = threadIdx.x; i < th * NTT * 16 * kks; i += 1024) sWd[i] = Wd[i];
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int half = wv % th, slot = wv / th, slots = 16 / th;
   const int i16 = lane & 15, kk = lane >> 4, kf = kk >> 1, im = kk & 1;
-  const int nfp8 = (nf + 7) & ~7;   // Wr holds zero columns up to 2*nfp8
+  const int nfp8 = (nf + 7) & ~7;   // Wd holds zero columns up to 2*nfp8
   const long long ntiles = (E + 15) / 16;
-  const float* wbase = sW + (half * NTT * 16 + i16) * kks + kk;
-  typedef float f4 __attribute__((ext_vector_type(4)));
+  const double* wbase = sWd + (half * NTT * 16 + i16) * kks + kk;
+  typedef double d4 __attribute__((ext_vector_type(4)));
   for (long long mt = (long long)blockIdx.x * slots + slot; mt < ntiles; mt += (long long)gridDim.x * slots) {
     const long long e = mt * 16 + i16;
     const bool oke = e < E;
-    f4 acc[NTT];
+    d4 acc[NTT];
 #pragma unroll
-    for (int i = 0; i < NTT; ++i) acc[i] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < NTT; ++i) acc[i] = d4{0.0, 0.0, 0.0, 0.0};
     // four K steps (eight frequencies) per iteration; each X register is
     // reloaded for the next iteration right after its use
     const long long ec = oke ? e : E - 1;
@@ -292,26 +297,66 @@ __global__ __launch_bounds__(1024) void k_fc2td_mfma(const double* __restrict__ 
     for (int k0 = 0; k0 < nfp8; k0 += 8) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const float b = (float)xq[j];
+        const double b = xq[j];
         xq[j] = Xe[2 * (long long)min(k0 + 8 + 2 * j + kf, nf - 1) * E];
         __builtin_amdgcn_sched_barrier(0);   // issue the reload before this step's MFMAs
-        const float* wrow = wbase + 2 * (k0 + 2 * j);
+        const double* wrow = wbase + 2 * (k0 + 2 * j);
 #pragma unroll
         for (int i = 0; i < NTT; ++i)
-          acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(wrow[i * 16 * kks], b, acc[i], 0, 0, 0);
+          acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(wrow[i * 16 * kks], b, acc[i], 0, 0, 0);
       }
     }
-    // accumulator register r of lane l holds D[4*(l/16) + r][l%16]
+    // accumulator register r of lane l holds D[l/16 + 4*r][l%16] (the f64
+    // 16x16x4 layout)
     if (oke) {
 #pragma unroll
       for (int i = 0; i < NTT; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int tt = t0 + (half * NTT + i) * 16 + 4 * kk + r;
-          if (tt < nt) out[(long long)tt * E + e] = acc[i][r];
+          const int tt = t0 + (half * NTT + i) * 16 + kk + 4 * r;
+          if (tt < nt) out[(long long)tt * E + e] = (float)acc[i][r];
         }
     }
   }
+}
+
+// Shape of the fc2td passes for nf frequencies: th wavefronts of 4 time
+// tiles (64 bins) per column tile, so th*64 bins per pass (2 while the
+// pass's Wd fits the LDS, else 1), and the LDS row stride kks (doubles; 2
+// mod 32, so the 16 rows a wavefront reads start in different banks).
+constexpr int kFc2tdNtt = 4;
+struct Fc2tdShape {
+  int th, kks;
+  int bins() const { return th * kFc2tdNtt * 16; }
+  size_t shm() const { return (size_t)bins() * kks * sizeof(double); }
+};
+static Fc2tdShape fc2td_shape(int nf) {
+  const int nfp8 = (nf + 7) & ~7;
+  Fc2tdShape s;
+  s.kks = ((2 * nfp8 + 31) / 32) * 32 + 2;
+  s.th = 2;
+  if (s.shm() > 160 * 1024) s.th = 1;
+  return s;
+}
+
+// The fc2td passes over all bins: Wd is rebuilt in d_wd (at least
+// bins * kks doubles) for each pass; ordered on `st`, asynchronous.
+static hipError_t fc2td_field_passes(const double* d_field, float* d_out, const double* d_f, const double* d_t,
+                                     int nf, int nt, long long ncol, double* d_wd, int ncu, hipStream_t st) {
+  const Fc2tdShape sh = fc2td_shape(nf);
+  const int ntp = sh.bins();
+  const long long ntiles = (ncol + 15) / 16, slots = 16 / sh.th;
+  const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(ncu, (ntiles + slots - 1) / slots));
+  hipError_t e = hipFuncSetAttribute((const void*)k_fc2td_mfma64<kFc2tdNtt>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.shm());
+  for (int t0 = 0; t0 < nt && e == hipSuccess; t0 += ntp) {
+    k_build_wd<<<div_up((long long)ntp * sh.kks, 256), 256, 0, st>>>(d_wd, d_f, d_t, t0, nt, nf, ntp, sh.kks);
+    e = hipGetLastError();
+    if (e != hipSuccess) break;
+    k_fc2td_mfma64<kFc2tdNtt><<<grid, 1024, sh.shm(), st>>>(d_field, d_wd, d_out, ncol, nf, sh.kks, t0, nt, sh.th);
+    e = hipGetLastError();
+  }
+  return e;
 }
 
 // MFMA J-TG.  tab = [dcol(32) | scol(32) | mlink(32x32)]: field columns of
@@ -447,33 +492,15 @@ int dogpup_fc2td_field(const double* d_field, float* d_out, const double* d_f, c
                        long long ncol, void* stream) {
   DP_ARG(d_field && d_out && d_f && d_t && nf >= 1 && nf <= 128 && nt >= 1 && ncol >= 1, "bad arguments");
   hipStream_t st = as_stream(stream);
-  const int nfp8 = (nf + 7) & ~7;
-  const int kks = ((2 * nfp8 + 63) / 64) * 64 + 4;  // row stride = 4 mod 64 banks: conflict-free A reads
   int dev = 0, ncu = 0;
   DP_HIP(hipGetDevice(&dev));
   DP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  float* Wr = nullptr;
-  DP_HIP(hipMalloc((void**)&Wr, (size_t)256 * kks * 4));
-  const long long ntiles = (ncol + 15) / 16;
-  hipError_t e = hipSuccess;
-  for (int t0 = 0; t0 < nt && e == hipSuccess; t0 += 256) {
-    const int left = nt - t0;
-    // 256-bin passes: two wavefronts of 8 time tiles per column tile;
-    // shorter tails: one wavefront of 8 or 4 tiles
-    const int ntt = 4, th = left > 128 ? 4 : (left > 64 ? 2 : 1);
-    const int ntp = th * ntt * 16;
-    const size_t shm = (size_t)ntp * kks * 4;
-    k_build_wr<<<div_up((long long)ntp * kks, 256), 256, 0, st>>>(Wr, d_f, d_t, t0, nt, nf, ntp, kks);
-    e = hipGetLastError();
-    if (e != hipSuccess) break;
-    const long long slots = 16 / th;
-    const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(ncu, (ntiles + slots - 1) / slots));
-    e = hipFuncSetAttribute((const void*)k_fc2td_mfma<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-    if (e == hipSuccess) k_fc2td_mfma<4><<<grid, 1024, shm, st>>>(d_field, Wr, d_out, ncol, nf, kks, t0, nt, th);
-    if (e == hipSuccess) e = hipGetLastError();
-  }
-  const hipError_t e2 = hipStreamSynchronize(st);   // Wr is freed after the kernels used it
-  (void)hipFree(Wr);
+  const Fc2tdShape sh = fc2td_shape(nf);
+  double* Wd = nullptr;
+  DP_HIP(hipMalloc((void**)&Wd, (size_t)sh.bins() * sh.kks * sizeof(double)));
+  const hipError_t e = fc2td_field_passes(d_field, d_out, d_f, d_t, nf, nt, ncol, Wd, ncu, st);
+  const hipError_t e2 = hipStreamSynchronize(st);   // Wd is freed after the kernels used it
+  (void)hipFree(Wd);
   if (e != hipSuccess) return fail(DOGPUP_ERR_HIP, "fc2td_field launch: %s", hipGetErrorString(e));
   if (e2 != hipSuccess) return fail(DOGPUP_ERR_HIP, "fc2td_field: %s", hipGetErrorString(e2));
   return DOGPUP_OK;

@@ -63,7 +63,7 @@ def fc2td(fun_fc, f, t, dim):
 
 def fc2td_field(field, f, t):
     """fc2td over a whole solver-layout field (fc2td.m:24-31 along the
-    frequency axis of phi) on FP32 MFMA: returns float32 [NT][groups][NN][NSP]
+    frequency axis of phi) on FP64 MFMA: returns float32 [NT][groups][NN][NSP]
     (the time-domain field in the solver's column layout;
     td_field_to_reference gives the reference's NN x NT x NCOLS)."""
     f = _dev(f, torch.float64).reshape(-1)

@@ -254,13 +254,15 @@ int dogpup_td2fc(const double* d_in, double* d_out, const double* d_f, const dou
 int dogpup_fc2td(const double* d_in, double* d_out, const double* d_f, const double* d_t, int nf,
                  int nt, int ncol, void* stream);
 
-/* fc2td (fc2td.m:24-31) of a whole field on FP32 MFMA with FP32 outputs:
+/* fc2td (fc2td.m:24-31) of a whole field on FP64 MFMA with FP32 outputs:
  * out[t][e] = 2 Re sum_k w_k F[k][e] exp(i f_k t_t), w_0 = 1/2, for every
  * column e < ncol of a complex128 block laid out [k][e] (a solver-layout
  * field: ncol = groups-per-frequency * nn * nsp, so out is the time-domain
  * field in the same [t][group][node][column] order).  This is the
  * reference's fc2td(phi, f, t, 2) over nodal fluence (NN x NF x NS ->
- * NN x NT x NS) with the layout of the solver.  nf <= 128. */
+ * NN x NT x NS) with the layout of the solver.  Products and sums are FP64
+ * (a bin of a curve far from its source is a near-total cancellation of the
+ * 2*nf terms); only the stored value is FP32.  nf <= 128.  Synchronous. */
 int dogpup_fc2td_field(const double* d_field, float* d_out, const double* d_f, const double* d_t, int nf,
                        int nt, long long ncol, void* stream);
 

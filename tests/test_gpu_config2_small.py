@@ -1,0 +1,107 @@
+"""The batched multi-frequency TD forward workload at a size the oracle
+covers: the 60 x 60 x 30 mm, 2 mm Kuhn slab (15,376 nodes, interior jitter
+0.2 mm) with a 6 mm sphere of mua 0.03/mm, 32 sources and 32 detectors on
+two interleaved 4 x 8 top-face grids, a 20 ns time axis, so 64 frequencies
+0-3.15 GHz in 50 MHz steps (2048 RHS in 32-wide groups through the two-half
+x-halo SpMVs), mixed COCG and mixed BiCGSTAB to the 5e-8 tolerance of the
+full-size run, then fc2td of the nodal fluence to 256 bins of 20 ps.
+
+Every column is compared with the oracle's direct sparse solve
+(dMesh.m:1238-1348 to convergence) and the TD curves with the oracle's fc2td
+(fc2td.m:24-31), curve by curve, including the curves of the nodes farthest
+from every source, whose bins are near-total cancellations of the 128
+frequency terms."""
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+TOL = 5e-8
+
+
+@pytest.fixture(scope="module")
+def c2():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dogpup_amd import meshgen, mesh as dm
+    from oracle import fem, optode as oopt, solver as osol, transforms as otr
+    m = meshgen.slab((60.0, 60.0, 30.0), 2.0, 0.01, 1.0, 1.4, jitter=0.2, seed=0,
+                     inclusions=[((30.0, 30.0, 18.0), 6.0, 0.03)])
+    src = meshgen.top_grid(m.extent, 4, 8, 9.0, 4.0, 12.0, 7.0)
+    det = meshgen.top_grid(m.extent, 4, 8, 15.0, 7.5, 12.0, 7.0)
+    link = meshgen.all_pairs(32, 32)
+    t = np.arange(0, 1001) * 20e-12
+    nf = 64
+    D = dm.DMesh(m.node, m.elem, m.nr, np.stack([m.mua, m.musp], 1))
+    opt = dm.DOptode(src, det, link, 1e3, meshgen.gaussian_tpsf(t, 250e-12), t, nf)
+    D.add_optode(opt)
+    om = fem.make_mesh(m.node, m.elem, m.nr, m.mua, m.musp)
+    f, df = otr.freq_axis(t, nf)
+    assert abs(f[-1] / (2 * np.pi) - 3.15e9) < 1e3
+    A = fem.full_system(om, df)
+    sid, sw = opt.s_bary
+    print("[case] config-2 workload: oracle direct solve, 64 frequencies x 32 sources", flush=True)
+    phi = osol.direct(A, oopt.source_matrix(om, sid, sw, opt.s_fpsf))      # NN x NF x NS
+    # TD curves are checked at 1,000 seeded nodes and the 300 nodes farthest
+    # from every source
+    dist = np.min(np.linalg.norm(m.node[:, None, :] - src[None, :, :], axis=2), axis=1)
+    rng = np.random.default_rng(2)
+    nodes = np.unique(np.concatenate([rng.choice(m.node.shape[0], 1000, replace=False),
+                                      np.argsort(dist)[-300:]]))
+    t_td = np.arange(256) * 20e-12
+    return dict(D=D, opt=opt, phi=phi, f=f, t_td=t_td, nodes=nodes, far=np.argsort(dist)[-300:],
+                td_lu=otr.fc2td(phi[nodes], f, t_td, 1))                          # nodes x NT x NS
+
+
+def _curve_err(got, ref):
+    """max over bins of |got - ref| / max |ref|, per (node, source) curve."""
+    return np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
+
+
+@pytest.mark.parametrize("mode,method,tol,bound", [("mixed", "cocg", TOL, 1e-6), ("mixed", "bicgstab", TOL, 1e-6),
+                                                  ("fp64", "bicgstab", 1e-12, 1e-9)])
+def test_config2_workload_fluence_and_td(c2, mode, method, tol, bound):
+    from dogpup_amd import transforms as tr
+    from oracle import transforms as otr
+    D, phi, nodes = c2["D"], c2["phi"], c2["nodes"]
+    print(f"[case] config-2 workload: {mode} {method} to {tol}", flush=True)
+    fw = D.solve_point_sources("s", mode=mode, tol=tol, method=method)
+    assert (fw.status == 0).all(), fw.status
+    assert (fw.relres < tol).all(), fw.relres.max()
+    got = fw.to_reference_layout().cpu().numpy()                  # NN x NF x NS
+    errs = np.array([[rel(got[:, k, s], phi[:, k, s]) for s in range(32)] for k in range(64)])
+    print(f"fluence rel err per column: max {errs.max():.2e}, median {np.median(errs):.2e}", flush=True)
+    assert errs.max() < bound                                      # mixed: the 1e-6 fluence target
+    print("[case] config-2 workload: fc2td_field to 256 bins", flush=True)
+    td = tr.td_field_to_reference(tr.fc2td_field(fw, c2["f"], c2["t_td"]), fw.ncols)
+    td = td[torch.as_tensor(nodes, device="cuda")].cpu().numpy().astype(np.float64)  # nodes x NT x NS
+    # the transform itself: FP64 MFMA against the oracle's fc2td of the same
+    # field, curve by curve (FP32 output rounding only), far nodes included
+    same = otr.fc2td(got[nodes], c2["f"], c2["t_td"], 1)
+    e_same = _curve_err(td, same)
+    far = np.isin(nodes, c2["far"])
+    print(f"TD per-curve err / peak, same field: max {e_same.max():.2e} (far nodes {e_same[far].max():.2e})",
+          flush=True)
+    assert e_same.max() < 1e-5
+    # end to end against the oracle's fc2td of the direct solve.  A curve
+    # whose 0-5.1 ns window holds a tiny fraction of its source's signal is
+    # an almost total cancellation of its node's frequency coefficients, so
+    # there the error is the solve's own (a norm-wise tolerance), relative to
+    # a much smaller number: reported per band of the curve's peak relative
+    # to its source's largest curve peak
+    ref = c2["td_lu"]
+    e_lu = _curve_err(td, ref)
+    pk = np.abs(ref).max(axis=1)                                    # nodes x NS
+    relpk = pk / pk.max(axis=0, keepdims=True)
+    bands = [(1e-2, 2.0), (1e-4, 1e-2), (1e-6, 1e-4), (0.0, 1e-6)]
+    for lo, hi in bands:
+        sel = (relpk >= lo) & (relpk < hi)
+        if sel.any():
+            print(f"  curves with peak in [{lo:g}, {hi:g}) of the source's largest: {int(sel.sum())}, "
+                  f"err/peak max {e_lu[sel].max():.2e}, median {np.median(e_lu[sel]):.2e}", flush=True)
+    strong = relpk >= 1e-2
+    assert e_lu[strong].max() < (1e-3 if mode == "mixed" else 1e-6)

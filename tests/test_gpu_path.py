@@ -213,8 +213,9 @@ def test_jacobian_tg(setup):
 
 
 def test_fc2td_field_mfma(setup):
-    """fc2td over a solved nodal field on FP32 MFMA against the FP64 oracle
-    fc2td(phi, f, t, 2), for 300 time bins (a 256-bin pass and a tail)."""
+    """fc2td over a solved nodal field on FP64 MFMA against the FP64 oracle
+    fc2td(phi, f, t, 2) of the same field, for 300 time bins (two 128-bin
+    passes and a tail): every curve to the FP32 rounding of its peak."""
     from dogpup_amd import transforms as tr
     from oracle import optode as oopt, solver as osol, transforms as otr
     D, A, om, opt = setup["D"], setup["A"], setup["om"], setup["opt"]
@@ -228,6 +229,9 @@ def test_fc2td_field_mfma(setup):
     got = tr.td_field_to_reference(td, fw.ncols).cpu().numpy()
     assert got.shape == ref.shape
     assert rel(got, ref) < 1e-5
+    same = otr.fc2td(fw.to_reference_layout().cpu().numpy(), setup["f"], t, 1)
+    curve = np.abs(got - same).max(axis=1) / np.abs(same).max(axis=1)
+    assert curve.max() < 1e-6, curve.max()
 
 
 @pytest.mark.parametrize("ng,nf", [(11, 7), (3, 64)])
