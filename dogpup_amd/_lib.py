@@ -50,6 +50,7 @@ SIGNATURES = {
     "dogpup_td2fc": [vp, vp, vp, vp, i32, i32, i32, i32, vp],
     "dogpup_fc2td": [vp, vp, vp, vp, i32, i32, i32, vp],
     "dogpup_fc2td_field": [vp, vp, vp, vp, i32, i32, C.c_longlong, vp],
+    "dogpup_fc2td_field_noalloc": [vp, vp, vp, vp, i32, i32, C.c_longlong, vp],
     "dogpup_fc2tg": [vp, vp, vp, i32, i32, i32, f64, vp],
     "dogpup_jacobian_tg": [vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32, vp],
     "dogpup_meas": [vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp],

@@ -40,6 +40,7 @@
 // node's fluence of every frequency in LDS (complex128) and every lane
 // produces whole (m, g) rows, summed in FP64.
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -468,6 +469,37 @@ __global__ __launch_bounds__(256) void k_jtg_mfma(const double* __restrict__ fie
   }
 }
 
+// Scratch of dogpup_fc2td_field_noalloc: the largest DFT table of one pass
+// (128 bins x 130 doubles for nf <= 64, 64 x 258 for nf <= 128), 133,120 B.
+constexpr int kFc2tdScratch = 128 * 130;
+static __device__ __align__(256) double scratch[kFc2tdScratch];
+
+constexpr int kMaxDevices = 64;
+static std::mutex fc2td_scr_mu;
+static double* fc2td_scr[kMaxDevices] = {};   // device address of `scratch` per device
+static int fc2td_ncu[kMaxDevices] = {};
+
+// The current device's copy of `scratch` (resolved on the first call on the
+// device, with that device current) and its CU count.
+static int fc2td_scratch(double** out, int* ncu) {
+  int dev = 0;
+  DP_HIP(hipGetDevice(&dev));
+  DP_ARG(dev >= 0 && dev < kMaxDevices, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(fc2td_scr_mu);
+  if (!fc2td_scr[dev]) {
+    void* p = nullptr;
+    DP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    int n = 0;
+    DP_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    DP_ARG(p != nullptr && n > 0, "fc2td scratch: no device address");
+    fc2td_ncu[dev] = n;
+    fc2td_scr[dev] = static_cast<double*>(p);
+  }
+  *out = fc2td_scr[dev];
+  *ncu = fc2td_ncu[dev];
+  return DOGPUP_OK;
+}
+
 }  // namespace dogpup
 
 using namespace dogpup;
@@ -503,6 +535,26 @@ int dogpup_fc2td_field(const double* d_field, float* d_out, const double* d_f, c
   (void)hipFree(Wd);
   if (e != hipSuccess) return fail(DOGPUP_ERR_HIP, "fc2td_field launch: %s", hipGetErrorString(e));
   if (e2 != hipSuccess) return fail(DOGPUP_ERR_HIP, "fc2td_field: %s", hipGetErrorString(e2));
+  return DOGPUP_OK;
+}
+
+// Allocation-free fc2td_field: the per-pass DFT table lives in `scratch`, a
+// static device array (one copy per GPU, kept until the process ends).  Host
+// code never passes the array's name to a kernel: its device address is
+// resolved once per device with hipGetSymbolAddress and kept in fc2td_scr.
+int dogpup_fc2td_field_noalloc(const double* d_field, float* d_out, const double* d_f, const double* d_t, int nf,
+                               int nt, long long ncol, void* stream) {
+  DP_ARG(d_field && d_out && d_f && d_t && nf >= 1 && nf <= 128 && nt >= 1 && ncol >= 1, "bad arguments");
+  const Fc2tdShape sh = fc2td_shape(nf);
+  const long long need = (long long)sh.bins() * sh.kks;
+  DP_ARG(need <= kFc2tdScratch, "fc2td_field_noalloc: nf = %d needs %lld doubles of scratch, the static buffer holds %d",
+         nf, need, kFc2tdScratch);
+  double* wd = nullptr;
+  int ncu = 0;
+  const int rc = fc2td_scratch(&wd, &ncu);
+  if (rc != DOGPUP_OK) return rc;
+  const hipError_t e = fc2td_field_passes(d_field, d_out, d_f, d_t, nf, nt, ncol, wd, ncu, as_stream(stream));
+  if (e != hipSuccess) return fail(DOGPUP_ERR_HIP, "fc2td_field_noalloc launch: %s", hipGetErrorString(e));
   return DOGPUP_OK;
 }
 

@@ -6,7 +6,9 @@
  *  - Every entry point returns an int status: DOGPUP_OK (0) or a negative
  *    DOGPUP_ERR_*; dogpup_last_error() gives the message of the last failure on
  *    the calling thread.  No C++ exception crosses this boundary.
- *  - Threads: the library keeps no mutable global state.  A handle (solver,
+ *  - Threads: the library keeps no mutable global state other than the
+ *    per-device scratch array of dogpup_fc2td_field_noalloc (whose address
+ *    lookup is locked).  A handle (solver,
  *    locator) is not locked: use one handle per thread, or serialise the
  *    calls on a handle; different handles may run on different threads at
  *    once (graph capture is thread-local to the capturing thread).
@@ -265,6 +267,18 @@ int dogpup_fc2td(const double* d_in, double* d_out, const double* d_f, const dou
  * 2*nf terms); only the stored value is FP32.  nf <= 128.  Synchronous. */
 int dogpup_fc2td_field(const double* d_field, float* d_out, const double* d_f, const double* d_t, int nf,
                        int nt, long long ncol, void* stream);
+
+/* Allocation-free dogpup_fc2td_field: same arguments, result and limits
+ * (nf <= 128), asynchronous on `stream`, and no device allocation or
+ * host synchronisation.  No counterpart in the reference: it is for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops).
+ * The per-pass DFT table lives in a static device array of 133,120 bytes
+ * (16,640 doubles) that every GPU the library runs on holds until the
+ * process ends; the first call on a device resolves that array's address
+ * (make it once before capturing a graph).  All calls on one GPU share the
+ * array: calls that overlap on one GPU (other streams or threads) race. */
+int dogpup_fc2td_field_noalloc(const double* d_field, float* d_out, const double* d_f, const double* d_t, int nf,
+                               int nt, long long ncol, void* stream);
 
 /* Replaces fc2tg (fc2tg.m:1-38): out(g,col) = nt_factor * 2 Re sum_k w_k
  * conj(gates(g,k)) in(k,col).  gates: NG x NF complex (row-major);
