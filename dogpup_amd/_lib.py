@@ -63,6 +63,8 @@ SIGNATURES = {
     "dogpup_solver_set_profile": [vp, i32],
     "dogpup_solver_set_method": [vp, i32],
     "dogpup_solver_set_rhs_tol": [vp, vp, i32],
+    "dogpup_solver_set_guess": [vp, i32, vp, vp],
+    "dogpup_solver_set_freq_continuation": [vp, i32, vp],
     "dogpup_solver_profile": [vp, vp, i32],
     "dogpup_locator_create": [P(vp), vp, i32, vp, i32, vp],
     "dogpup_locator_destroy": [vp],

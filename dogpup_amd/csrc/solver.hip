@@ -1085,14 +1085,15 @@ __global__ __launch_bounds__(256) void k_init(const T* __restrict__ b, T* __rest
 // is caller row order[i]), so the right-hand side needs no permuted copy.
 // When `write`, stores complex64 r (and p and rhat, if given) = r64 and
 // x32 = 0 for columns whose outer status is active (zeros elsewhere);
-// part1 = sum |r64|^2 (FP64), part2 = sum r32 r32.
+// part1 = sum |r64|^2 (FP64), part2 = sum r32 r32, part3 (if given) =
+// sum |b|^2 (the reference norm when x64 starts from a guess).
 template <int NSP>
 __global__ __launch_bounds__(256) void k_residual64(
     const int* __restrict__ rptr, const int* __restrict__ cptr, const double* __restrict__ are,
     const double* __restrict__ aim, const double* __restrict__ b, const int* __restrict__ order,
     const double* __restrict__ x64, float* __restrict__ r32, float* __restrict__ rhat32,
     float* __restrict__ p32, float* __restrict__ x32, const int* __restrict__ ostatus, int write,
-    double* __restrict__ part1, double* __restrict__ part2, Work W, int ilv) {
+    double* __restrict__ part1, double* __restrict__ part2, Work W, int ilv, double* __restrict__ part3 = nullptr) {
   using G = Geo<double, NSP>;  // one complex128 per lane
   Item it;
   if (!work_item(W, it)) return;
@@ -1105,13 +1106,15 @@ __global__ __launch_bounds__(256) void k_residual64(
   const long long rhs = (long long)it.gl * NSP + c;
   const bool act = ostatus ? (ostatus[rhs] == ST_ACTIVE) : true;
   const long long vb = it.vb;
-  Acc<1> d1, d2;
+  Acc<1> d1, d2, d3;
   d1.zero();
   d2.zero();
+  d3.zero();
   auto epi = [&](int row, const double* yr, const double* yi) {
     const long long off = (vb + row) * NSP + c;
     const long long boff = order ? (vb + order[row]) * NSP + c : off;
     const double2 bv = *reinterpret_cast<const double2*>(b + 2 * boff);
+    d3.re[0] += bv.x * bv.x + bv.y * bv.y;
     double rr = bv.x - yr[0], ri = bv.y - yi[0];
     if (!act) { rr = 0.0; ri = 0.0; }
     d1.re[0] += rr * rr + ri * ri;
@@ -1145,6 +1148,57 @@ __global__ __launch_bounds__(256) void k_residual64(
   if (write) {
     __syncthreads();
     block_reduce_store<double, NSP>(d2, sh, part2, it.wl);
+  }
+  if (part3) {
+    __syncthreads();
+    block_reduce_store<double, NSP>(d3, sh, part3, it.wl);
+  }
+}
+
+// Initial FP64 iterate of every group of the work list from its guess
+// terms: x64[g] = sum_q w[g][q] x64[src[g][q]] (w complex; src -1 ends the terms, no
+// term gives 0).  A source is another group of the same mesh and chunk
+// (solved in an earlier phase, so not written here), the group itself
+// (the caller's x on entry, read before this element is written), or a group
+// of an earlier chunk (src < W.g0: its solution is already in the caller's
+// x, xout, in the caller's node order).
+template <int NSP>
+__global__ __launch_bounds__(256) void k_guess(double* __restrict__ x64, Work W, const int* __restrict__ gsrc,
+                                               const double* __restrict__ gw, int nterm,
+                                               const double* __restrict__ xout, const int* __restrict__ order) {
+  Item it;
+  if (!work_item(W, it)) return;
+  const int nr = it.row1 - it.row0;
+  for (int i = threadIdx.x; i < nr * NSP; i += 256) {
+    const int row = it.row0 + i / NSP, c = i % NSP;
+    double re = 0.0, im = 0.0;
+    for (int q = 0; q < nterm; ++q) {
+      const int sg = gsrc[(long long)it.g * nterm + q];
+      if (sg < 0) break;
+      const double2 w = reinterpret_cast<const double2*>(gw)[(long long)it.g * nterm + q];
+      double2 v;
+      if (sg < W.g0)
+        v = reinterpret_cast<const double2*>(xout)[(W.gvb[sg] - W.grow0[sg] + order[row]) * NSP + c];
+      else
+        v = reinterpret_cast<const double2*>(x64)[(W.gvb[sg] - W.vb0 - W.grow0[sg] + row) * NSP + c];
+      re += w.x * v.x - w.y * v.y;
+      im += w.x * v.y + w.y * v.x;
+    }
+    reinterpret_cast<double2*>(x64)[(it.vb + row) * NSP + c] = make_double2(re, im);
+  }
+}
+
+// x64 = 0 in the columns whose outer status is DOGPUP_RHS_ZERO (a zero
+// right-hand side, or a padding column), after a guess wrote them.
+template <int NSP>
+__global__ __launch_bounds__(256) void k_zero_cols(double* __restrict__ x64, const int* __restrict__ ostatus, Work W) {
+  Item it;
+  if (!work_item(W, it)) return;
+  const int nr = it.row1 - it.row0;
+  for (int i = threadIdx.x; i < nr * NSP; i += 256) {
+    const int c = i % NSP;
+    if (ostatus[(long long)it.gl * NSP + c] != DOGPUP_RHS_ZERO) continue;
+    reinterpret_cast<double2*>(x64)[(it.vb + it.row0 + i / NSP) * NSP + c] = make_double2(0.0, 0.0);
   }
 }
 
@@ -1578,22 +1632,34 @@ struct RState {
 
 // After a residual pass: decide each column's outer status and its inner
 // tolerance.  part1 = |r64|^2.
+// part3 (sweep 0 only, when x64 starts from a guess): |b|^2, the reference
+// norm.  glevel (optional): only groups of level `level` take part (the
+// current phase of a continuation solve); the others are left untouched.
 __global__ __launch_bounds__(kScalThreads) void k_rf_check(RState o, BState in,
                                                            const double* __restrict__ part1, int nsp, Work W,
                                                            int sweep, double tol, double tol_in, int maxit,
-                                                           const double* __restrict__ ctol) {
+                                                           const double* __restrict__ ctol,
+                                                           const double* __restrict__ part3 = nullptr,
+                                                           const int* __restrict__ glevel = nullptr,
+                                                           int level = 0) {
   const int g = blockIdx.x, c = threadIdx.x;
-  double n2 = 0, dummy = 0;
+  if (glevel && glevel[W.g0 + g] != level) {
+    group_flag(in, W, g, false);
+    return;
+  }
+  double n2 = 0, dummy = 0, b2 = 0;
   block_sum_part(part1, W, g, nsp, n2, dummy);
+  if (part3) block_sum_part(part3, W, g, nsp, b2, dummy);
   bool active = false;
   if (c < nsp) {
     const long long i = (long long)g * nsp + c;
     const int ncols = W.gcols[W.g0 + g], col = c;
     if (sweep == 0) {
-      o.bnorm[i] = sqrt(n2);
+      if (!part3) b2 = n2;
+      o.bnorm[i] = sqrt(b2);
       o.iters[i] = 0;
       o.prev[i] = INFINITY;
-      o.status[i] = (col >= ncols || n2 == 0.0) ? DOGPUP_RHS_ZERO : ST_ACTIVE;
+      o.status[i] = (col >= ncols || b2 == 0.0) ? DOGPUP_RHS_ZERO : ST_ACTIVE;
       o.relres[i] = (o.status[i] == ST_ACTIVE) ? 1.0 : 0.0;
     }
     if (o.status[i] == ST_ACTIVE) {
@@ -1727,7 +1793,7 @@ struct Buf {
 struct dogpup_solver {
   int nmesh = 0, nn = 0, nsp = 0, ngroups = 0, nwork = 0, mode = 0;
   int nnzA = 0, nnzG = 0, nrhs_out = 0, nf_total = 0;
-  int method = DOGPUP_METHOD_BICGSTAB;  // Krylov method of the fused forms
+  int method = DOGPUP_METHOD_COCG;  // Krylov method of the fused forms (the plain form: BiCGSTAB)
   // meshes: rows [m_row0[i], m_row0[i+1]), nf, ncols, column blocks, first
   // group, first A and G entries, G entries, G value base (complex elements,
   // [f][nnzG_i] per mesh), first fscale entry, first output RHS
@@ -1738,6 +1804,7 @@ struct dogpup_solver {
   // h_gibeg: first work item of each group (items padded to multiples of
   // 8); h_gwbeg/h_gwcnt: its partial-sum slots (one per row tile)
   std::vector<int> h_grow0, h_grows, h_gcols, h_gmesh, h_gf, h_gcb, h_gibeg, h_gwbeg, h_gwcnt;
+  std::vector<int> h_wgrp, h_wtile, h_wpart;   // host copy of the work list
   std::vector<long long> h_gvb, h_ggv;
   int nparts = 0;
   dogpup::Buf t_wgrp, t_wtile, t_wpart, t_grow0, t_grows, t_gvb, t_gfsc, t_ggv, t_gwbeg, t_gwcnt, t_gcols;
@@ -1762,13 +1829,35 @@ struct dogpup_solver {
   std::vector<int> mesh_chb_tmp;   // first halo chunk of each mesh (while building)
   // values: a64re/a64im and g64 in reference order; the rest in solver order
   dogpup::Buf a64re, a64im, a64pre, a64pim, a32re, a32im, fscale, g64, gp64, gtp64, g32, gt32;
-  dogpup::Buf ws, part1, part2, state, rstate, bperm, xperm;
+  dogpup::Buf ws, part1, part2, part3, state, rstate, bperm, xperm;
   int tile_rows = 512, lds_pad = 0;
   int unroll_a = 8, unroll_g = 4;  // gathers in flight per row (k_spmv: 4, 8 or 16)
   double ws_budget = 0;             // workspace budget, fixed when the handle is made
   // per-RHS tolerances in group-slot order (g * nsp + c); 0 = the solve's tol
   dogpup::Buf rhs_tol;
   bool have_rhs_tol = false;
+  // Initial guesses (refinement form).  Per frequency (global over the
+  // meshes) up to guess_nterm (source frequency, weight) terms, as given by
+  // dogpup_solver_set_guess or built from fscale by the frequency
+  // continuation schedule (freq_cont); per group the same terms as source
+  // groups (h_gsrc, -1 ends them; the group itself = the caller's x on
+  // entry) and its level (phase).  Each chunk runs its groups level by level
+  // over a work list permuted so that every level is one contiguous range.
+  int guess_nterm = 0;
+  std::vector<int> guess_fsrc;       // nf_total * guess_nterm
+  std::vector<double> guess_fw;      // complex: 2 * nf_total * guess_nterm
+  std::vector<double> h_spec;        // right-hand-side spectrum of the schedule (complex per frequency; empty: flat)
+  bool freq_cont = false;            // schedule from fscale (rebuilt at set_matrix)
+  bool phases_dirty = true;
+  struct Phase {
+    int level, w0, nw;
+    bool guess;        // some group of the phase starts from a guess
+    long long nodes;   // rows summed over the phase's groups
+  };
+  std::vector<std::vector<Phase>> chunk_phases;
+  bool guess_self = false;           // some group reads the caller's x
+  std::vector<double> h_fscale;      // host copy of fscale (set_matrix)
+  dogpup::Buf t_pwgrp, t_pwtile, t_pwpart, t_gsrc, t_gw, t_glevel;
   bool have_matrix = false;
   double stats[4] = {0, 0, 0, 0};
   int* h_counter = nullptr;  // pinned: [0] active RHS, [1] active groups, [2] their rows
@@ -1888,6 +1977,7 @@ struct Chunk {
   dogpup_solver* S;
   hipStream_t st;
   int cidx, gcount;
+  int phase = 0;   // phase of a continuation solve (its own work list and graphs)
   Work W;
   BState bs;
   double *p1, *p2;
@@ -2149,8 +2239,8 @@ struct Chunk {
       // Replay a captured iteration K times between convergence polls;
       // converged groups make every kernel exit early, so replays past
       // convergence cost only launch latency.
-      const long long key = ((long long)cidx << 32) ^ ((long long)maxit << 3) ^ ((long long)S->method << 2) ^
-                            (sizeof(T) == 4 ? 1 : 2);
+      const long long key = ((long long)phase << 52) ^ ((long long)cidx << 32) ^ ((long long)maxit << 3) ^
+                            ((long long)S->method << 2) ^ (sizeof(T) == 4 ? 1 : 2);
       hipGraphExec_t ex = nullptr;
       auto found = S->graphs.find(key);
       if (found != S->graphs.end()) {
@@ -2194,18 +2284,28 @@ static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 // Vector slots carved from the workspace: x (only when the solver owns it,
 // i.e. the complex64 correction), r, rhat, p, v, h, s, z, t, u (u also holds
 // Gs), and w (only the reference-structure form materialises G t).
+// The refinement form's COCG (cocg_only) needs x, r, p, v and u alone.
 template <typename T>
-static Vecs<T> carve(const Buf& ws, size_t vbytes, bool own_x, bool with_w) {
+static Vecs<T> carve(const Buf& ws, size_t vbytes, bool own_x, bool with_w, bool cocg_only = false) {
   Vecs<T> v{};
   size_t off = 0;
   auto take = [&]() { T* p = ws.as<T>(off); off += align256(vbytes); return p; };
   v.x = own_x ? take() : nullptr;
-  v.r = take(); v.rhat = take(); v.p = take(); v.v = take(); v.h = take();
-  v.s = take(); v.z = take(); v.t = take(); v.u = take();
+  v.r = take();
+  v.p = take();
+  v.v = take();
+  v.u = take();
+  if (!cocg_only) {
+    v.rhat = take(); v.h = take(); v.s = take(); v.z = take(); v.t = take();
+  }
   v.w = with_w ? take() : nullptr;
   return v;
 }
-static int nvec_for(int mode) { return mode == DOGPUP_MODE_C64_REFINE ? 10 : (mode == DOGPUP_MODE_C128_PLAIN ? 10 : 9); }
+static bool refine_cocg(int mode, int method) { return mode == DOGPUP_MODE_C64_REFINE && method == DOGPUP_METHOD_COCG; }
+static int nvec_for(int mode, int method) {
+  if (refine_cocg(mode, method)) return 5;
+  return mode == DOGPUP_MODE_C64_REFINE ? 10 : (mode == DOGPUP_MODE_C128_PLAIN ? 10 : 9);
+}
 
 static BState carve_state(const Buf& b, int nrhs, int ngroups) {
   BState s;
@@ -2351,62 +2451,112 @@ static int solve_c128(dogpup_solver* S, hipStream_t st, const double* b, double*
   return DOGPUP_OK;
 }
 
+static int build_phases(dogpup_solver* S);
+
+// a[i] = v for i < n
+__global__ void k_fill_int(int* __restrict__ a, int v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
 template <int NSP>
 static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, double* x, double tol,
                         int maxit, int* h_iters, double* h_relres, int* h_status) {
   const char* env_tol = getenv("DOGPUP_INNER_TOL");
   const double tol_in = env_tol ? atof(env_tol) : 1e-4;
+  if (S->guess_nterm > 0 && S->phases_dirty) DP_TRY(build_phases(S));
+  const bool phased = S->guess_nterm > 0;
   for (int c = 0; c + 1 < (int)S->chunk_g.size(); ++c) {
     Chunk<float, NSP> C = make_chunk<float, NSP>(S, st, c);
     const int gc = C.gcount, g0 = S->chunk_g[c];
     RState R = carve_rstate(S->rstate, gc * NSP);
-    Vecs<float> V = carve<float>(S->ws, (size_t)C.nodes * NSP * 8, true, false);
+    Vecs<float> V = carve<float>(S->ws, (size_t)C.nodes * NSP * 8, true, false, refine_cocg(S->mode, S->method));
     const size_t goff = (size_t)C.W.vb0 * NSP * 2;
     // b is read in the caller's order by every residual pass; x64 (solver
-    // order) is first written by the first sweep's correction
+    // order) is first written by the first sweep's correction, or by the
+    // guess of its phase
     const double* bc = b + goff;
     double* xc = S->xperm.as<double>();
     const bool cocg = S->method == DOGPUP_METHOD_COCG;
-    const double e64 = (double)C.nodes * NSP;  // complex elements of the chunk
-    for (int sweep = 0;; ++sweep) {
-      C.pbegin();
-      // COCG starts from r alone (p = G^T G r is written by its start), so
-      // only BiCGSTAB needs p and rhat
-      k_residual64<NSP><<<C.vgrid(), 256, 0, st>>>(
-          S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), bc, S->order.as<int>(),
-          sweep == 0 ? nullptr : xc, V.r, cocg ? nullptr : V.rhat, cocg ? nullptr : V.p, V.x,
-          sweep == 0 ? nullptr : R.status, 1, C.p1, C.p2, C.W, S->halo_ilv ? 1 : 0);
+    const Work Wfull = C.W;
+    std::vector<dogpup_solver::Phase> one{{0, Wfull.w0, Wfull.nw, false, C.nodes}};
+    const std::vector<dogpup_solver::Phase>& phases = phased ? S->chunk_phases[c] : one;
+    if (phased) {
+      // groups of later phases take no part until their phase
+      k_fill_int<<<div_up(gc * NSP, 256), 256, 0, st>>>(R.status, ST_SKIP, gc * NSP);
       DP_LAUNCH_CHECK();
-      C.pend(3, e64 * (16 + (sweep == 0 ? 0 : 16) + (cocg ? 2 : 4) * 8) + (sweep == 0 ? 0.0 : (double)S->nnzA * 20) +
-                    4.0 * (S->nn + 1));
-      DP_TRY(C.reset_count());
-      k_rf_check<<<gc, kScalThreads, 0, st>>>(R, C.bs, C.p1, NSP, C.W, sweep, tol, tol_in, maxit,
-                                               rhs_tol_at(S, g0));
-      DP_LAUNCH_CHECK();
-      int active = 0;
-      DP_TRY(C.wait_count(&active));
-      if (active == 0) {
-        // every column met the tolerance at x = 0 (zero right-hand sides)
-        if (sweep == 0) DP_HIP(hipMemsetAsync(xc, 0, (size_t)C.nodes * NSP * 16, st));
-        break;
-      }
-      const double sweep_rows = (double)S->active_rows;
-      // inner state from the residual pass: bnorm = |r64|, rho = r32.r32
-      k_sc_init<<<gc, kScalThreads, 0, st>>>(C.bs, C.p1, C.p2, R.status, NSP, C.W);
-      DP_LAUNCH_CHECK();
-      int its = 0;
-      DP_TRY(C.run_fused(V, nullptr, maxit, R.status, &its));
-      C.pbegin();
-      k_add_correction<NSP><<<C.vgrid(), 256, 0, st>>>(xc, V.x, C.bs.status, sweep == 0, C.W);
-      DP_LAUNCH_CHECK();
-      C.pend(3, (sweep == 0 ? e64 * 16 : 0.0) + sweep_rows * NSP * (sweep == 0 ? 8 : 32 + 8));
-      k_rf_accum<<<gc, kScalThreads, 0, st>>>(R, C.bs, NSP);
-      DP_LAUNCH_CHECK();
-      S->stats[0] = std::max(S->stats[0], (double)its);
-      S->stats[2] += 1;
-      S->stats[3] += its;
-      if (sweep > 64) break;
+      if (S->guess_self) DP_TRY(permute_block<NSP>(S, st, Wfull, x + goff, xc, 0));
     }
+    for (size_t ip = 0; ip < phases.size(); ++ip) {
+      const dogpup_solver::Phase& ph = phases[ip];
+      C.W = Wfull;
+      if (phased) {
+        C.W.wgrp = S->t_pwgrp.as<int>();
+        C.W.wtile = S->t_pwtile.as<int>();
+        C.W.wpart = S->t_pwpart.as<int>();
+        C.W.w0 = ph.w0;
+        C.W.nw = ph.nw;
+      }
+      C.phase = (int)ip;
+      const double e64 = (double)ph.nodes * NSP;  // complex elements of the phase
+      if (ph.guess) {
+        k_guess<NSP><<<C.vgrid(), 256, 0, st>>>(xc, C.W, S->t_gsrc.as<int>(), S->t_gw.as<double>(), S->guess_nterm, x,
+                                               S->order.as<int>());
+        DP_LAUNCH_CHECK();
+      }
+      for (int sweep = 0;; ++sweep) {
+        const bool from0 = sweep == 0 && !ph.guess;   // x64 = 0: the residual is b
+        double* p3 = (sweep == 0 && ph.guess) ? S->part3.as<double>() : nullptr;
+        C.pbegin();
+        // COCG starts from r alone (p = G^T G r is written by its start), so
+        // only BiCGSTAB needs p and rhat
+        k_residual64<NSP><<<C.vgrid(), 256, 0, st>>>(
+            S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), bc, S->order.as<int>(),
+            from0 ? nullptr : xc, V.r, cocg ? nullptr : V.rhat, cocg ? nullptr : V.p, V.x,
+            sweep == 0 ? nullptr : R.status, 1, C.p1, C.p2, C.W, S->halo_ilv ? 1 : 0, p3);
+        DP_LAUNCH_CHECK();
+        C.pend(3, e64 * (16 + (from0 ? 0 : 16) + (cocg ? 2 : 4) * 8) + (from0 ? 0.0 : (double)S->nnzA * 20) +
+                      4.0 * (S->nn + 1));
+        DP_TRY(C.reset_count());
+        k_rf_check<<<gc, kScalThreads, 0, st>>>(R, C.bs, C.p1, NSP, C.W, sweep, tol, tol_in, maxit,
+                                                 rhs_tol_at(S, g0), p3, phased ? S->t_glevel.as<int>() : nullptr,
+                                                 ph.level);
+        DP_LAUNCH_CHECK();
+        if (sweep == 0 && ph.guess) {
+          // zero right-hand sides (and padding columns) have x = 0
+          k_zero_cols<NSP><<<C.vgrid(), 256, 0, st>>>(xc, R.status, C.W);
+          DP_LAUNCH_CHECK();
+        }
+        int active = 0;
+        DP_TRY(C.wait_count(&active));
+        if (active == 0) {
+          // every column met the tolerance at its first iterate: x = 0 for
+          // zero right-hand sides, the guess otherwise
+          if (from0) {
+            k_guess<NSP><<<C.vgrid(), 256, 0, st>>>(xc, C.W, nullptr, nullptr, 0, nullptr, nullptr);
+            DP_LAUNCH_CHECK();
+          }
+          break;
+        }
+        const double sweep_rows = (double)S->active_rows;
+        // inner state from the residual pass: bnorm = |r64|, rho = r32.r32
+        k_sc_init<<<gc, kScalThreads, 0, st>>>(C.bs, C.p1, C.p2, R.status, NSP, C.W);
+        DP_LAUNCH_CHECK();
+        int its = 0;
+        DP_TRY(C.run_fused(V, nullptr, maxit, R.status, &its));
+        C.pbegin();
+        k_add_correction<NSP><<<C.vgrid(), 256, 0, st>>>(xc, V.x, C.bs.status, from0, C.W);
+        DP_LAUNCH_CHECK();
+        C.pend(3, (from0 ? e64 * 16 : 0.0) + sweep_rows * NSP * (from0 ? 8 : 32 + 8));
+        k_rf_accum<<<gc, kScalThreads, 0, st>>>(R, C.bs, NSP);
+        DP_LAUNCH_CHECK();
+        S->stats[0] = std::max(S->stats[0], (double)its);
+        S->stats[2] += 1;
+        S->stats[3] += its;
+        if (sweep > 64) break;
+      }
+    }
+    C.W = Wfull;
     DP_TRY(permute_block<NSP>(S, st, C.W, xc, x + goff, 1));
     DP_TRY(export_results(S, st, g0, gc, R.iters, R.relres, R.status, h_iters, h_relres, h_status));
   }
@@ -2659,6 +2809,9 @@ static int build_work(dogpup_solver* S) {
   }
   S->nwork = (int)wg.size();
   S->nparts = parts;
+  S->h_wgrp = wg;
+  S->h_wtile = wt;
+  S->h_wpart = wp;
   DP_TRY(upload(S->t_wgrp, wg.data(), wg.size() * 4));
   DP_TRY(upload(S->t_wtile, wt.data(), wt.size() * 4));
   DP_TRY(upload(S->t_wpart, wp.data(), wp.size() * 4));
@@ -2666,7 +2819,7 @@ static int build_work(dogpup_solver* S) {
   DP_TRY(upload(S->t_gwcnt, S->h_gwcnt.data(), ng * 4));
   // chunks: consecutive groups whose vectors fit the workspace budget
   const size_t csize = S->mode == DOGPUP_MODE_C64_REFINE ? 8 : 16;
-  const double per_node = (double)S->nsp * (csize * nvec_for(S->mode) + 2.0 * 16);
+  const double per_node = (double)S->nsp * (csize * nvec_for(S->mode, S->method) + 2.0 * 16);
   if (S->ws_budget <= 0) {
     double io_rows = 0;
     for (int g = 0; g < ng; ++g) io_rows += S->h_grows[g];
@@ -2693,14 +2846,229 @@ static int build_work(dogpup_solver* S) {
     max_work = std::max(max_work, (g1 < ng ? S->h_gwbeg[g1] : S->nparts) - S->h_gwbeg[g0]);
   }
   const size_t vbytes = align256((size_t)max_nodes * S->nsp * csize);
-  DP_TRY(S->ws.ensure((size_t)nvec_for(S->mode) * vbytes));
+  DP_TRY(S->ws.ensure((size_t)nvec_for(S->mode, S->method) * vbytes));
   DP_TRY(S->bperm.ensure((size_t)max_nodes * S->nsp * 16));
   DP_TRY(S->xperm.ensure((size_t)max_nodes * S->nsp * 16));
   const size_t pbytes = (size_t)max_work * S->nsp * 16;
   DP_TRY(S->part1.ensure(pbytes));
   DP_TRY(S->part2.ensure(pbytes));
+  DP_TRY(S->part3.ensure(pbytes));
+  S->phases_dirty = true;
   DP_TRY(S->state.ensure(state_bytes(max_groups * S->nsp, max_groups)));
   DP_TRY(S->rstate.ensure(5 * align256((size_t)max_groups * S->nsp * 8)));
+  S->clear_graphs();
+  return DOGPUP_OK;
+}
+
+// Frequency continuation schedule of one mesh with nf frequencies at the
+// scales fs[0..nf): frequency k is in level 0 when k % 8 == 0 or k == nf - 1,
+// level 1 when k % 8 == 4, level 2 when k % 4 == 2 and level 3 otherwise.  A
+// frequency of a later level starts from the Lagrange interpolation, in the
+// frequency scale, of the kGuessTerms frequencies already solved nearest to
+// it (ties to the lower index): those of earlier levels in its chunk, and
+// every frequency of an earlier chunk.  The diffusion fields are smooth in
+// the modulation frequency: on the 50 MHz grid of a 20 ns period the cubic
+// interpolation of the odd frequencies from their even neighbours leaves a
+// first residual of 1e-8..3e-6 of |b| on the 269k-node slab, against |b|
+// itself from x0 = 0.  When the right-hand side of frequency k is spec[k]
+// times a vector that does not depend on the frequency (point sources: the
+// source spectrum, dMesh.m:128,196), it is x / spec that is smooth (the
+// spectrum of a delayed pulse turns by 0.2 rad per step), so a source j's
+// term carries spec[k] / spec[j] (spec NULL: flat); frequencies with a zero
+// spectrum are no source.
+constexpr int kGuessTerms = 4;
+constexpr int kMaxGuessTerms = 8;
+// Level of frequency index k: 0 for every stride-th and the last, then one
+// level per halving of the spacing (stride 8: k % 8 == 4 -> 1, k % 4 == 2 ->
+// 2, odd -> 3).
+static int cont_stride() {
+  static const int s = [] {
+    const char* e = getenv("DOGPUP_CONT_STRIDE");
+    int v = e ? atoi(e) : 8;
+    if (v < 2 || (v & (v - 1))) v = 8;
+    return v;
+  }();
+  return s;
+}
+static int cont_level(int k, int nf) {
+  const int st = cont_stride();
+  if (k % st == 0 || k == nf - 1) return 0;
+  int L = 0, tz = 0;
+  for (int v = st; v > 1; v >>= 1) ++L;
+  while (!((k >> tz) & 1)) ++tz;
+  return L - tz;
+}
+// Terms of frequency k: the kGuessTerms frequencies j with avail(j) nearest
+// to k (distinct scales, nonzero spectrum), with the Lagrange weights at
+// fs[k] times spec[k] / spec[j].  Returns the number of terms.
+template <typename Avail>
+static int cont_terms(const double* fs, const double* spec, int nf, int k, Avail avail, int* sk, double* wk) {
+  auto sp = [&](int j) { return spec ? cpx<double>{spec[2 * j], spec[2 * j + 1]} : cpx<double>{1.0, 0.0}; };
+  std::vector<int> cand;
+  for (int j = 0; j < nf; ++j) {
+    const cpx<double> sj = sp(j);
+    if (j != k && avail(j) && (sj.re != 0.0 || sj.im != 0.0)) cand.push_back(j);
+  }
+  if (cand.empty()) return 0;
+  std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) {
+    const double da = std::fabs(fs[a] - fs[k]), db = std::fabs(fs[b] - fs[k]);
+    return da < db || (da == db && a < b);
+  });
+  const cpx<double> skk = sp(k);
+  if (fs[cand[0]] == fs[k]) {   // the same frequency scale: its solution itself
+    const cpx<double> r = cdiv(skk, sp(cand[0]));
+    sk[0] = cand[0];
+    wk[0] = r.re;
+    wk[1] = r.im;
+    return 1;
+  }
+  // distinct scales only (a repeated node would make the weights singular)
+  std::vector<int> nodes;
+  for (int j : cand) {
+    bool dup = false;
+    for (int q : nodes) dup |= fs[q] == fs[j];
+    if (!dup) nodes.push_back(j);
+    if ((int)nodes.size() == kGuessTerms) break;
+  }
+  const int n = (int)nodes.size();
+  for (int i = 0; i < n; ++i) {
+    double wi = 1.0;
+    for (int m = 0; m < n; ++m)
+      if (m != i) wi *= (fs[k] - fs[nodes[m]]) / (fs[nodes[i]] - fs[nodes[m]]);
+    const cpx<double> r = cdiv(skk, sp(nodes[i]));
+    sk[i] = nodes[i];
+    wk[2 * i] = wi * r.re;
+    wk[2 * i + 1] = wi * r.im;
+  }
+  return n;
+}
+
+// Switches the guesses to the continuation schedule; build_phases derives
+// each group's terms from fscale, the spectrum and the chunk plan.
+static void build_freq_cont(dogpup_solver* S) {
+  S->guess_nterm = kGuessTerms;
+  S->guess_fsrc.clear();
+  S->guess_fw.clear();
+  S->phases_dirty = true;
+  S->clear_graphs();
+}
+
+// Per chunk: each group's guess terms as source groups (a term whose
+// frequency lies in a later chunk drops the group's guess; one in an earlier
+// chunk is read from the caller's x), its level (0
+// without terms of other frequencies, else one more than the latest source)
+// and the phases: the levels present, each one contiguous range of a copy of
+// the work list permuted level by level (a group's items stay together and
+// keep their position modulo 8, so their XCD mapping is unchanged).
+static int build_phases(dogpup_solver* S) {
+  S->chunk_phases.clear();
+  S->phases_dirty = false;
+  S->guess_self = false;
+  const int ng = S->ngroups, nt = S->guess_nterm;
+  if (nt <= 0) return DOGPUP_OK;
+  std::vector<int> gsrc((size_t)ng * nt, -1), level(ng, 0);
+  std::vector<double> gw((size_t)ng * nt * 2, 0.0);
+  std::vector<int> pwg, pwt, pwp;
+  pwg.reserve(S->nwork);
+  pwt.reserve(S->nwork);
+  pwp.reserve(S->nwork);
+  auto gend = [&](int g) { return g + 1 < ng ? S->h_gibeg[g + 1] : S->nwork; };
+  for (size_t c = 0; c + 1 < S->chunk_g.size(); ++c) {
+    const int g0 = S->chunk_g[c], g1 = S->chunk_g[c + 1];
+    bool any_guess = false;
+    for (int g = g0; g < g1; ++g) {
+      const int m = S->h_gmesh[g], f = S->m_fs0[m] + S->h_gf[g];
+      int n = 0;
+      bool ok = true;
+      if (S->freq_cont) {
+        // the schedule: a later-level frequency interpolates the nearest
+        // frequencies already solved, of lower levels in this chunk or of an
+        // earlier chunk
+        const int nfm = S->m_nf[m], k = S->h_gf[g], lk = cont_level(k, nfm);
+        auto grp = [&](int j) { return S->m_g0[m] + j * S->m_ncb[m] + S->h_gcb[g]; };
+        auto avail = [&](int j) {
+          const int sg = grp(j);
+          return sg < g0 || (sg < g1 && cont_level(j, nfm) < lk);
+        };
+        int sk[kGuessTerms];
+        double wk[2 * kGuessTerms];
+        if (lk > 0)
+          n = cont_terms(S->h_fscale.data() + S->m_fs0[m], S->h_spec.empty() ? nullptr : S->h_spec.data() + 2 * S->m_fs0[m],
+                         nfm, k, avail, sk, wk);
+        for (int q = 0; q < n; ++q) {
+          gsrc[(size_t)g * nt + q] = grp(sk[q]);
+          gw[2 * ((size_t)g * nt + q)] = wk[2 * q];
+          gw[2 * ((size_t)g * nt + q) + 1] = wk[2 * q + 1];
+        }
+        any_guess |= n > 0;
+        continue;
+      }
+      for (int q = 0; q < nt && ok; ++q) {
+        const int sf = S->guess_fsrc[(size_t)f * nt + q];
+        if (sf < 0) break;
+        const int sg = S->m_g0[m] + (sf - S->m_fs0[m]) * S->m_ncb[m] + S->h_gcb[g];
+        if (sg >= g1) ok = false;   // solved after this chunk: no guess (earlier chunks are read from x)
+        else {
+          gsrc[(size_t)g * nt + n] = sg;
+          gw[2 * ((size_t)g * nt + n)] = S->guess_fw[2 * ((size_t)f * nt + q)];
+          gw[2 * ((size_t)g * nt + n) + 1] = S->guess_fw[2 * ((size_t)f * nt + q) + 1];
+          ++n;
+        }
+      }
+      if (!ok) {
+        for (int q = 0; q < nt; ++q) gsrc[(size_t)g * nt + q] = -1;
+        n = 0;
+      }
+      for (int q = 0; q < n; ++q) S->guess_self |= gsrc[(size_t)g * nt + q] == g;
+      any_guess |= n > 0;
+    }
+    // levels: longest chain of other-frequency sources
+    for (int round = 0, changed = 1; changed; ++round) {
+      DP_ARG(round <= g1 - g0 + 1, "initial-guess terms form a cycle");
+      changed = 0;
+      for (int g = g0; g < g1; ++g) {
+        int L = 0;
+        for (int q = 0; q < nt; ++q) {
+          const int sg = gsrc[(size_t)g * nt + q];
+          if (sg < 0) break;
+          if (sg != g && sg >= g0) L = std::max(L, level[sg] + 1);
+        }
+        if (L != level[g]) {
+          level[g] = L;
+          changed = 1;
+        }
+      }
+    }
+    std::vector<int> lv(level.begin() + g0, level.begin() + g1);
+    std::sort(lv.begin(), lv.end());
+    lv.erase(std::unique(lv.begin(), lv.end()), lv.end());
+    std::vector<dogpup_solver::Phase> ph;
+    for (int L : lv) {
+      dogpup_solver::Phase p{L, (int)pwg.size(), 0, false, 0};
+      for (int g = g0; g < g1; ++g) {
+        if (level[g] != L) continue;
+        for (int w = S->h_gibeg[g]; w < gend(g); ++w) {
+          pwg.push_back(S->h_wgrp[w]);
+          pwt.push_back(S->h_wtile[w]);
+          pwp.push_back(S->h_wpart[w]);
+        }
+        p.nw += gend(g) - S->h_gibeg[g];
+        p.guess |= gsrc[(size_t)g * nt] >= 0;
+        p.nodes += S->h_grows[g];
+      }
+      ph.push_back(p);
+    }
+    if (!any_guess) ph.assign(1, dogpup_solver::Phase{0, S->h_gibeg[g0], gend(g1 - 1) - S->h_gibeg[g0], false,
+                                                       group_nodes(S, g0, g1)});
+    S->chunk_phases.push_back(ph);
+  }
+  DP_ARG((int)pwg.size() == S->nwork, "phase work list lost items");
+  DP_TRY(upload(S->t_pwgrp, pwg.data(), pwg.size() * 4));
+  DP_TRY(upload(S->t_pwtile, pwt.data(), pwt.size() * 4));
+  DP_TRY(upload(S->t_pwpart, pwp.data(), pwp.size() * 4));
+  DP_TRY(upload(S->t_gsrc, gsrc.data(), gsrc.size() * 4));
+  DP_TRY(upload(S->t_gw, gw.data(), gw.size() * 8));
+  DP_TRY(upload(S->t_glevel, level.data(), level.size() * 4));
   S->clear_graphs();
   return DOGPUP_OK;
 }
@@ -2722,6 +3090,7 @@ static int create_batch(dogpup_solver_t* out, int nmesh, const int* h_row0, cons
   DP_ARG(d_rptrA && d_cptrA && d_rptrG && d_cptrG, "NULL pattern");
   std::unique_ptr<dogpup_solver> S(new dogpup_solver());
   S->nmesh = nmesh; S->nn = nn; S->nsp = nsp; S->mode = mode;
+  if (mode == DOGPUP_MODE_C128_PLAIN) S->method = DOGPUP_METHOD_BICGSTAB;
   S->nnzA = nnzA; S->nnzG = nnzG;
   S->rA0 = d_rptrA; S->cA0 = d_cptrA; S->rG0 = d_rptrG; S->cG0 = d_cptrG;
   // Row tile of the kernels (a multiple of 64 rows).
@@ -2908,6 +3277,8 @@ static int create_batch(dogpup_solver_t* out, int nmesh, const int* h_row0, cons
   if (const char* e = getenv("DOGPUP_HALO_HALVES")) S->halo_halves = atoi(e) != 0;
   if (const char* e = getenv("DOGPUP_HALO_INTERLEAVE")) S->halo_ilv = atoi(e) != 0;
   if (const char* e = getenv("DOGPUP_CG_RFREE")) S->cg_rfree = atoi(e) != 0;
+  S->freq_cont = mode == DOGPUP_MODE_C64_REFINE;
+  if (const char* e = getenv("DOGPUP_FREQ_CONT")) S->freq_cont = S->freq_cont && atoi(e) != 0;
   DP_TRY(S->t_gfsc.ensure((size_t)g * 8));
   DP_TRY(S->fscale.ensure((size_t)fs * 8));
   DP_TRY(build_work(S.get()));
@@ -2967,6 +3338,8 @@ int dogpup_solver_set_matrix(dogpup_solver_t S, const double* d_valA_re, const d
   DP_HIP(hipStreamSynchronize(st));
   for (int g = 0; g < S->ngroups; ++g) gf[g] = hf[S->m_fs0[S->h_gmesh[g]] + S->h_gf[g]];
   DP_HIP(hipMemcpy(S->t_gfsc.p, gf.data(), gf.size() * 8, hipMemcpyHostToDevice));
+  S->h_fscale = hf;
+  if (S->freq_cont) build_freq_cont(S);
   // FSAI factors in the reference node order (pattern = the caller's tril(A)),
   // one launch per mesh: mesh i's values for frequency f at
   // m_gbase[i] + f*nnzG_i + (p - pG0_i)
@@ -3058,7 +3431,10 @@ int dogpup_solver_set_method(dogpup_solver_t S, int method) {
   DP_ARG(method == DOGPUP_METHOD_BICGSTAB || S->mode != DOGPUP_MODE_C128_PLAIN,
          "the reference-structure form is BiCGSTAB only");
   if (method != S->method) S->clear_graphs();
+  const int nv_old = nvec_for(S->mode, S->method);
   S->method = method;
+  // the refinement form's COCG carves half the vectors: re-plan the chunks
+  if (nvec_for(S->mode, method) != nv_old) DP_TRY(build_work(S));
   return DOGPUP_OK;
 }
 
@@ -3080,6 +3456,57 @@ int dogpup_solver_set_rhs_tol(dogpup_solver_t S, const double* h_tol, int n) {
     }
   DP_TRY(upload(S->rhs_tol, t.data(), t.size() * 8));
   S->have_rhs_tol = true;
+  return DOGPUP_OK;
+}
+
+int dogpup_solver_set_guess(dogpup_solver_t S, int nterm, const int* h_src, const double* h_w) {
+  DP_ARG(S != nullptr, "NULL solver");
+  DP_ARG(nterm >= 0 && nterm <= kMaxGuessTerms, "nterm must be 0..%d (got %d)", kMaxGuessTerms, nterm);
+  S->freq_cont = false;
+  S->phases_dirty = true;
+  S->clear_graphs();
+  if (nterm == 0) {
+    S->guess_nterm = 0;
+    S->guess_fsrc.clear();
+    S->guess_fw.clear();
+    return DOGPUP_OK;
+  }
+  DP_ARG(S->mode == DOGPUP_MODE_C64_REFINE, "initial guesses are for the refinement form");
+  DP_ARG(h_src && h_w, "NULL guess terms");
+  std::vector<int> fmesh(S->nf_total);
+  for (int i = 0; i < S->nmesh; ++i)
+    for (int f = 0; f < S->m_nf[i]; ++f) fmesh[S->m_fs0[i] + f] = i;
+  for (int f = 0; f < S->nf_total; ++f)
+    for (int q = 0; q < nterm; ++q) {
+      const int sf = h_src[(size_t)f * nterm + q];
+      if (sf < 0) break;
+      DP_ARG(sf < S->nf_total && fmesh[sf] == fmesh[f], "guess term %d of frequency %d names frequency %d of another mesh",
+             q, f, sf);
+      DP_ARG(std::isfinite(h_w[2 * ((size_t)f * nterm + q)]) && std::isfinite(h_w[2 * ((size_t)f * nterm + q) + 1]),
+             "guess weight %d of frequency %d is not finite", q, f);
+    }
+  S->guess_nterm = nterm;
+  S->guess_fsrc.assign(h_src, h_src + (size_t)S->nf_total * nterm);
+  S->guess_fw.assign(h_w, h_w + (size_t)S->nf_total * nterm * 2);
+  return DOGPUP_OK;
+}
+
+int dogpup_solver_set_freq_continuation(dogpup_solver_t S, int on, const double* h_spectrum) {
+  DP_ARG(S != nullptr, "NULL solver");
+  DP_ARG(!on || S->mode == DOGPUP_MODE_C64_REFINE, "frequency continuation is for the refinement form");
+  if (h_spectrum) {
+    for (int f = 0; f < 2 * S->nf_total; ++f) DP_ARG(std::isfinite(h_spectrum[f]), "spectrum entry %d is not finite", f / 2);
+    S->h_spec.assign(h_spectrum, h_spectrum + 2 * (size_t)S->nf_total);
+  } else {
+    S->h_spec.clear();
+  }
+  S->guess_nterm = 0;
+  S->guess_fsrc.clear();
+  S->guess_fw.clear();
+  S->freq_cont = on != 0;
+  S->phases_dirty = true;
+  S->clear_graphs();
+  if (S->freq_cont && !S->h_fscale.empty()) build_freq_cont(S);
   return DOGPUP_OK;
 }
 

@@ -53,7 +53,7 @@ def tg_coefficients(optode, gate_fc, dxyz):
 
 
 def jacobian_tg(mesh, gate_fc, mode="mixed", tol=1e-10, dxyz=None, scale=None, field=None,
-                method="bicgstab", adj_tol=None):
+                method=None, adj_tol=None):
     """J_TG as float32 [NV][NG][NM] (= the reference's NM x NG x NV,
     column-major); NV = nodes, or in-mesh voxels when ``mesh.basis`` is set.
     ``field``: a forward+adjoint FluenceField from

@@ -82,7 +82,7 @@ def _forward(mesh, gate_fc, mode, tol, method, adj_tol=None):
 
 
 def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed", tol=1e-8,
-                   max_iter=10, spot_fwhm=1.5, verbose=False, method="bicgstab", adj_tol=None):
+                   max_iter=10, spot_fwhm=1.5, verbose=False, method=None, adj_tol=None):
     """wghtd_LM_recon(mesh, data_m, gates, pos_s, lam): returns LMResult with
     the reconstructed nodal mua.  data_m: NM x NG measured TG data; gates:
     NG x NT gating functions on the optode's time axis.  mode/method: the
@@ -180,7 +180,7 @@ def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed
     return res
 
 
-def simulate_tg_data(mesh, gates, mode="fp64", tol=1e-10, peak_counts=None, seed=0, method="cocg"):
+def simulate_tg_data(mesh, gates, mode="fp64", tol=1e-10, peak_counts=None, seed=0, method=None):
     """Time-gated boundary data of ``mesh`` (NM x NG): fc2tg(meas_flu(phi))
     of a forward solve, with Poisson noise when ``peak_counts`` is given (the
     data scaled so that their maximum is peak_counts photons, drawn, and

@@ -34,6 +34,12 @@ def _f64(a):
     return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=DEV)
 
 
+def default_method(mode: str) -> str:
+    """COCG for the fused and mixed modes, BiCGSTAB (the reference's, the
+    only one it has) for the reference-structure mode."""
+    return "bicgstab" if MODES[mode] == _lib.MODE_C128_PLAIN else "cocg"
+
+
 def group_width(ncols: int, mode: str = "mixed") -> int:
     """Default columns per solver group, measured on the 269k-node,
     32-source workload:
@@ -258,29 +264,39 @@ class DMesh:
         return _i32(nodes), _f64(w)
 
     # -- solver -----------------------------------------------------------------
-    def solver(self, ncols, mode="mixed", method="bicgstab"):
+    def solver(self, ncols, mode="mixed", method=None):
         key = (ncols, mode)
         s = self._solvers.get(key)
         if s is None:
             s = FluenceSolver(self, ncols, MODES[mode], nsp=group_width(ncols, mode))
             self._solvers[key] = s
-        s.set_method(method)
+        s.set_method(default_method(mode) if method is None else method)
         return s
 
     def solve_point_sources(self, which="s", mode="mixed", tol=None, maxit=1000, extra=None,
-                            method="bicgstab", adj_tol=None):
+                            method=None, adj_tol=None, cont=None):
         """Solve for the sources ('s'), detectors ('d') or both ('sd', forward
         and adjoint batched in one launch: columns 0..NS-1 sources,
-        NS..NS+ND-1 detectors).  method: 'bicgstab' (the reference's) or
-        'cocg' (conjugate orthogonal CG, same solution).  adj_tol: relative
-        residual of the detector (adjoint) columns when it differs from tol
-        (fused and mixed modes), as separate flu_solve / adj_flu_solve calls
-        with two tolerances would have."""
+        NS..NS+ND-1 detectors).  method: 'cocg' (conjugate orthogonal CG, the
+        default of the fused and mixed modes) or 'bicgstab' (the reference's,
+        the only method of the 'plain' mode); both converge to the same
+        solution.  adj_tol: relative residual of the detector (adjoint)
+        columns when it differs from tol (fused and mixed modes), as separate
+        flu_solve / adj_flu_solve calls with two tolerances would have.
+        cont (mixed mode): frequency continuation, each frequency of a later
+        level started from the interpolation of neighbouring frequencies
+        solved before it, normalised by the source spectrum s_fpsf that every
+        right-hand side carries (the default, True); False starts every
+        right-hand side from 0, as the reference does."""
         parts = [self._bary_dev(w) for w in which]
         nodes = torch.cat([p[0] for p in parts])
         w = torch.cat([p[1] for p in parts])
         ncols = nodes.shape[0]
         S = self.solver(ncols, mode, method)
+        if MODES[mode] == _lib.MODE_C64_REFINE:
+            S.set_continuation(True if cont is None else cont, self.optode.s_fpsf)
+        elif cont:
+            raise ValueError("frequency continuation is for the mixed mode")
         spec = torch.as_tensor(self.optode.s_fpsf, dtype=torch.complex128, device=DEV)
         b = S.empty_block(zero=False)               # dogpup_point_sources clears it
         _lib.call("dogpup_point_sources", _lib.ptr(torch.view_as_real(b)), self.nn, S.nf, ncols, S.nsp,
@@ -337,7 +353,10 @@ class FluenceSolver:
         self.h = h
         self._version = None
         self._col_tol = False
-        self.method = "bicgstab"
+        self.method = "cocg" if mode != _lib.MODE_C128_PLAIN else "bicgstab"
+        # continuation state: the library's default (on, flat spectrum, in the
+        # refinement mode)
+        self._cont_key = (mode == _lib.MODE_C64_REFINE, None)
 
     def set_method(self, method: str):
         """Krylov method of the fused modes: 'bicgstab' or 'cocg'."""
@@ -345,6 +364,23 @@ class FluenceSolver:
             raise ValueError(f"unknown method {method!r}; expected one of {sorted(METHODS)}")
         _lib.call("dogpup_solver_set_method", self.h, METHODS[method])
         self.method = method
+
+    def set_continuation(self, on: bool, spectrum=None):
+        """Frequency continuation of the mixed mode
+        (dogpup_solver_set_freq_continuation); spectrum: the complex factor
+        per frequency that every right-hand side carries (None: flat)."""
+        on = bool(on)
+        spec = None if spectrum is None else np.ascontiguousarray(np.asarray(spectrum, np.complex128).reshape(-1))
+        key = (on, None if spec is None else spec.tobytes())
+        if key == self._cont_key:
+            return
+        if on and self.mode != _lib.MODE_C64_REFINE:
+            raise ValueError("frequency continuation is for the mixed (refinement) mode")
+        if spec is not None and spec.size != self.nf:
+            raise ValueError(f"spectrum needs {self.nf} values, got {spec.size}")
+        _lib.call("dogpup_solver_set_freq_continuation", self.h, int(on),
+                  None if spec is None else _lib.ptr(spec.view(np.float64)))
+        self._cont_key = key
 
     def __del__(self):
         try:

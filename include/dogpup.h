@@ -51,8 +51,8 @@ extern "C" {
 #define DOGPUP_MODE_C128_PLAIN 2    /* complex128, reference structure: unfused passes, lockstep */
 
 /* Krylov method of the fused forms (DOGPUP_MODE_C128, DOGPUP_MODE_C64_REFINE) */
-#define DOGPUP_METHOD_BICGSTAB 0    /* the reference's BiCGSTAB (dMesh.m:1247-1348), the default */
-#define DOGPUP_METHOD_COCG 1        /* conjugate orthogonal CG: CG with the unconjugated form */
+#define DOGPUP_METHOD_BICGSTAB 0    /* the reference's BiCGSTAB (dMesh.m:1247-1348); the plain form's only one */
+#define DOGPUP_METHOD_COCG 1        /* conjugate orthogonal CG: CG with the unconjugated form; the fused forms' default */
 
 const char* dogpup_last_error(void);
 int dogpup_version(void);
@@ -207,6 +207,39 @@ int dogpup_solver_set_method(dogpup_solver_t s, int method);
  * (dMesh.m:106-202) would with two tolerances; no counterpart in the
  * reference, whose gpuBicstab_FSAIP hard-codes one (dMesh.m:1243). */
 int dogpup_solver_set_rhs_tol(dogpup_solver_t s, const double* h_tol, int n);
+
+/* Initial guesses of the following solves (DOGPUP_MODE_C64_REFINE): the
+ * first FP64 iterate of right-hand side (f, col) is
+ * sum_q w_q * x(h_src[f*nterm + q], col), w_q the complex weight
+ * (h_w[2*(f*nterm + q)], h_w[2*(f*nterm + q) + 1]), where x(f', col) is the
+ * converged solution at frequency f' (same mesh, same column) of the same
+ * solve, or, for h_src = f itself, what d_x holds for (f, col) on entry to
+ * dogpup_solver_solve (a warm start from an earlier solve; d_x must then be
+ * finite).  h_src = -1 ends a frequency's terms; frequencies are global over
+ * the handle's meshes, as in d_fscale.  The solve runs in levels: a
+ * frequency without terms naming other frequencies is in level 0, any other
+ * one level after the latest frequency it names, and each level is solved
+ * to convergence before the next starts.  Convergence is on the same
+ * ||b - A x|| / ||b|| <= tol.  nterm = 0 clears the guesses (x0 = 0); 1..8
+ * terms.  No counterpart in the reference, which starts every RHS from
+ * x0 = 0 (dMesh.m:1283). */
+int dogpup_solver_set_guess(dogpup_solver_t s, int nterm, const int* h_src, const double* h_w);
+
+/* Frequency continuation (on by default in DOGPUP_MODE_C64_REFINE, with a
+ * flat spectrum; the environment variable DOGPUP_FREQ_CONT=0 turns the
+ * default off): guesses built from the frequency scales of
+ * dogpup_solver_set_matrix.  Frequency k of a mesh is in level 0 if
+ * k % 8 == 0 or k is the last, level 1 if k % 8 == 4, level 2 if k % 4 == 2
+ * and level 3 otherwise, and a later-level frequency starts from the
+ * Lagrange interpolation, in the frequency scale, of the four frequencies of
+ * earlier levels nearest to it.  h_spectrum (NULL, or nf_total complex
+ * values): the right-hand side of frequency f is h_spectrum[f] times a
+ * vector that does not depend on f, as for point sources, whose right-hand
+ * sides carry the source spectrum s_fpsf (dMesh.m:128,196); the
+ * interpolation then runs on x / spectrum, which is smooth in the frequency
+ * where x is not.  on = 0 solves every RHS from x0 = 0, as the reference
+ * does; either call clears guesses set with dogpup_solver_set_guess. */
+int dogpup_solver_set_freq_continuation(dogpup_solver_t s, int on, const double* h_spectrum);
 
 /* Counters of the last solve: number of vector-kernel passes weighted by the
  * columns they moved, for roofline accounting.  h_out[0] = BiCGSTAB
