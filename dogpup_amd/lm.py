@@ -64,6 +64,10 @@ class LMResult:
     errDel: list = field(default_factory=list)
     lam: list = field(default_factory=list)
     exit: str = ""
+    # per update: the branch taken ('accept' / 'step back') and the nodal mua
+    # after it
+    branch: list = field(default_factory=list)
+    mua_hist: list = field(default_factory=list)
     # seconds per iteration: forward+adjoint solve, J-TG and data; the dense
     # algebra (reshape_J, tik_inv)
     t_forward: list = field(default_factory=list)
@@ -82,13 +86,16 @@ def _forward(mesh, gate_fc, mode, tol, method, adj_tol=None):
 
 
 def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed", tol=1e-8,
-                   max_iter=10, spot_fwhm=1.5, verbose=False, method=None, adj_tol=None):
+                   max_iter=10, spot_fwhm=1.5, verbose=False, method=None, adj_tol=None, forward=None):
     """wghtd_LM_recon(mesh, data_m, gates, pos_s, lam): returns LMResult with
     the reconstructed nodal mua.  data_m: NM x NG measured TG data; gates:
     NG x NT gating functions on the optode's time axis.  mode/method: the
     solver form of the forward+adjoint solves ('plain' is BiCGSTAB only);
     adj_tol: tolerance of the adjoint solves, which feed only the Jacobian
-    (default tol; the fused and mixed forms)."""
+    (default tol; the fused and mixed forms).  forward (optional): a wrapper
+    f(mesh, run) -> (data NM x NG, J rows x NV) around run(mesh), the GPU
+    forward of this function (J_complex and fc2tg of one batched
+    forward+adjoint solve), called once per iteration."""
     opt = mesh.optode
     data_m = np.asarray(data_m, float)
     incl = id_thresh(torch.as_tensor(data_m), 0.8).cpu().numpy().ravel(order="F")   # :25-26
@@ -124,7 +131,8 @@ def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed
     while True:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        yc, J = _forward(mesh, gate_fc, mode, tol, method, adj_tol)                # :58-61
+        run = lambda m: _forward(m, gate_fc, mode, tol, method, adj_tol)          # noqa: E731
+        yc, J = run(mesh) if forward is None else forward(mesh, run)               # :58-61
         torch.cuda.synchronize()
         res.t_forward.append(time.perf_counter() - t0)
         yc = yc.cpu().numpy()
@@ -150,14 +158,18 @@ def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed
             res.exit = "converged"
             break
         res.count += 1
-        if res.count == 1 or np.sign(res.errDel[-1]) == -1:                       # :101-106
+        if res.count == 1 or np.sign(res.errDel[-1]) == -1:                       # :103-107
             G = to_grid(mesh.mua)
+            Jg = J * torch.as_tensor(G, dtype=torch.float32, device=DEV)[None, :]  # J .* G'
             lam1 = lam1 / k
-        else:                                                                       # :107-116
+            res.branch.append("accept")
+        else:                                                                       # :108-116
+            # the previous values: y and the J .* G' of the last accepted
+            # step; G (of the accepted mua) only rescales this update
             G = to_grid(mesh.mua - d_mu)
-            y, J = y_prev, J_prev
+            y, Jg = y_prev, J_prev
             lam1 = lam1 * k
-        Jg = J * torch.as_tensor(G, dtype=torch.float32, device=DEV)[None, :]     # J .* G'
+            res.branch.append("step back")
         if lam1 < res.err[-1]:                                                     # :119-121
             lam1 = res.err[-1]
         res.lam.append(lam1)
@@ -172,10 +184,11 @@ def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed
             res.exit = "negative mua"
             break
         mesh.update_properties(np.stack([new_mu, mesh.musp], 1))                   # :155
+        res.mua_hist.append(mesh.mua.copy())
         if res.count > max_iter - 1:                                               # :159-162
             res.exit = "max iterations"
             break
-        y_prev, J_prev = y, J                                                      # :165-166
+        y_prev, J_prev = y, Jg                                                     # :165-166
     res.mua = mesh.mua.copy()
     return res
 

@@ -49,3 +49,73 @@ def test_lm_reconstruction_mixed_and_fp64():
     a, b = out["mixed"].mua, out["fp64"].mua
     assert np.linalg.norm(a - b) / np.linalg.norm(b - 0.01 + 1e-30) < 1e-2
     assert out["mixed"].count == out["fp64"].count
+
+
+def test_lm_matches_oracle_with_two_step_backs():
+    """The LM loop on the GPU path (FP64 solves, J-TG on MFMA, data-space
+    tik_inv and reshape_J) against the oracle's restatement of
+    wghtd_LM_recon.m:23-168 with its FP64 FEM forward model (direct solves,
+    J_complex, fc2tg), on the same time-gated data.  Both forward models are
+    wrapped so that the third and fourth evaluations return data scaled by
+    3 and 6: the misfit rises twice in a row, so the loop takes the
+    step-back branch (:108-116) twice in a row, reusing the cached J .* G'
+    and y of the last accepted step.  errPerc, the branch, lambda and mua
+    are compared after every update."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dogpup_amd import lm, meshgen, mesh as dm
+    from oracle import fem, lm as olm, optode as oopt, transforms as otr
+    m = meshgen.slab((24.0, 24.0, 12.0), 2.0, 0.01, 1.0, 1.4, jitter=0.2, seed=1,
+                     inclusions=[((12.0, 12.0, 7.0), 3.0, 0.1)])
+    t = np.arange(0, 1001) * 20e-12
+    nf = 64
+    src = meshgen.top_grid(m.extent, 3, 3, 5.0, 5.0, 7.0, 7.0)
+    det = meshgen.top_grid(m.extent, 3, 3, 8.5, 8.5, 3.5, 3.5)
+    link = meshgen.all_pairs(9, 9)
+    tpsf = meshgen.gaussian_tpsf(t, 250e-12)
+    gates = meshgen.boxcar_gates(t, 8, 500e-12, 200e-12, start=500e-12)
+    f, df = otr.freq_axis(t, nf)
+    _, s_fpsf = otr.source_spectrum(tpsf, t, f, 1e3)
+    gate_fc = otr.td2fc(gates, f, t, 1)
+    print("[case] LM parity: oracle data", flush=True)
+    om_true = fem.make_mesh(m.node, m.elem, m.nr, m.mua, m.musp)
+    snap = oopt.snap2mesh(om_true, src, det)
+    data_m, _ = olm.fem_forward(om_true, snap, link, s_fpsf, gate_fc, f, df, t.size, m.h)(m.mua)
+    mua0 = np.full(m.node.shape[0], 0.01)
+    pos_s = np.array([[12.0, 12.0, 7.0], [6.0, 6.0, 8.0], [18.0, 18.0, 8.0]])
+    lam = (10.0, 0.1)
+    scale = {2: 3.0, 3: 6.0}                 # forward call -> data factor
+
+    def scripted(fwd):
+        calls = [0]
+
+        def wrapped(*a):
+            y, J = fwd(*a)
+            s = scale.get(calls[0], 1.0)
+            calls[0] += 1
+            return y * s, J
+        return wrapped
+
+    print("[case] LM parity: oracle loop", flush=True)
+    om = fem.make_mesh(m.node, m.elem, m.nr, mua0, m.musp)
+    fo = olm.fem_forward(om, oopt.snap2mesh(om, src, det), link, s_fpsf, gate_fc, f, df, t.size, m.h)
+    ref = olm.wghtd_lm_recon(scripted(fo), mua0, data_m, olm.target_spots(m.node, pos_s, 1.5, "gauss"), lam,
+                             max_iter=6)
+    print("[case] LM parity: GPU loop", flush=True)
+    D = dm.DMesh(m.node, m.elem, m.nr, np.stack([mua0, m.musp], 1))
+    D.add_optode(dm.DOptode(src, det, link, 1e3, tpsf, t, nf))
+    D.dxyz = m.h
+    gw = scripted(lambda mesh, run: run(mesh))
+    r = lm.wghtd_lm_recon(D, data_m, gates, pos_s, lam, mode="fp64", tol=1e-12, max_iter=6,
+                          forward=lambda mesh, run: gw(mesh, run))
+    print("oracle:", ref["branch"], [f"{e:.4f}" for e in ref["errPerc"]], ref["exit"], flush=True)
+    print("GPU:   ", r.branch, [f"{e:.4f}" for e in r.errPerc], r.exit, flush=True)
+    assert ref["branch"][:4] == ["accept", "accept", "step back", "step back"]
+    assert r.branch == ref["branch"] and r.exit == ref["exit"] and r.count == ref["count"]
+    assert np.allclose(r.errPerc, ref["errPerc"], rtol=1e-5, atol=0)
+    assert np.allclose(r.lam, ref["lam"], rtol=1e-5, atol=0)
+    for i, (a, b) in enumerate(zip(r.mua_hist, ref["mua_hist"])):
+        err = np.linalg.norm(a - b) / np.linalg.norm(b - mua0)
+        print(f"update {i + 1}: mua rel diff (to the update size) {err:.2e}", flush=True)
+        assert err < 1e-4, (i, err)
+    assert np.linalg.norm(r.mua - ref["mua"]) <= 1e-4 * np.linalg.norm(ref["mua"] - mua0)

@@ -97,6 +97,46 @@ def test_compat_smv_fsaimv(setup):
     assert rel(got3, refG) < 1e-14
 
 
+def test_compat_device_launchers_fsaimv_fsaip(setup):
+    """The device launchers behind FSAImv_mex and FSAIP_gen_mex
+    (FSAImv_call, FSAImvMex.hpp:4-15 / FSAImv_CUDA.cu:8-53, and FSAIP_call,
+    FSAIP.hpp:4-13 / FSAIP_CUDA.cu:162-180), called directly on planar device
+    arrays in the reference layout, against the oracle: y(:,c,k) = G_k x(:,c,k)
+    with G_k at valG[nnz*k + i], and the factors of every frequency at
+    out[k + nfreqs*j]."""
+    from dogpup_amd import _lib
+    from oracle import fsai as ofsai, solver as osol
+    A = setup["A"].tocsr()
+    A.sort_indices()
+    n, nf, ns = A.shape[0], 3, 4
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(n, ns, nf)) + 1j * rng.normal(size=(n, ns, nf))
+    rG, cG, valG = ofsai.fsaip_gen(A, nf)                # per-row partial-pivot GE (the reference's)
+    nnz = cG.size
+    ref = osol.fsaimv(rG, cG, valG, X)
+    xr = dev(np.transpose(X, (2, 1, 0)).real.copy())
+    xi = dev(np.transpose(X, (2, 1, 0)).imag.copy())
+    gv = np.ascontiguousarray(valG.T)                     # [k][i]: nnz*k + i
+    yr, yi = torch.empty_like(xr), torch.empty_like(xi)
+    _lib.call("dogpup_fsaimv_call", _lib.ptr(yr), _lib.ptr(yi), _lib.ptr(xr), _lib.ptr(xi), _lib.ptr(dev(gv.real)),
+              _lib.ptr(dev(gv.imag)), _lib.ptr(dev(rG, torch.int32)), _lib.ptr(dev(cG, torch.int32)), nnz, n, ns, nf,
+              None)
+    torch.cuda.synchronize()
+    got = np.transpose(yr.cpu().numpy() + 1j * yi.cpu().numpy(), (2, 1, 0))
+    assert rel(got, ref) < 1e-14
+    # FSAIP_call: tril(A) in CSR, factors of k = 0..nf-1
+    At = sp.tril(A).tocsr()
+    At.sort_indices()
+    outr = torch.full((nf * nnz,), float("nan"), dtype=torch.float64, device="cuda")
+    outi = torch.full_like(outr, float("nan"))
+    _lib.call("dogpup_fsaip_call", _lib.ptr(outr), _lib.ptr(outi), _lib.ptr(dev(At.data.real)),
+              _lib.ptr(dev(At.data.imag)), _lib.ptr(dev(At.indptr, torch.int32)), _lib.ptr(dev(At.indices, torch.int32)),
+              _lib.ptr(dev(rG, torch.int32)), _lib.ptr(dev(cG, torch.int32)), nf, n, None)
+    torch.cuda.synchronize()
+    got = (outr.cpu().numpy() + 1j * outi.cpu().numpy()).reshape(nnz, nf)        # out[k + nf*j]
+    assert rel(got, valG) < 1e-13
+
+
 def test_compat_fsaip_gen(setup):
     """FSAIP_gen_mex vs the oracle's per-row partial-pivot GE (brute force)."""
     from dogpup_amd import _lib
