@@ -80,7 +80,9 @@ def tune(meshes, which="s", mode="mixed", method="cocg", iters=6, widths=WIDTHS,
             continue
         B = None
         try:
-            B = MeshBatch(meshes, which, mode=mode, nsp=nsp, method=method)
+            # every RHS from 0 while timing: the per-iteration cost, not the
+            # continuation's iteration savings, is what the shapes change
+            B = MeshBatch(meshes, which, mode=mode, nsp=nsp, method=method, cont=False)
             B.ensure_matrix()
             for tile in tiles:
                 for u in unrolls:

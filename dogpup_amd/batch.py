@@ -30,11 +30,14 @@ class MeshBatch:
 
     ``meshes``: DMesh instances with optodes.  ``which``: 's', 'd' or 'sd' as
     in DMesh.solve_point_sources.  ``nsp``: columns per group (8, 16, 32 or
-    64; default: default_width, the width with the least padded work).  ``tuning``: (tile_rows, unroll_a, unroll_g, graph_k), e.g. from
-    dogpup_amd.autotune.
+    64; default: default_width, the width with the least padded work).
+    ``tuning``: (tile_rows, unroll_a, unroll_g, graph_k), e.g. from
+    dogpup_amd.autotune.  ``cont`` (mixed mode): frequency continuation
+    within each mesh, normalised by each mesh's source spectrum (the
+    right-hand sides of rhs() carry it); False starts every RHS from 0.
     """
 
-    def __init__(self, meshes, which="s", mode="mixed", nsp=None, method="cocg", tuning=None):
+    def __init__(self, meshes, which="s", mode="mixed", nsp=None, method="cocg", tuning=None, cont=True):
         self.meshes = list(meshes)
         if not self.meshes:
             raise ValueError("empty batch")
@@ -77,6 +80,11 @@ class MeshBatch:
         self.set_method(method)
         if tuning is not None:
             self.set_tuning(*tuning)
+        if MODES[mode] == _lib.MODE_C64_REFINE:
+            spec = np.ascontiguousarray(np.concatenate([np.asarray(m.optode.s_fpsf, np.complex128).reshape(-1)
+                                                        for m in self.meshes]))
+            _lib.call("dogpup_solver_set_freq_continuation", self.h, int(bool(cont)),
+                      _lib.ptr(spec.view(np.float64)) if cont else None)
         self._versions = None
         # block of each mesh in the batch vectors: nf*ncb groups of nn rows
         sizes = [f * b * n * self.nsp for f, b, n in zip(self.nf, self.ncb, nn)]

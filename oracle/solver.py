@@ -30,6 +30,14 @@ def bicgstab_fsai(A_full: sp.csr_matrix, Q, tol=1e-12, maxit=1000, fsai=None):
     GTs = [G.T.tocsr() for G in Gs]
     As = [(A_full.real + 1j * k * A_full.imag).tocsr() for k in range(nf)]
     b = np.transpose(Q, (0, 2, 1)).astype(complex)           # nodes x sources x freqs
+    x, it, err = bicgstab_core(As, Gs, GTs, b, tol, maxit)
+    return np.transpose(x, (0, 2, 1)), it, err
+
+
+def bicgstab_core(As, Gs, GTs, b, tol=1e-12, maxit=1000):
+    """The iteration of dMesh.m:1283-1341 on given matrices A_k, G_k, G_k^T
+    (lists over frequencies) for b (NN x NS x NF); returns (x, iterations,
+    largest relative residual)."""
     x = np.zeros_like(b)
     r = b - _apply(As, x)
     err0 = np.sqrt((np.abs(r) ** 2).sum(axis=0))
@@ -63,7 +71,7 @@ def bicgstab_fsai(A_full: sp.csr_matrix, Q, tol=1e-12, maxit=1000, fsai=None):
         beta = rho / rho0 * (1.0 / (alph * om))
         rho0 = rho
         p = r + beta * p
-    return np.transpose(x, (0, 2, 1)), it, err
+    return x, it, err
 
 
 def direct(A_full: sp.csr_matrix, Q):

@@ -46,6 +46,8 @@ def test_batch_equals_single_mesh_solves(mode, method):
         assert (f.status == 0).all(), f.status
         S = dm.FluenceSolver(D, f.ncols, dm.MODES[mode], nsp=8)
         S.set_method(method)
+        if mode == "mixed":
+            S.set_continuation(True, D.optode.s_fpsf)          # as the batch does for each mesh
         nodes, w = D._bary_dev("s")
         spec = torch.as_tensor(D.optode.s_fpsf, dtype=torch.complex128, device="cuda")
         b = S.empty_block()
