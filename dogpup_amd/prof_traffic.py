@@ -1,19 +1,21 @@
-"""HBM traffic per kernel launch from two rocprofv3 PMC runs.
+"""HBM traffic of the solver's kernel classes from two rocprofv3 PMC runs.
 
 Usage (after ``rocprofv3 --pmc FETCH_SIZE`` and ``rocprofv3 --pmc WRITE_SIZE``
-runs of the same ``bench.py`` command, each writing ``*_counter_collection.csv``):
+runs of the same ``bench.py`` command, each writing ``*_counter_collection.csv``
+and the bench's JSON line):
 
-    python -m dogpup_amd.prof_traffic FETCH.csv WRITE.csv --method cocg \
-        --json profiles/traffic_cocg_c64_slab269k_2048rhs.json
+    python -m dogpup_amd.prof_traffic FETCH.csv WRITE.csv --bench-json BENCH.json \
+        --solves 2 --json profiles/traffic_cocg_c64_slab269k_2048rhs.json
 
 FETCH_SIZE and WRITE_SIZE are in KiB per dispatch; each is scaled by what it
 reads for the bench's plain 2 GiB device copy (``__amd_rocclr_copyBuffer``),
-so that bytes = counter x factor.  For every solver kernel the dispatch with
-the largest traffic is taken (all frequency groups active) and compared with
-the algorithmic bytes of such a launch at the workload's sizes.  The
-per-class ratios (traffic / algorithmic summed over the kernels of a class:
-0 G/G^T SpMV, 1 A SpMV, 2 Krylov vector passes, 3 FP64 refinement) are what
-bench.py applies to its measured launches.
+so that bytes = counter x factor.  The traffic of every dispatch of a class's
+kernels (0 G/G^T SpMV, 1 A SpMV, 2 Krylov vector passes, 3 FP64 refinement)
+is summed over the run and divided by the algorithmic bytes the bench's
+profiled step counted for that class (``extra.kernel_classes``) times the
+solves the run made (the timed step and the profiled step: 2; their work is
+identical).  That per-class ratio is what bench.py applies to its measured
+launches.  The largest dispatch of every kernel is listed too.
 """
 from __future__ import annotations
 
@@ -49,40 +51,30 @@ def read_counter(path: str, counter: str):
     return rows
 
 
-def algorithmic(name: str, method: str, nodes: int, groups: int, nsp: int, nnz_a: int, nnz_g: int):
-    """(class, algorithmic bytes) of a full-activity launch of a complex64
-    solver kernel, or None for kernels outside the table."""
-    V = float(nodes) * nsp * 8 * groups              # one complex64 vector stream, all groups
-    MA = nnz_a * 12.0 + 4.0 * (nodes + 1)            # A: re, im (float) and column, once per launch
-    MG = groups * nnz_g * 12.0 + 4.0 * (nodes + 1)   # G: complex64 per frequency and column
-    m = (re.match(r"dogpup::k_spmv<float, (\d+), (\d), (\d)(?:, \d+)?>", name)
-         or re.match(r"dogpup::k_spmv_halo<(\d+), (\d), (\d)>", name)
-         or re.match(r"dogpup::k_spmv_halo2<()(\d), (\d)>", name))
+CLASS_NAMES = ["G/G^T SpMV", "A SpMV", "Krylov vector passes", "FP64 refinement passes"]
+
+
+def kernel_class(name: str):
+    """Class of a complex64 solver kernel (as the bench's profiled step
+    counts them), or None."""
+    m = (re.match(r"dogpup::k_spmv<float, \d+, (\d), \d+(?:, \d+)?>", name)
+         or re.match(r"dogpup::k_spmv_halo<\d+, (\d), \d+>", name)
+         or re.match(r"dogpup::k_spmv_halo2<(\d), \d+>", name))
     if m:
-        mat, epi = int(m.group(2)), int(m.group(3))
-        # vector streams per epilogue (STORE_DOT in COCG reads w = x, the
-        # gathered vector); AXPY_SELF: x in, y in and out; STORE_SELF2: x, y
-        streams = {0: 2, 1: 2 if method == "cocg" else 3, 2: 2, 3: 2, 4: 3, 5: 5, 6: 3, 7: 2}[epi]
-        return (1, streams * V + MA) if mat == 0 else (0, streams * V + MG)
-    table = {f"dogpup::k_cg_update<float, {nsp}>": (2, 3 * V), f"dogpup::k_update_xr<float, {nsp}>": (2, 8 * V),
-             f"dogpup::k_update_p<float, {nsp}>": (2, 4 * V), f"dogpup::k_axpy_s<float, {nsp}>": (2, 3 * V),
-             # x64 and b in complex128 (2 V each), r and x out in complex64 (+ p and rhat for
-             # BiCGSTAB), A in FP64
-             f"dogpup::k_residual64<{nsp}>": (3, (6 if method == "cocg" else 8) * V + nnz_a * 20.0
-                                              + 4.0 * (nodes + 1))}
-    return table.get(name)
+        return 1 if int(m.group(1)) == 0 else 0
+    if re.match(r"dogpup::k_(cg_update|update_xr|update_p|axpy_s)<float, \d+>", name):
+        return 2
+    if re.match(r"dogpup::k_(residual64|add_correction)<\d+>", name):
+        return 3
+    return None
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("fetch_csv")
     ap.add_argument("write_csv")
-    ap.add_argument("--method", default="cocg", choices=["cocg", "bicgstab"])
-    ap.add_argument("--nodes", type=int, default=269001)
-    ap.add_argument("--groups", type=int, default=64, help="frequency x column-block groups")
-    ap.add_argument("--nsp", type=int, default=32)
-    ap.add_argument("--nnz-a", type=int, default=3930201)
-    ap.add_argument("--nnz-g", type=int, default=2099601)
+    ap.add_argument("--bench-json", required=True, help="the bench line of one of the two PMC runs")
+    ap.add_argument("--solves", type=int, default=2, help="solves of the profiled command with the same work")
     ap.add_argument("--copy-bytes", type=float, default=float(2 << 30))
     ap.add_argument("--json", required=True)
     a = ap.parse_args(argv)
@@ -94,34 +86,42 @@ def main(argv=None):
         raise SystemExit("no 2 GiB copy dispatch found for the calibration")
     f_fac = a.copy_bytes / (sum(cf) / len(cf))       # bytes per FETCH_SIZE unit
     w_fac = a.copy_bytes / (sum(cw) / len(cw))
-    wmax = {}
-    for _, n, v in write:
-        wmax.setdefault(n, []).append(v)
+    with open(a.bench_json) as fh:
+        line = [ln for ln in fh.read().splitlines() if ln.startswith("{")][-1]
+    kc = json.loads(line)["extra"]["kernel_classes"]
     kernels, classes = {}, {}
-    for n in sorted({n for _, n, _ in fetch}):
-        alg = algorithmic(n, a.method, a.nodes, a.groups, a.nsp, a.nnz_a, a.nnz_g)
-        if alg is None:
-            continue
-        cls, alg_b = alg
-        fb = max(v for _, m, v in fetch if m == n) * f_fac
-        wb = max(wmax.get(n, [0.0])) * w_fac
-        kernels[n] = {"class": cls, "fetch_bytes": fb, "write_bytes": wb, "traffic_bytes": fb + wb,
-                      "algorithmic_bytes": alg_b, "ratio": (fb + wb) / alg_b}
-        c = classes.setdefault(str(cls), {"traffic_bytes": 0.0, "algorithmic_bytes": 0.0})
-        c["traffic_bytes"] += fb + wb
-        c["algorithmic_bytes"] += alg_b
-    for c in classes.values():
-        c["ratio"] = c["traffic_bytes"] / c["algorithmic_bytes"]
-    out = {"method": a.method, "copy_calibration": {"bytes": a.copy_bytes, "fetch_factor": f_fac / 1024,
-                                                    "write_factor": w_fac / 1024},
-           "kernels": kernels, "classes": classes}
+    for rows, fac, key in ((fetch, f_fac, "fetch_bytes"), (write, w_fac, "write_bytes")):
+        for _, n, v in rows:
+            cls = kernel_class(n)
+            if cls is None:
+                continue
+            k = kernels.setdefault(n, {"class": cls, "dispatches": 0, "fetch_bytes": 0.0, "write_bytes": 0.0,
+                                       "max_dispatch_fetch_bytes": 0.0, "max_dispatch_write_bytes": 0.0})
+            if key == "fetch_bytes":
+                k["dispatches"] += 1
+            k[key] += v * fac
+            k["max_dispatch_" + key] = max(k["max_dispatch_" + key], v * fac)
+    for n, k in kernels.items():
+        c = classes.setdefault(str(k["class"]), {"name": CLASS_NAMES[k["class"]], "traffic_bytes": 0.0})
+        c["traffic_bytes"] += k["fetch_bytes"] + k["write_bytes"]
+    for cid, c in classes.items():
+        alg = kc[c["name"]]["GB"] * 1e9 * a.solves
+        c["algorithmic_bytes"] = alg
+        c["ratio"] = c["traffic_bytes"] / alg
+    out = {"copy_calibration": {"bytes": a.copy_bytes, "fetch_factor": f_fac / 1024, "write_factor": w_fac / 1024},
+           "solves": a.solves, "kernels": kernels, "classes": classes}
     with open(a.json, "w") as fh:
         json.dump(out, fh, indent=1)
-    print("| kernel | fetched GB | written GB | traffic GB | algorithmic GB | traffic / algorithmic |")
-    print("|---|---|---|---|---|---|")
-    for n, k in kernels.items():
-        print(f"| `{n}` | {k['fetch_bytes'] / 1e9:.2f} | {k['write_bytes'] / 1e9:.2f} | "
-              f"{k['traffic_bytes'] / 1e9:.2f} | {k['algorithmic_bytes'] / 1e9:.2f} | {k['ratio']:.2f} |")
+    print("| class | traffic GB | algorithmic GB | traffic / algorithmic |")
+    print("|---|---|---|---|")
+    for cid in sorted(classes):
+        c = classes[cid]
+        print(f"| {c['name']} | {c['traffic_bytes'] / 1e9:.1f} | {c['algorithmic_bytes'] / 1e9:.1f} | {c['ratio']:.2f} |")
+    print("\n| kernel | dispatches | fetched GB | written GB | largest dispatch GB (fetch + write) |")
+    print("|---|---|---|---|---|")
+    for n, k in sorted(kernels.items(), key=lambda kv: -(kv[1]["fetch_bytes"] + kv[1]["write_bytes"])):
+        print(f"| `{n}` | {k['dispatches']} | {k['fetch_bytes'] / 1e9:.1f} | {k['write_bytes'] / 1e9:.1f} | "
+              f"{(k['max_dispatch_fetch_bytes'] + k['max_dispatch_write_bytes']) / 1e9:.2f} |")
     print(f"\ncalibration: bytes = FETCH_SIZE x {f_fac / 1024:.3f} KiB, WRITE_SIZE x {w_fac / 1024:.3f} KiB")
     return 0
 
