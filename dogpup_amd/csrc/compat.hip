@@ -185,11 +185,27 @@ int dogpup_fsaip_call(double* d_out_real, double* d_out_imag, const double* d_va
                      nfreqs, nullptr, out, as_stream(stream));
 }
 
+// Host check of a CSR structure that a mex-level call receives as host
+// arrays: the row pointer starts at 0, never decreases and ends at nnz, and
+// every column lies in [0, ncol).  A malformed structure would send the
+// kernels out of bounds, so it is an argument error before any upload.
+static int check_csr_host(const char* who, const char* name, const int* rp, const int* cp, int nrows,
+                          int nnz, int ncol) {
+  DP_ARG(rp[0] == 0 && rp[nrows] == nnz, "%s: rPtr%s must run from 0 to nnz = %d (got %d .. %d)", who,
+         name, nnz, rp[0], rp[nrows]);
+  for (int i = 0; i < nrows; ++i)
+    DP_ARG(rp[i] <= rp[i + 1], "%s: rPtr%s decreases at row %d", who, name, i);
+  for (int q = 0; q < nnz; ++q)
+    DP_ARG(cp[q] >= 0 && cp[q] < ncol, "%s: cPtr%s[%d] = %d outside [0, %d)", who, name, q, cp[q], ncol);
+  return DOGPUP_OK;
+}
+
 int dogpup_smv_mex(double* h_y, const double* h_x, const double* h_f, const double* h_A,
                    const int* h_rPtrA, const int* h_cPtrA, int nrows, int ncols, int nfreqs,
                    int nnz) {
   DP_ARG(h_y && h_x && h_f && h_A && h_rPtrA && h_cPtrA, "smv_mex: NULL pointer");
   DP_ARG(nrows >= 0 && ncols >= 0 && nfreqs >= 0 && nnz >= 0, "smv_mex: negative dimension");
+  DP_TRY(check_csr_host("smv_mex", "A", h_rPtrA, h_cPtrA, nrows, nnz, nrows));
   const size_t nel = (size_t)nrows * ncols * nfreqs;
   DevBuf x, f, a, rp, cp, y;
   DP_TRY(x.upload(h_x, nel * 16));
@@ -209,6 +225,7 @@ int dogpup_fsaimv_mex(double* h_y, const double* h_x, const double* h_G, const i
                       const int* h_cPtrG, int nrows, int ncols, int nfreqs, int nnzG) {
   DP_ARG(h_y && h_x && h_G && h_rPtrG && h_cPtrG, "FSAImv_mex: NULL pointer");
   DP_ARG(nrows >= 0 && ncols >= 0 && nfreqs >= 0 && nnzG >= 0, "FSAImv_mex: negative dimension");
+  DP_TRY(check_csr_host("FSAImv_mex", "G", h_rPtrG, h_cPtrG, nrows, nnzG, nrows));
   const size_t nel = (size_t)nrows * ncols * nfreqs;
   DevBuf x, g, rp, cp, y;
   DP_TRY(x.upload(h_x, nel * 16));
@@ -229,6 +246,8 @@ int dogpup_fsaip_gen_mex(double* h_valG, const double* h_valA, const int* h_rPtr
   DP_ARG(h_valG && h_valA && h_rPtrA && h_cPtrA && h_rPtrG && h_cPtrG,
          "FSAIP_gen_mex: NULL pointer");
   DP_ARG(nrows >= 0 && nnzA >= 0 && nnzG >= 0 && nfreqs >= 0, "FSAIP_gen_mex: negative dimension");
+  DP_TRY(check_csr_host("FSAIP_gen_mex", "A", h_rPtrA, h_cPtrA, nrows, nnzA, nrows));
+  DP_TRY(check_csr_host("FSAIP_gen_mex", "G", h_rPtrG, h_cPtrG, nrows, nnzG, nrows));
   DevBuf a, ra, ca, rg, cg, outr, outi;
   DP_TRY(a.upload(h_valA, (size_t)nnzA * 16));
   DP_TRY(ra.upload(h_rPtrA, (size_t)(nrows + 1) * 4));

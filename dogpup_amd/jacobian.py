@@ -105,8 +105,22 @@ def tg_data(mesh, field, gate_fc):
 
 def vjp(mesh, field, gate_fc, ybar, exact=False, dxyz=None):
     """g = J_TG^T ybar for nodal mua (kappa fixed), J not formed.  exact=False
-    is the reference's nodal-product Jacobian, exact=True the exact FEM
-    derivative."""
+    is the reference's nodal-product Jacobian on the nodal basis, scaled by
+    dxyz^3 as jacobian_tg scales it (dxyz: the argument, else mesh.dxyz, else
+    ValueError); exact=True the exact FEM derivative, which has no voxel
+    factor.  The nodal-product form is rejected on a mesh with a voxel basis
+    attached: its J_TG has voxel columns, of which a nodal gradient is not the
+    transpose."""
+    if not exact:
+        if getattr(mesh, "basis", None) is not None:
+            raise ValueError("vjp: the nodal-product J_TG of a mesh with a voxel basis has voxel columns; "
+                             "use jacobian_tg, or clear_grid() for the nodal basis")
+        if dxyz is None:
+            dxyz = getattr(mesh, "dxyz", None)
+        if dxyz is None:
+            raise ValueError("vjp: voxel side dxyz is needed (set mesh.dxyz or pass dxyz)")
+    else:
+        dxyz = 1.0                                    # unused by the exact derivative
     opt = mesh.optode
     ns = len(opt.s_bary[0])
     nf, nt = opt.Nf, opt.tAxis.size
@@ -121,8 +135,6 @@ def vjp(mesh, field, gate_fc, ybar, exact=False, dxyz=None):
     link_d, lptr_d, lidx_d = (torch.as_tensor(a, device=DEV) for a in (link, lptr, lidx))
     E = torch.empty((mesh.nn, nf, ns), dtype=torch.complex128, device=DEV)
     out = torch.empty(mesh.nn, dtype=torch.float64, device=DEV)
-    if dxyz is None:
-        dxyz = getattr(mesh, "dxyz", 1.0)
     x = torch.view_as_real(field.data)
     _lib.call("dogpup_jacobian_vjp", _lib.ptr(x), _lib.ptr(torch.view_as_real(Dd)), _lib.ptr(link_d),
               _lib.ptr(lptr_d), _lib.ptr(lidx_d), _lib.ptr(mesh._dev["node"]), _lib.ptr(mesh._dev["elem"]),
@@ -134,22 +146,30 @@ def vjp(mesh, field, gate_fc, ybar, exact=False, dxyz=None):
 class AbsorptionTG(torch.autograd.Function):
     """mua (NN, float64) -> time-gated boundary data (NM x NG) with kappa held
     at the mesh's current values; backward = J_TG^T ybar through
-    dogpup_jacobian_vjp (nodal-product Jacobian of dMesh.m:206-247, or the
-    exact derivative with exact=True)."""
+    dogpup_jacobian_vjp (nodal-product Jacobian of dMesh.m:206-247 with the
+    voxel side dxyz, or the exact derivative with exact=True).  The mesh's
+    properties (mua, musp, kappa and the assembled system) are restored after
+    the forward solve, so the call leaves the caller's mesh as it found it."""
 
     @staticmethod
-    def forward(ctx, mua, mesh, gate_fc, exact=False, mode="fp64", tol=1e-12):
-        mesh.update_properties(np.stack([mua.detach().cpu().numpy(), mesh.musp], 1), kappa=mesh.kappa.copy())
-        field = mesh.solve_point_sources("sd", mode=mode, tol=tol)
-        ctx.mesh, ctx.field, ctx.gate_fc, ctx.exact = mesh, field, gate_fc, exact
+    def forward(ctx, mua, mesh, gate_fc, exact=False, mode="fp64", tol=1e-12, dxyz=None):
+        props0 = np.stack([mesh.mua.copy(), mesh.musp.copy()], 1)
+        kappa0 = mesh.kappa.copy()
+        mesh.update_properties(np.stack([mua.detach().cpu().numpy(), props0[:, 1]], 1), kappa=kappa0)
+        try:
+            field = mesh.solve_point_sources("sd", mode=mode, tol=tol)
+            y = tg_data(mesh, field, gate_fc)
+        finally:
+            mesh.update_properties(props0, kappa=kappa0)
+        ctx.mesh, ctx.field, ctx.gate_fc, ctx.exact, ctx.dxyz = mesh, field, gate_fc, exact, dxyz
         ctx.in_device, ctx.in_dtype = mua.device, mua.dtype
-        return tg_data(mesh, field, gate_fc)
+        return y
 
     @staticmethod
     def backward(ctx, ybar):
-        g = vjp(ctx.mesh, ctx.field, ctx.gate_fc, ybar, exact=ctx.exact)
-        return g.to(ctx.in_device, ctx.in_dtype), None, None, None, None, None
+        g = vjp(ctx.mesh, ctx.field, ctx.gate_fc, ybar, exact=ctx.exact, dxyz=ctx.dxyz)
+        return g.to(ctx.in_device, ctx.in_dtype), None, None, None, None, None, None
 
 
-def absorption_tg(mua, mesh, gate_fc, exact=False, mode="fp64", tol=1e-12):
-    return AbsorptionTG.apply(mua, mesh, gate_fc, exact, mode, tol)
+def absorption_tg(mua, mesh, gate_fc, exact=False, mode="fp64", tol=1e-12, dxyz=None):
+    return AbsorptionTG.apply(mua, mesh, gate_fc, exact, mode, tol, dxyz)

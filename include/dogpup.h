@@ -90,8 +90,10 @@ int dogpup_fsaip_call(double* d_out_real, double* d_out_imag, const double* d_va
 
 /* ---------------------------------------------------------------------------
  * Mex-level entry points: host arrays in the interleaved-complex MATLAB
- * layout (R2018a complex API), synchronous.  A MATLAB mex gateway for AMD
- * forwards its mxArrays here (gpuArray is CUDA-only).
+ * layout (R2018a complex API), synchronous.  The MATLAB gateways in
+ * dogpup_amd/mex/ forward their mxArrays here (gpuArray is CUDA-only).  Each
+ * CSR structure is checked on the host first (row pointer from 0 to nnz,
+ * never decreasing; columns in [0, nrows)): DOGPUP_ERR_ARG otherwise.
  * ------------------------------------------------------------------------- */
 
 /* y = smv_mex(x, f, A, rPtrA, cPtrA)  (smv_mexgateway.cpp:8-100).

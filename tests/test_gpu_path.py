@@ -348,15 +348,18 @@ def test_jacobian_tg_cancellation(monkeypatch):
 
 def test_autograd_vjp_vs_fd_and_reference_jacobian(setup):
     """AbsorptionTG: exact-mode gradient vs central finite differences in
-    nodal mua (kappa fixed); nodal-mode gradient vs the contraction of the
-    oracle's reference Jacobian."""
+    nodal mua (kappa fixed); nodal-mode gradient at dxyz = 2 vs the
+    contraction of the oracle's reference Jacobian at dxyz = 2; the mesh is
+    left as it was (same properties, bit-identical system); the nodal mode
+    needs a dxyz and refuses a voxel basis."""
     from dogpup_amd import jacobian as jac
     from oracle import jacobian as ojac, optode as oopt, solver as osol, transforms as otr
     D, om, opt = setup["D"], setup["om"], setup["opt"]
     t, f, nf = setup["t"], setup["f"], setup["nf"]
     gates = np.stack([((t >= a) & (t < a + 500e-12)).astype(float) for a in (0.0, 200e-12, 400e-12)])
     gfc = otr.td2fc(gates, f, t, 1)
-    kappa0 = D.kappa.copy()
+    mua_before, kappa_before = D.mua.copy(), D.kappa.copy()
+    K_before, M_before = D.K.clone(), D.M.clone()
     mua0 = torch.tensor(D.mua.copy(), dtype=torch.float64, requires_grad=True)
     rng = np.random.default_rng(5)
     ybar = torch.as_tensor(rng.normal(size=(opt.link.shape[0], gfc.shape[0])), device="cuda")
@@ -368,26 +371,39 @@ def test_autograd_vjp_vs_fd_and_reference_jacobian(setup):
         d = 1e-5
         vals = []
         for sgn in (1, -1):
-            mu = D.mua.copy() * 0 + mua0.detach().numpy()
+            mu = mua0.detach().numpy().copy()
             mu[v] += sgn * d
             with torch.no_grad():
                 yy = jac.absorption_tg(torch.as_tensor(mu), D, gfc, True, "fp64", 1e-13)
             vals.append(float((yy * ybar).sum()))
         fd = (vals[0] - vals[1]) / (2 * d)
         assert abs(fd - g_exact[v]) <= 1e-5 * abs(g_exact).max(), (v, fd, g_exact[v])
-    # restore the mesh and check the nodal mode against the oracle's J
-    D.update_properties(np.stack([mua0.detach().numpy(), D.musp], 1), kappa=kappa0)
+    # the forward calls restored the mesh
+    assert np.array_equal(D.mua, mua_before) and np.array_equal(D.kappa, kappa_before)
+    assert torch.equal(D.K, K_before) and torch.equal(D.M, M_before)
+    # nodal mode, dxyz = 2, against the oracle's J^T ybar
+    assert getattr(D, "basis", None) is None
+    saved = getattr(D, "dxyz", None)
+    D.dxyz = None
     mu1 = torch.tensor(D.mua.copy(), dtype=torch.float64, requires_grad=True)
-    y = jac.absorption_tg(mu1, D, gfc, False, "fp64", 1e-13)
+    with pytest.raises(ValueError, match="dxyz"):
+        jac.absorption_tg(mu1, D, gfc, False, "fp64", 1e-13).sum().backward()
+    mu1 = torch.tensor(D.mua.copy(), dtype=torch.float64, requires_grad=True)
+    y = jac.absorption_tg(mu1, D, gfc, False, "fp64", 1e-13, dxyz=2.0)
     (y * ybar).sum().backward()
     g_nodal = mu1.grad.numpy()
     sid, sw = opt.s_bary
     did, dw = opt.d_bary
     phi = osol.direct(setup["A"], oopt.source_matrix(om, sid, sw, opt.s_fpsf))
     phiA = osol.direct(setup["A"], oopt.source_matrix(om, did, dw, opt.s_fpsf))
-    Jref = ojac.j_tg(phi, phiA, setup["link"], opt.s_fpsf, 1.0, gfc, f, t.size)   # NM x NG x NV, dxyz = 1
+    Jref = ojac.j_tg(phi, phiA, setup["link"], opt.s_fpsf, 2.0, gfc, f, t.size)   # NM x NG x NV
     gref = np.einsum("mg,mgv->v", ybar.cpu().numpy(), Jref)
     assert rel(g_nodal, gref) < 1e-8
+    # and it is jacobian_tg's J^T ybar at the same dxyz
+    Jt, _ = jac.jacobian_tg(D, torch.as_tensor(gfc), mode="fp64", tol=1e-13, dxyz=2.0)
+    gJ = torch.einsum("vgm,mg->v", Jt.double(), ybar).cpu().numpy()
+    assert rel(g_nodal, gJ) < 1e-5                                  # J_TG is float32
+    D.dxyz = saved
 
 
 @pytest.mark.parametrize("method", ["bicgstab", "cocg"])
