@@ -17,6 +17,21 @@ ERR_ARG, ERR_HIP, ERR_UNSUPPORTED, ERR_NOMEM = -1, -2, -3, -4
 RHS_CONVERGED, RHS_MAXIT, RHS_BREAKDOWN, RHS_ZERO = 0, 1, 2, 3
 MODE_C128, MODE_C64_REFINE, MODE_C128_PLAIN = 0, 1, 2
 METHOD_BICGSTAB, METHOD_COCG = 0, 1
+CONT_GALERKIN, CONT_LAGRANGE = 1, 2
+CONT_KINDS = {"galerkin": CONT_GALERKIN, "lagrange": CONT_LAGRANGE}
+
+
+def cont_kind(cont) -> int:
+    """Frequency-continuation argument (True / None: Galerkin guesses, False:
+    none, 'galerkin' or 'lagrange') as dogpup_solver_set_freq_continuation's
+    `on`."""
+    if cont is None or cont is True:
+        return CONT_GALERKIN
+    if cont is False:
+        return 0
+    if cont not in CONT_KINDS:
+        raise ValueError(f"unknown continuation {cont!r}; expected True, False, 'galerkin' or 'lagrange'")
+    return CONT_KINDS[cont]
 
 _lock = threading.Lock()
 _lib = None

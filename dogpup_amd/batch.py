@@ -83,8 +83,9 @@ class MeshBatch:
         if MODES[mode] == _lib.MODE_C64_REFINE:
             spec = np.ascontiguousarray(np.concatenate([np.asarray(m.optode.s_fpsf, np.complex128).reshape(-1)
                                                         for m in self.meshes]))
-            _lib.call("dogpup_solver_set_freq_continuation", self.h, int(bool(cont)),
-                      _lib.ptr(spec.view(np.float64)) if cont else None)
+            on = _lib.cont_kind(cont)
+            _lib.call("dogpup_solver_set_freq_continuation", self.h, on,
+                      _lib.ptr(spec.view(np.float64)) if on else None)
         self._versions = None
         # block of each mesh in the batch vectors: nf*ncb groups of nn rows
         sizes = [f * b * n * self.nsp for f, b, n in zip(self.nf, self.ncb, nn)]

@@ -284,9 +284,11 @@ class DMesh:
         columns when it differs from tol (fused and mixed modes), as separate
         flu_solve / adj_flu_solve calls with two tolerances would have.
         cont (mixed mode): frequency continuation, each frequency of a later
-        level started from the interpolation of neighbouring frequencies
-        solved before it, normalised by the source spectrum s_fpsf that every
-        right-hand side carries (the default, True); False starts every
+        level started from the converged solutions of neighbouring
+        frequencies solved before it, using the source spectrum s_fpsf that
+        every right-hand side carries: True / None / 'galerkin' (the
+        default) combines the six nearest by the Galerkin solution in their
+        span, 'lagrange' interpolates the four nearest; False starts every
         right-hand side from 0, as the reference does."""
         parts = [self._bary_dev(w) for w in which]
         nodes = torch.cat([p[0] for p in parts])
@@ -294,7 +296,7 @@ class DMesh:
         ncols = nodes.shape[0]
         S = self.solver(ncols, mode, method)
         if MODES[mode] == _lib.MODE_C64_REFINE:
-            S.set_continuation(True if cont is None else cont, self.optode.s_fpsf)
+            S.set_continuation(cont, self.optode.s_fpsf)
         elif cont:
             raise ValueError("frequency continuation is for the mixed mode")
         spec = torch.as_tensor(self.optode.s_fpsf, dtype=torch.complex128, device=DEV)
@@ -354,9 +356,9 @@ class FluenceSolver:
         self._version = None
         self._col_tol = False
         self.method = "cocg" if mode != _lib.MODE_C128_PLAIN else "bicgstab"
-        # continuation state: the library's default (on, flat spectrum, in the
-        # refinement mode)
-        self._cont_key = (mode == _lib.MODE_C64_REFINE, None)
+        # continuation state: the library's default (Galerkin guesses, flat
+        # spectrum, in the refinement mode)
+        self._cont_key = (_lib.CONT_GALERKIN if mode == _lib.MODE_C64_REFINE else 0, None)
 
     def set_method(self, method: str):
         """Krylov method of the fused modes: 'bicgstab' or 'cocg'."""
@@ -365,11 +367,12 @@ class FluenceSolver:
         _lib.call("dogpup_solver_set_method", self.h, METHODS[method])
         self.method = method
 
-    def set_continuation(self, on: bool, spectrum=None):
+    def set_continuation(self, on=True, spectrum=None):
         """Frequency continuation of the mixed mode
-        (dogpup_solver_set_freq_continuation); spectrum: the complex factor
-        per frequency that every right-hand side carries (None: flat)."""
-        on = bool(on)
+        (dogpup_solver_set_freq_continuation): on = True / None / 'galerkin',
+        'lagrange' or False; spectrum: the complex factor per frequency that
+        every right-hand side carries (None: flat)."""
+        on = _lib.cont_kind(on)
         spec = None if spectrum is None else np.ascontiguousarray(np.asarray(spectrum, np.complex128).reshape(-1))
         key = (on, None if spec is None else spec.tobytes())
         if key == self._cont_key:
@@ -378,7 +381,7 @@ class FluenceSolver:
             raise ValueError("frequency continuation is for the mixed (refinement) mode")
         if spec is not None and spec.size != self.nf:
             raise ValueError(f"spectrum needs {self.nf} values, got {spec.size}")
-        _lib.call("dogpup_solver_set_freq_continuation", self.h, int(on),
+        _lib.call("dogpup_solver_set_freq_continuation", self.h, on,
                   None if spec is None else _lib.ptr(spec.view(np.float64)))
         self._cont_key = key
 

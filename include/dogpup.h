@@ -227,20 +227,32 @@ int dogpup_solver_set_rhs_tol(dogpup_solver_t s, const double* h_tol, int n);
  * x0 = 0 (dMesh.m:1283). */
 int dogpup_solver_set_guess(dogpup_solver_t s, int nterm, const int* h_src, const double* h_w);
 
-/* Frequency continuation (on by default in DOGPUP_MODE_C64_REFINE, with a
- * flat spectrum; the environment variable DOGPUP_FREQ_CONT=0 turns the
- * default off): guesses built from the frequency scales of
- * dogpup_solver_set_matrix.  Frequency k of a mesh is in level 0 if
- * k % 8 == 0 or k is the last, level 1 if k % 8 == 4, level 2 if k % 4 == 2
- * and level 3 otherwise, and a later-level frequency starts from the
- * Lagrange interpolation, in the frequency scale, of the four frequencies of
- * earlier levels nearest to it.  h_spectrum (NULL, or nf_total complex
- * values): the right-hand side of frequency f is h_spectrum[f] times a
- * vector that does not depend on f, as for point sources, whose right-hand
- * sides carry the source spectrum s_fpsf (dMesh.m:128,196); the
- * interpolation then runs on x / spectrum, which is smooth in the frequency
- * where x is not.  on = 0 solves every RHS from x0 = 0, as the reference
- * does; either call clears guesses set with dogpup_solver_set_guess. */
+/* Frequency continuation (on by default in DOGPUP_MODE_C64_REFINE, with
+ * Galerkin guesses and a flat spectrum; the environment variable
+ * DOGPUP_FREQ_CONT=0 turns the default off): guesses built from the
+ * frequency scales of dogpup_solver_set_matrix.  The frequencies of a mesh
+ * are solved in levels, each level to convergence before the next: level 0
+ * is every stride-th frequency and the last, and each later level halves the
+ * spacing (stride 16 for Galerkin guesses, 8 for Lagrange guesses).  A
+ * later-level frequency starts from a combination of the converged solutions
+ * of earlier levels nearest to it in the frequency scale:
+ *   on = DOGPUP_CONT_GALERKIN (1): the six nearest, combined per column as
+ *     the Galerkin solution of its system in their span (the M-Gram matrix
+ *     of the solutions and their dots with the right-hand side give that
+ *     small system exactly, see dogpup_amd/csrc/solver.hip);
+ *   on = DOGPUP_CONT_LAGRANGE (2): the four nearest, combined by Lagrange
+ *     interpolation in the frequency scale.
+ * h_spectrum (NULL, or nf_total complex values): the right-hand side of
+ * frequency f is h_spectrum[f] times a vector that does not depend on f, as
+ * for point sources, whose right-hand sides carry the source spectrum s_fpsf
+ * (dMesh.m:128,196); the Lagrange interpolation then runs on x / spectrum,
+ * which is smooth in the frequency where x is not, and the Galerkin system
+ * uses it to express K x_j through b_j.  Convergence is the same
+ * ||b - A x|| / ||b|| <= tol per RHS whatever the guess.  on = 0 solves every
+ * RHS from x0 = 0, as the reference does; either call clears guesses set with
+ * dogpup_solver_set_guess. */
+#define DOGPUP_CONT_GALERKIN 1
+#define DOGPUP_CONT_LAGRANGE 2
 int dogpup_solver_set_freq_continuation(dogpup_solver_t s, int on, const double* h_spectrum);
 
 /* Counters of the last solve: number of vector-kernel passes weighted by the

@@ -1,7 +1,7 @@
 """Initial guesses of the refinement solver: frequency continuation (each
-frequency of a later level started from the Lagrange interpolation of
-neighbouring frequencies solved before it) and explicit guess terms (a warm
-start from the caller's x).  Every solution is checked against the oracle's
+frequency of a later level started from the Galerkin combination, or the
+Lagrange interpolation, of neighbouring frequencies solved before it) and
+explicit guess terms (a warm start from the caller's x).  Every solution is checked against the oracle's
 direct solve (dMesh.m:1238-1348 to convergence); the reference starts every
 right-hand side from x0 = 0 (dMesh.m:1283), so the guesses may change the
 iterations and nothing else."""
@@ -43,7 +43,7 @@ def sixteen():
 def test_frequency_continuation_vs_direct(sixteen, method):
     D, phi = sixteen["D"], sixteen["phi"]
     out = {}
-    for cont in (False, True):
+    for cont in (False, "lagrange", "galerkin"):
         print(f"[case] continuation={cont} {method}", flush=True)
         fw = D.solve_point_sources("s", mode="mixed", tol=1e-10, method=method, cont=cont)
         assert (fw.status == 0).all(), fw.status
@@ -53,14 +53,20 @@ def test_frequency_continuation_vs_direct(sixteen, method):
             for s in range(got.shape[2]):
                 assert rel(got[:, k, s], phi[:, k, s]) < 1e-7, (cont, k, s)
         out[cont] = fw.iters.copy()
-    # levels 1-3 start from interpolated neighbours: fewer iterations there,
-    # level 0 (every 8th frequency and the last) is unchanged
-    lvl0 = [0, 8, 15]
-    print("iterations per frequency (max over sources) without / with continuation:",
-          out[False].max(axis=1).tolist(), out[True].max(axis=1).tolist(), flush=True)
-    assert (out[True][lvl0] == out[False][lvl0]).all()
-    later = [k for k in range(16) if k not in lvl0]
-    assert (out[True][later].max(axis=1) < out[False][later].max(axis=1)).all()
+    print("iterations per frequency (max over sources) without / Lagrange / Galerkin:",
+          out[False].max(axis=1).tolist(), out["lagrange"].max(axis=1).tolist(),
+          out["galerkin"].max(axis=1).tolist(), "totals", out[False].sum(), out["lagrange"].sum(),
+          out["galerkin"].sum(), flush=True)
+    # level 0 starts from x0 = 0 (Lagrange: every 8th frequency and the last;
+    # Galerkin: every 16th and the last); every later level starts from a
+    # guess and needs fewer iterations
+    for cont, lvl0 in (("lagrange", [0, 8, 15]), ("galerkin", [0, 15])):
+        assert (out[cont][lvl0] == out[False][lvl0]).all(), cont
+        later = [k for k in range(16) if k not in lvl0]
+        assert (out[cont][later].max(axis=1) < out[False][later].max(axis=1)).all(), cont
+    # the Galerkin guesses (six solutions combined per column) beat the
+    # Lagrange interpolation of four, with fewer frequencies solved from 0
+    assert out["galerkin"].sum() < out["lagrange"].sum()
 
 
 def test_warm_start_and_guess_terms(sixteen):
