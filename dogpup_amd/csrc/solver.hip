@@ -1778,7 +1778,6 @@ constexpr int kRbTerms = 6;            // solutions in a guess's window
 constexpr int kRbPairsPerThread = 24;  // Gram pairs accumulated per thread
 constexpr int kRbTgtPerThread = 4;     // target groups per thread of k_rb_guess
 constexpr int kRbBlocks = 512;         // blocks of the Gram and guess launches
-constexpr int kRbLds = 65536;          // LDS bytes for staged rows
 
 // mx = M x64 for every group of the work list (M = the unit-frequency
 // imaginary part of A, real), stored at x64's offsets, and part1 = sum over
@@ -1833,11 +1832,56 @@ __global__ __launch_bounds__(kScalThreads) void k_rb_csum(const double* __restri
   }
 }
 
+// Row staging of the Gram and guess kernels.  A tile is R consecutive rows
+// (mesh-relative) of nl listed vectors, [list][row][c] in LDS, at most
+// kRbStage double2 per thread: the element e = threadIdx.x + 256 i of list
+// e / (R NSP) sits at a fixed offset from the tile's first row, so every
+// thread keeps its kRbStage source pointers and issues the next tile's
+// loads (all in flight) before the current tile's products.
+constexpr int kRbStage = 8;
+constexpr int kRbLds = 256 * kRbStage * 16;   // LDS bytes of a tile
+template <int NSP>
+struct RbStage {
+  const double2* src[kRbStage];   // column of the element in row 0 of its vector
+  int row[kRbStage];              // row within the tile; -1: no element
+  double2 v[kRbStage];
+  // vector u of the list starts at base(u) (complex elements, row 0)
+  template <typename Base>
+  __device__ inline void init(int nl, int R, Base base) {
+    const int per = R * NSP;
+#pragma unroll
+    for (int i = 0; i < kRbStage; ++i) {
+      const int e = threadIdx.x + 256 * i;
+      row[i] = -1;
+      src[i] = nullptr;
+      if (e < nl * per) {
+        const int u = e / per, k = e - u * per;
+        row[i] = k / NSP;
+        src[i] = base(u) + (k - row[i] * NSP);
+      }
+    }
+  }
+  // the tile of rows [r0, r0 + nr); an element past the mesh's last row
+  // reads row r0 instead (in bounds, never used)
+  __device__ inline void load(int r0, int nr) {
+#pragma unroll
+    for (int i = 0; i < kRbStage; ++i) {
+      const int r = (row[i] >= 0 && row[i] < nr) ? row[i] : 0;
+      v[i] = src[i] ? src[i][((long long)r0 + r) * NSP] : make_double2(0.0, 0.0);
+    }
+  }
+  __device__ inline void store(double2* lds) const {
+#pragma unroll
+    for (int i = 0; i < kRbStage; ++i)
+      if (row[i] >= 0) lds[threadIdx.x + 256 * i] = v[i];
+  }
+};
+
 // Column-wise Gram entries over the nrows rows of one mesh: pair p gives
 // part[(blk*np + p)*NSP + c] = sum over this block's rows of
 // conj(x64[offa[pia[p]] + row][c]) * mx[offb[pib[p]] + row][c].  Tiles of R
-// rows, tile t = blockIdx.x + k*gridDim.x; each tile's rows of the na + nb
-// listed vectors are staged in LDS ([list][row][c], R*NSP complex each).
+// rows, tile t = blockIdx.x + k*gridDim.x, staged as above (the na listed
+// x64 vectors, then the nb listed mx vectors).
 template <int NSP>
 __global__ __launch_bounds__(256) void k_rb_gram(const double* __restrict__ x64, const double* __restrict__ mx,
                                                  const long long* __restrict__ offa, int na,
@@ -1846,38 +1890,41 @@ __global__ __launch_bounds__(256) void k_rb_gram(const double* __restrict__ x64,
                                                  int nrows, int R, double* __restrict__ part) {
   extern __shared__ double2 rb_lds[];
   constexpr int PQ = 256 / NSP;
+  // LDS offsets (in double2, row 0, column 0) of each pair's two vectors
+  __shared__ int lpa[PQ * kRbPairsPerThread], lpb[PQ * kRbPairsPerThread];
   const int c = threadIdx.x % NSP, q = threadIdx.x / NSP;
-  int ia[kRbPairsPerThread], ib[kRbPairsPerThread];
+  for (int p = threadIdx.x; p < np; p += 256) {
+    lpa[p] = pia[p] * R * NSP;
+    lpb[p] = (na + pib[p]) * R * NSP;
+  }
   double ar[kRbPairsPerThread], ai[kRbPairsPerThread];
   const int npt = np > q ? (np - q + PQ - 1) / PQ : 0;   // this thread's pairs
 #pragma unroll
   for (int i = 0; i < kRbPairsPerThread; ++i) {
-    const int p = min(q + i * PQ, np - 1);
-    ia[i] = pia[p];
-    ib[i] = na + pib[p];
     ar[i] = 0.0;
     ai[i] = 0.0;
   }
-  const int nl = na + nb, per = R * NSP, ntile = (nrows + R - 1) / R;
-  for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
-    const int r0 = t * R, nr = min(R, nrows - r0);
+  RbStage<NSP> stg;
+  stg.init(na + nb, R, [&](int u) {
+    return reinterpret_cast<const double2*>(u < na ? x64 : mx) + (u < na ? offa[u] : offb[u - na]) * NSP;
+  });
+  const int ntile = (nrows + R - 1) / R;
+  int t = blockIdx.x;
+  if (t < ntile) stg.load(t * R, min(R, nrows - t * R));
+  for (; t < ntile; t += gridDim.x) {
+    const int nr = min(R, nrows - t * R);
+    __syncthreads();   // the previous tile's products are done
+    stg.store(rb_lds);
     __syncthreads();
-    for (int e = threadIdx.x; e < nl * per; e += 256) {
-      const int u = e / per, k = e - u * per;
-      double2 v = make_double2(0.0, 0.0);
-      if (k / NSP < nr) {
-        const long long o = ((u < na ? offa[u] : offb[u - na]) + r0) * NSP + k;
-        v = reinterpret_cast<const double2*>(u < na ? x64 : mx)[o];
-      }
-      rb_lds[e] = v;
-    }
-    __syncthreads();
+    const int tn = t + gridDim.x;
+    if (tn < ntile) stg.load(tn * R, min(R, nrows - tn * R));
     for (int r = 0; r < nr; ++r) {
 #pragma unroll
       for (int i = 0; i < kRbPairsPerThread; ++i) {
         if (i < npt) {
-          const double2 a = rb_lds[(ia[i] * R + r) * NSP + c];
-          const double2 bb = rb_lds[(ib[i] * R + r) * NSP + c];
+          const int p = q + i * PQ;
+          const double2 a = rb_lds[lpa[p] + r * NSP + c];
+          const double2 bb = rb_lds[lpb[p] + r * NSP + c];
           ar[i] += a.x * bb.x + a.y * bb.y;
           ai[i] += a.x * bb.y - a.y * bb.x;
         }
@@ -1898,20 +1945,32 @@ __global__ __launch_bounds__(256) void k_rb_gram(const double* __restrict__ x64,
 __global__ void k_rb_reduce(const double* __restrict__ part, double* __restrict__ out, int n, int nblk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double sr = 0.0, si = 0.0;
-  for (int b = 0; b < nblk; ++b) {
-    sr += part[2 * ((long long)b * n + i)];
-    si += part[2 * ((long long)b * n + i) + 1];
+  constexpr int U = 8;   // eight loads in flight
+  double sr[U], si[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) { sr[u] = 0.0; si[u] = 0.0; }
+  int b = 0;
+  for (; b + U <= nblk; b += U)
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      sr[u] += part[2 * ((long long)(b + u) * n + i)];
+      si[u] += part[2 * ((long long)(b + u) * n + i) + 1];
+    }
+  for (; b < nblk; ++b) {
+    sr[0] += part[2 * ((long long)b * n + i)];
+    si[0] += part[2 * ((long long)b * n + i) + 1];
   }
-  out[2 * i] = sr;
-  out[2 * i + 1] = si;
+  double r = 0.0, m = 0.0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) { r += sr[u]; m += si[u]; }
+  out[2 * i] = r;
+  out[2 * i + 1] = m;
 }
 
 // Guesses of nt target groups of one mesh: x64[offt[t] + row][c] =
 // sum_{i < kRbTerms} w[(t*kRbTerms + i)*NSP + c] * x64[offa[src[t*kRbTerms + i]] + row][c]
 // (src -1: no term; na = 0 writes zeros).  The rows of the na source
-// solutions (none of them a target) are staged in LDS per tile of R rows, as
-// in k_rb_gram.
+// solutions (none of them a target) are staged as in k_rb_gram.
 template <int NSP>
 __global__ __launch_bounds__(256) void k_rb_guess(double* __restrict__ x64, const long long* __restrict__ offa,
                                                   int na, const long long* __restrict__ offt, int nt,
@@ -1936,17 +1995,18 @@ __global__ __launch_bounds__(256) void k_rb_guess(double* __restrict__ x64, cons
     }
   }
   const int ntt = nt > q ? (nt - q + PQ - 1) / PQ : 0;   // this thread's targets
-  const int per = R * NSP, ntile = (nrows + R - 1) / R;
-  for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
+  RbStage<NSP> stg;
+  stg.init(na, R, [&](int u) { return reinterpret_cast<const double2*>(x64) + offa[u] * NSP; });
+  const int ntile = (nrows + R - 1) / R;
+  int t = blockIdx.x;
+  if (t < ntile) stg.load(t * R, min(R, nrows - t * R));
+  for (; t < ntile; t += gridDim.x) {
     const int r0 = t * R, nr = min(R, nrows - r0);
     __syncthreads();
-    for (int e = threadIdx.x; e < na * per; e += 256) {
-      const int u = e / per, k = e - u * per;
-      double2 v = make_double2(0.0, 0.0);
-      if (k / NSP < nr) v = reinterpret_cast<const double2*>(x64)[(offa[u] + r0) * NSP + k];
-      rb_lds[e] = v;
-    }
+    stg.store(rb_lds);
     __syncthreads();
+    const int tn = t + gridDim.x;
+    if (tn < ntile) stg.load(tn * R, min(R, nrows - tn * R));
     for (int r = 0; r < nr; ++r) {
 #pragma unroll
       for (int j = 0; j < kRbTgtPerThread; ++j) {
@@ -2738,8 +2798,16 @@ static int rb_guess_phase(dogpup_solver* S, hipStream_t st, const Work& Wfull, d
     return cd(S->h_spec[2 * f], S->h_spec[2 * f + 1]);
   };
   const std::vector<int> tgt = groups_of(phases[ip]);
+  // solutions whose M x this call computes (phases not done yet)
+  cache.mx_done.resize(phases.size(), 0);
+  std::vector<char> is_new(ng, 0);
+  for (size_t q = 0; q < ip; ++q)
+    if (!cache.mx_done[q])
+      for (int g : groups_of(phases[q])) is_new[g] = 1;
   // the windows, the solutions they read and the pairs they need, per mesh
-  // and column block
+  // and column block; a pair (a, b) is computed as x_a^H (M x_b) with a new
+  // solution on the M x side where there is one, so that a launch stages
+  // the M x of the new solutions only
   std::map<long long, std::vector<int>> fam_tgt, fam_src;
   std::map<long long, std::vector<std::pair<int, int>>> fam_pairs;
   for (int t : tgt) {
@@ -2749,8 +2817,10 @@ static int rb_guess_phase(dogpup_solver* S, hipStream_t st, const Work& Wfull, d
     for (int i = 0; i < nt && w[i] >= 0; ++i) {
       fam_src[fk].push_back(w[i]);
       for (int j = i; j < nt && w[j] >= 0; ++j) {
-        const int a = std::min(w[i], w[j]), b = std::max(w[i], w[j]);
-        if (!cache.gram.count((long long)a * ng + b)) fam_pairs[fk].push_back({a, b});
+        const int lo = std::min(w[i], w[j]), hi = std::max(w[i], w[j]);
+        if (cache.gram.count((long long)lo * ng + hi)) continue;
+        if (!is_new[hi] && is_new[lo]) fam_pairs[fk].push_back({hi, lo});
+        else fam_pairs[fk].push_back({lo, hi});
       }
     }
   }
@@ -2764,7 +2834,6 @@ static int rb_guess_phase(dogpup_solver* S, hipStream_t st, const Work& Wfull, d
     kv.second.erase(std::unique(kv.second.begin(), kv.second.end()), kv.second.end());
   }
   // 1. M x and x^H b of the phases before ip not done yet
-  cache.mx_done.resize(phases.size(), 0);
   std::vector<int> newg;
   RbUpload up;
   std::vector<std::pair<size_t, size_t>> mx_ranges;   // (phase, first index in newg)
@@ -2869,9 +2938,14 @@ static int rb_guess_phase(dogpup_solver* S, hipStream_t st, const Work& Wfull, d
   for (auto& mr : mx_ranges) cache.mx_done[mr.first] = 1;
   for (auto& L : gl)
     for (int p = 0; p < L.np; ++p) {
+      // stored as x_lo^H M x_hi (lo < hi); a pair computed the other way
+      // round gives its conjugate (M is real symmetric)
       const auto& ab = L.pairs[p];
       const size_t o = (L.o_out + p) * NSP;
-      cache.gram[(long long)ab.first * ng + ab.second].assign(h_gram.begin() + o, h_gram.begin() + o + NSP);
+      std::vector<cd>& v = cache.gram[(long long)std::min(ab.first, ab.second) * ng + std::max(ab.first, ab.second)];
+      v.assign(h_gram.begin() + o, h_gram.begin() + o + NSP);
+      if (ab.first > ab.second)
+        for (cd& e : v) e = std::conj(e);
     }
   // 3. weights of every target column, and the guess launches
   RbUpload up2;
