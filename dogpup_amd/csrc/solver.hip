@@ -1107,6 +1107,9 @@ __global__ __launch_bounds__(256) void k_residual64(
   const int c = lane % G::LPR;
   const long long rhs = (long long)it.gl * NSP + c;
   const bool act = ostatus ? (ostatus[rhs] == ST_ACTIVE) : true;
+  // a group none of whose columns is still iterating takes no part: its
+  // partial sums are not read and the inner solve skips it
+  if (ostatus && !__syncthreads_or(act)) return;
   const long long vb = it.vb;
   Acc<1> d1, d2, d3;
   d1.zero();
