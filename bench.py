@@ -296,10 +296,11 @@ def target_leg(D, gfc, tol, method, adj_tol, nsample=4000, seed=1):
     inside the timing, as in the headline step) against the
     reference-structure FP64 form (BiCGSTAB from x0 = 0 to 1e-12, the
     tolerance dMesh.m:1243 hard-codes; same J-TG kernel).  The mixed form
-    runs at two settings: the source columns to tol and the detector
-    (adjoint) columns to adj_tol (the Jacobian's 1e-4 target), and every
-    column to 1e-8 (the setting whose LM reconstruction stays within 0.1 %
-    of the FP64 form's).  The mixed J is compared with J of the FP64 field,
+    runs at three settings: the source columns to tol and the detector
+    (adjoint) columns to adj_tol or to adj_tol / 10, and every column to
+    1e-8.  `speedup` is the fastest setting whose J meets the 1e-4 target
+    on every gate (largest error relative to the gate's largest entry).
+    The mixed J is compared with J of the FP64 field,
     J64[v, g, m] = Re sum_k C_gk phi(v, k, s_m) psi(v, k, d_m) in complex128
     at nsample seeded nodes, over every (pair, gate) column: the Frobenius
     relative error, and per gate the largest error of a column relative to
@@ -321,7 +322,7 @@ def target_leg(D, gfc, tol, method, adj_tol, nsample=4000, seed=1):
         torch.cuda.synchronize()
         return time.perf_counter() - t0, f, J, info
     ns = len(D.optode.s_bary[0])
-    settings = {"tol_adj": (tol, adj_tol), "uniform_1e-8": (1e-8, None)}
+    settings = {"tol_adj": (tol, adj_tol), "tol_adj10": (tol, adj_tol / 10), "uniform_1e-8": (1e-8, None)}
     mixed, Js = {}, {}
     for key, (ft, at) in settings.items():
         run_mixed(ft, at)                                                   # warm-up at this setting
@@ -358,10 +359,14 @@ def target_leg(D, gfc, tol, method, adj_tol, nsample=4000, seed=1):
     for key in settings:
         fro, col, gate = errs(Js[key])
         mixed[key].update(speedup=t_plain / mixed[key]["s"], J_rel_err_fro=fro, J_col_rel_err_max_per_gate=col,
-                          J_err_over_gate_max_per_gate=gate)
+                          J_err_over_gate_max_per_gate=gate, meets_J_target=bool(max(gate) <= 1e-4))
     fro_p, col_p, gate_p = errs(Jp_s)
+    ok = [k for k in settings if mixed[k]["meets_J_target"]]
+    best = min(ok, key=lambda k: mixed[k]["s"]) if ok else None
     out = {"rhs": 2 * 2048, "plain_s": t_plain, "plain_iters": its_p, "mixed": mixed,
-           "speedup": mixed["tol_adj"]["speedup"], "speedup_uniform_1e-8": mixed["uniform_1e-8"]["speedup"],
+           "speedup": mixed[best]["speedup"] if best else None, "speedup_setting": best,
+           "J_target": "largest |J - J64| of every gate <= 1e-4 of the gate's largest |J64|",
+           "speedup_uniform_1e-8": mixed["uniform_1e-8"]["speedup"],
            "sampled_nodes": nsample,
            "fp64_field_J_rel_err_fro": fro_p, "fp64_field_J_col_rel_err_max_per_gate": col_p,
            "note": "mixed = COCG c64 + FP64 refinement with frequency continuation, warm handle with FSAI setup; "
@@ -434,8 +439,8 @@ def lm_leg(max_iter=10):
     a 0.75 mm mesh of the slab with two inclusions, 32 x 32 pairs x 8 gates,
     Poisson noise at 1e6 peak counts; inversion on the 1 mm (269k-node) mesh
     from the homogeneous background, in the mixed COCG form with frequency
-    continuation at two settings (every column to 1e-8; forward columns to
-    5e-8 and adjoint columns to 1e-6) and in the reference-structure FP64
+    continuation at three settings (every column to 1e-8; forward columns
+    to 5e-8 and adjoint columns to 1e-6 or 1e-7) and in the reference-structure FP64
     form (1e-12); the reconstructed mua of each mixed setting against the
     FP64 form's."""
     import gc
@@ -461,7 +466,8 @@ def lm_leg(max_iter=10):
     truth = m.mua.copy()
     xs = np.arange(8.0, 73.0, 8.0)
     pos_s = np.stack(np.meshgrid(xs, xs, [30.0, 25.0, 20.0], indexing="ij"), -1).reshape(-1, 3)
-    runs = (("mixed", "cocg", 1e-8, None), ("mixed_tol_adj", "cocg", 5e-8, 1e-6), ("plain", "bicgstab", 1e-12, None))
+    runs = (("mixed", "cocg", 1e-8, None), ("mixed_tol_adj", "cocg", 5e-8, 1e-6), ("mixed_tol_adj10", "cocg", 5e-8, 1e-7),
+            ("plain", "bicgstab", 1e-12, None))
     for mode, method, tol, atol in runs:
         D = dm.DMesh(m.node, m.elem, m.nr, np.stack([np.full(m.node.shape[0], 0.01), m.musp], 1))
         D.add_optode(dm.DOptode(src, det, link, 1e3, tpsf, t, 64))
@@ -485,10 +491,11 @@ def lm_leg(max_iter=10):
         gc.collect()
         torch.cuda.empty_cache()
     b = out.pop("plain_mua")
-    for key in ("mixed", "mixed_tol_adj"):
+    for key in ("mixed", "mixed_tol_adj", "mixed_tol_adj10"):
         a = out.pop(key + "_mua")
         out[key]["mua_rel_diff_vs_plain"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
         out[key]["speedup_vs_plain"] = out["plain"]["s"] / out[key]["s"]
+        out[key]["within_0.1pct_of_plain"] = bool(out[key]["mua_rel_diff_vs_plain"] <= 1e-3)
     out["mixed_vs_plain_mua_rel_diff"] = out["mixed"]["mua_rel_diff_vs_plain"]
     out["speedup_mixed_vs_plain"] = out["mixed"]["speedup_vs_plain"]
     return out

@@ -1215,6 +1215,9 @@ __global__ __launch_bounds__(256) void k_add_correction(double* __restrict__ x64
                                                         const int* __restrict__ istatus, int first, Work W) {
   Item it;
   if (!work_item(W, it)) return;
+  // a group none of whose columns took part in the sweep is left as it is
+  if (!first && !__syncthreads_or((int)threadIdx.x < NSP && istatus[(long long)it.gl * NSP + threadIdx.x] != ST_SKIP))
+    return;
   const int nr = it.row1 - it.row0;
   for (int i = threadIdx.x; i < nr * NSP; i += 256) {
     const int c = i % NSP;
@@ -1329,7 +1332,7 @@ struct BState {
 // All threads of the (kScalThreads-wide) block call it; thread c < nsp
 // receives column c's sum.  Each column is reduced by kScalThreads/nsp
 // threads in parallel.
-constexpr int kScalThreads = 256;
+constexpr int kScalThreads = 1024;
 __device__ inline void block_sum_part(const double* __restrict__ part, const Work& W, int gl, int nsp,
                                       double& re, double& im) {
   __shared__ double sr[kScalThreads], si[kScalThreads];
