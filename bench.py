@@ -520,7 +520,8 @@ def main(argv=None):
     ap.add_argument("--method", default="cocg", choices=["bicgstab", "cocg"],
                     help="Krylov method of the headline mixed solve")
     ap.add_argument("--no-alt", action="store_true",
-                    help="skip the mixed solve with the other method and without frequency continuation")
+                    help="skip the mixed solves with the other method, with Lagrange continuation and "
+                         "without continuation")
     ap.add_argument("--adj-tol", type=float, default=1e-6,
                     help="relative FP64 residual of the adjoint columns in the forward + Jacobian leg "
                          "(the Jacobian's 1e-4 error target)")
@@ -607,15 +608,17 @@ def main(argv=None):
         extra[f"mixed_{alt}_vs_{a.method}_rel_diff_max"] = float(err.max())
         del pha
         release(D)
-        log("mixed solve without frequency continuation (every RHS from 0)")
-        dtn, phn, _ = timed_solves(D, "s", "mixed", a.tol, 1, 1, method=a.method, cont=False)
-        err = rel_err_per_rhs(phn.data.permute(0, 2, 1).reshape(-1, D.nn),
-                              x_mixed.permute(0, 2, 1).reshape(-1, D.nn))
-        extra["mixed_no_continuation_s_per_solve"] = dtn
-        extra["mixed_no_continuation_total_iters"] = int(phn.iters.sum())
-        extra["mixed_no_continuation_vs_continuation_rel_diff_max"] = float(err.max())
-        del phn
-        release(D)
+        for cont, name in (("lagrange", "mixed_lagrange"), (False, "mixed_no_continuation")):
+            log(f"mixed solve, continuation {cont}")
+            dtn, phn, _ = timed_solves(D, "s", "mixed", a.tol, 1, 1, method=a.method, cont=cont)
+            err = rel_err_per_rhs(phn.data.permute(0, 2, 1).reshape(-1, D.nn),
+                                  x_mixed.permute(0, 2, 1).reshape(-1, D.nn))
+            extra[f"{name}_s_per_solve"] = dtn
+            extra[f"{name}_total_iters"] = int(phn.iters.sum())
+            extra[f"{name}_iters_per_freq"] = phn.iters.max(axis=1).tolist()
+            extra[f"{name}_vs_galerkin_rel_diff_max"] = float(err.max())
+            del phn
+            release(D)
 
     names = ["G/G^T SpMV", "A SpMV", "Krylov vector passes", "FP64 refinement passes"]
     achieved = byts = nl = ms = None
