@@ -3,8 +3,10 @@ its full size: the 80 x 80 x 40 mm, 1 mm Kuhn slab (269,001 nodes, 1,536,000
 tetrahedra) with a 6 mm sphere of mua 0.03/mm, 32 sources, 64 frequencies
 0-3.15 GHz (2048 RHS), mixed COCG with frequency continuation to 5e-8.
 
-The relative residual ||b - A_k x|| / ||b|| of sampled columns at 0, 1.6 and
-3.15 GHz is computed off the solver's path: A_k from the oracle's own FP64
+The relative residual ||b - A_k x|| / ||b|| of every column at sampled
+frequencies (0, 1.6 and 3.15 GHz, solved from x0 = 0, and 50, 100, 200,
+400 MHz and 3.05 GHz, started from Galerkin guesses) is computed off the
+solver's path: A_k from the oracle's own FP64
 assembly (dMesh.m:1033-1244), b from the oracle's source matrix on the
 product's placement (dMesh.m:124-129), the product in scipy."""
 import numpy as np
@@ -35,7 +37,9 @@ def test_full_size_residuals_off_solver_path():
     om = fem.make_mesh(m.node, m.elem, m.nr, m.mua, m.musp)
     A = fem.full_system(om, opt.df)
     eid, w = opt.s_bary
-    ks = (0, 32, 63)
+    # level-0 frequencies (solved from x0 = 0) and frequencies of every later
+    # continuation level (started from Galerkin guesses): 8, 4, 2, 1 and 61
+    ks = (0, 1, 2, 4, 8, 32, 61, 63)
     Q = oopt.source_matrix(om, np.asarray(eid), np.asarray(w), opt.s_fpsf[list(ks)])  # NN x 3 x NS
     X = fw.to_reference_layout()
     for q, k in enumerate(ks):
