@@ -1825,16 +1825,59 @@ __global__ __launch_bounds__(256) void k_rb_mx(const int* __restrict__ rptr, con
   block_reduce_store<double, NSP>(d1, sh, part1, it.wl);
 }
 
-// out[i*nsp + c] = column c's sum of part1 over the items of chunk-local
-// group glist[i] (one block per listed group).
+// out[g*nsp + c] = column c's sum of part1 over the items of chunk-local
+// group g = glist[i] (one block per listed group).
 __global__ __launch_bounds__(kScalThreads) void k_rb_csum(const double* __restrict__ part1,
                                                           double* __restrict__ out, const int* __restrict__ glist,
                                                           int nsp, Work W) {
+  const int g = glist[blockIdx.x];
   double re = 0.0, im = 0.0;
-  block_sum_part(part1, W, glist[blockIdx.x], nsp, re, im);
+  block_sum_part(part1, W, g, nsp, re, im);
   if ((int)threadIdx.x < nsp) {
-    out[2 * ((long long)blockIdx.x * nsp + threadIdx.x)] = re;
-    out[2 * ((long long)blockIdx.x * nsp + threadIdx.x) + 1] = im;
+    out[2 * ((long long)g * nsp + threadIdx.x)] = re;
+    out[2 * ((long long)g * nsp + threadIdx.x) + 1] = im;
+  }
+}
+
+// Weights of every (target, column) of a phase, one thread each, from the
+// Gram entries and the dots x^H b (rb_weights).  Per target t, tinfo holds
+// 2 + kRbTerms^2 + kRbTerms ints: the window size n, the columns holding a
+// right-hand side, the code 2 * pair + conjugate of Gram entry (a, b) (-1
+// outside the window) and each window solution's chunk-local group; tspec
+// holds 3 kRbTerms + 3 doubles: the spectra (complex) and scales of the
+// window, then the target's spectrum and scale.  w[(t*kRbTerms + i)*nsp + c]
+// receives weight i (zero for padding columns).
+__global__ __launch_bounds__(64) void k_rb_weights(const int* __restrict__ tinfo, const double* __restrict__ tspec,
+                                                   const double* __restrict__ gram, const double* __restrict__ cdot,
+                                                   double* __restrict__ w, int ntgt, int nsp) {
+  constexpr int K = kRbTerms, TI = 2 + K * K + K, TS = 3 * K + 3;
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (long long)ntgt * nsp) return;
+  const int t = (int)(id / nsp), c = (int)(id % nsp);
+  const int* ti = tinfo + (size_t)t * TI;
+  const double* ts = tspec + (size_t)t * TS;
+  const int n = ti[0];
+  double G[2 * K * K], cq[2 * K], y[2 * K];
+  for (int i = 0; i < 2 * K; ++i) y[i] = 0.0;
+  if (c < ti[1] && n > 0) {
+    for (int a = 0; a < n; ++a) {
+      // x^H q = x^H b / s (the window's spectra are nonzero)
+      const double* d = cdot + 2 * ((long long)ti[2 + K * K + a] * nsp + c);
+      const RbC q = rb_div(RbC{d[0], d[1]}, RbC{ts[2 * a], ts[2 * a + 1]});
+      cq[2 * a] = q.re;
+      cq[2 * a + 1] = q.im;
+      for (int b = 0; b < n; ++b) {
+        const int code = ti[2 + a * K + b];
+        const double* e = gram + 2 * ((long long)(code >> 1) * nsp + c);
+        G[2 * (a * n + b)] = e[0];
+        G[2 * (a * n + b) + 1] = (code & 1) ? -e[1] : e[1];
+      }
+    }
+    rb_weights<K>(n, G, cq, ts, ts + 2 * K, ts[3 * K], ts[3 * K + 1], ts[3 * K + 2], y);
+  }
+  for (int i = 0; i < K; ++i) {
+    w[2 * (((long long)t * K + i) * nsp + c)] = y[2 * i];
+    w[2 * (((long long)t * K + i) * nsp + c) + 1] = y[2 * i + 1];
   }
 }
 
@@ -2141,7 +2184,29 @@ struct dogpup_solver {
   bool cont_rb = true;
   std::vector<int> h_gsrc, h_pwgrp;
   std::vector<unsigned char> chunk_rb;
-  dogpup::Buf rb_part, rb_misc;
+  // the Galerkin plan of every chunk and phase (rb_build_plan): launch
+  // descriptors with offsets into rb_plan (the uploaded lists and tables),
+  // the per-group dots x^H b (rb_cdot), the Gram entries (rb_gram), the
+  // weights of one phase (rb_w) and the Gram kernel's block partials
+  struct RbGram {
+    size_t o_offa = 0, o_offb = 0, o_pia = 0, o_pib = 0;
+    int na = 0, nb = 0, np = 0, nrows = 0, R = 1;
+    long long out = 0;     // first pair id of the launch
+  };
+  struct RbGuess {
+    size_t o_offa = 0, o_offt = 0, o_src = 0;
+    int na = 0, nt = 0, nrows = 0, R = 1, w0 = 0;   // w0: first target in the phase's weights
+  };
+  struct RbPhase {
+    bool ok = false;
+    std::vector<int> mx_phases;   // phases whose solutions get M x and x^H b first
+    size_t o_glist = 0, o_tinfo = 0, o_tspec = 0;
+    int nglist = 0, ntgt = 0;
+    std::vector<RbGram> gram;
+    std::vector<RbGuess> guess;
+  };
+  std::vector<std::vector<RbPhase>> rb_phase;
+  dogpup::Buf rb_plan, rb_cdot, rb_gram, rb_w, rb_part;
   bool have_matrix = false;
   double stats[4] = {0, 0, 0, 0};
   int* h_counter = nullptr;  // pinned: [0] active RHS, [1] active groups, [2] their rows
@@ -2744,20 +2809,10 @@ __global__ void k_fill_int(int* __restrict__ a, int v, int n) {
 }
 
 // ------------------------------------------- Galerkin guesses, host side --
-using cd = std::complex<double>;
 static_assert(kRbTerms <= kRbWeightsMax, "window size");
 
-// Dots of one chunk's solve that the Galerkin guesses reuse: column-wise
-// x_a^H M x_b of solution pairs a <= b (key a * ngroups + b), x_g^H b_g of
-// each solution, and the phases whose M x is computed.
-struct RbCache {
-  std::map<long long, std::vector<cd>> gram;
-  std::map<int, std::vector<cd>> cdot;
-  std::vector<char> mx_done;
-};
-
-// Host data of one batch of launches, uploaded as one block: typed arrays
-// appended at 256-byte-aligned offsets.
+// Host data uploaded as one block: typed arrays appended at 256-byte-aligned
+// offsets.
 struct RbUpload {
   std::vector<char> bytes;
   template <typename V>
@@ -2767,143 +2822,248 @@ struct RbUpload {
     if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(V));
     return off;
   }
-  size_t reserve(size_t nbytes) {
-    const size_t off = (bytes.size() + 255) & ~size_t(255);
-    bytes.resize(off + std::max<size_t>(nbytes, 1));
-    return off;
-  }
 };
 
-// Galerkin guesses of phase ip (>= 1) of a chunk (see k_rb_*): M x and x^H b
-// of the solutions of earlier phases not yet done, the Gram entries the
-// phase's windows lack, then on the host the weights of every target column,
-// and the guesses.  *done = false leaves the phase to the Lagrange guess
-// (its windows do not fit the LDS staging).  x64 (xc) holds every earlier
-// phase's converged solutions; bperm receives M x at the same offsets.
-template <int NSP>
-static int rb_guess_phase(dogpup_solver* S, hipStream_t st, const Work& Wfull, double* p1,
-                          const std::vector<dogpup_solver::Phase>& phases, size_t ip, const double* bc, double* xc,
-                          RbCache& cache, bool* done) {
-  constexpr int PQ = 256 / NSP;
-  constexpr int kMaxPairs = PQ * kRbPairsPerThread, kMaxTgt = PQ * kRbTgtPerThread;
-  *done = false;
-  const int nt = S->guess_nterm, ng = S->ngroups;
-  if (nt != kRbTerms) return DOGPUP_OK;
-  auto groups_of = [&](const dogpup_solver::Phase& ph) {
-    std::vector<int> gs;
-    for (int w = ph.w0; w < ph.w0 + ph.nw; ++w)
-      if (gs.empty() || gs.back() != S->h_pwgrp[w]) gs.push_back(S->h_pwgrp[w]);
-    return gs;
-  };
-  auto family = [&](int g) { return (long long)S->h_gmesh[g] * 65536 + S->h_gcb[g]; };
-  auto off_of = [&](int g) { return S->h_gvb[g] - Wfull.vb0; };
-  auto scale = [&](int g) { return S->h_fscale[S->m_fs0[S->h_gmesh[g]] + S->h_gf[g]]; };
-  auto spec = [&](int g) {
-    if (S->h_spec.empty()) return cd(1.0);
-    const int f = S->m_fs0[S->h_gmesh[g]] + S->h_gf[g];
-    return cd(S->h_spec[2 * f], S->h_spec[2 * f + 1]);
-  };
-  const std::vector<int> tgt = groups_of(phases[ip]);
-  // solutions whose M x this call computes (phases not done yet)
-  cache.mx_done.resize(phases.size(), 0);
-  std::vector<char> is_new(ng, 0);
-  for (size_t q = 0; q < ip; ++q)
-    if (!cache.mx_done[q])
-      for (int g : groups_of(phases[q])) is_new[g] = 1;
-  // the windows, the solutions they read and the pairs they need, per mesh
-  // and column block; a pair (a, b) is computed as x_a^H (M x_b) with a new
-  // solution on the M x side where there is one, so that a launch stages
-  // the M x of the new solutions only
-  std::map<long long, std::vector<int>> fam_tgt, fam_src;
-  std::map<long long, std::vector<std::pair<int, int>>> fam_pairs;
-  for (int t : tgt) {
-    const int* w = &S->h_gsrc[(size_t)t * nt];
-    const long long fk = family(t);
-    fam_tgt[fk].push_back(t);
-    for (int i = 0; i < nt && w[i] >= 0; ++i) {
-      fam_src[fk].push_back(w[i]);
-      for (int j = i; j < nt && w[j] >= 0; ++j) {
-        const int lo = std::min(w[i], w[j]), hi = std::max(w[i], w[j]);
-        if (cache.gram.count((long long)lo * ng + hi)) continue;
-        if (!is_new[hi] && is_new[lo]) fam_pairs[fk].push_back({hi, lo});
-        else fam_pairs[fk].push_back({lo, hi});
-      }
-    }
-  }
-  for (auto& kv : fam_src) {
-    std::sort(kv.second.begin(), kv.second.end());
-    kv.second.erase(std::unique(kv.second.begin(), kv.second.end()), kv.second.end());
-    if ((size_t)kv.second.size() * NSP * 16 > (size_t)kRbLds) return DOGPUP_OK;
-  }
-  for (auto& kv : fam_pairs) {
-    std::sort(kv.second.begin(), kv.second.end());
-    kv.second.erase(std::unique(kv.second.begin(), kv.second.end()), kv.second.end());
-  }
-  // 1. M x and x^H b of the phases before ip not done yet
-  std::vector<int> newg;
+// The plan of every chunk's Galerkin guesses (dogpup_solver::RbPhase, one
+// per phase), built with the phases: which phases' solutions get M x at the
+// start of a phase, the Gram pairs its windows still need (each pair once
+// over the solve, a new solution on the M x side where there is one), the
+// per-target tables of the weight kernel and the guess launches.  It
+// depends on the schedule only, so the solve issues the launches without
+// waiting for any result.  A chunk whose windows do not fit the LDS staging
+// keeps the Lagrange guesses (chunk_rb = 0).
+static int upload(Buf& b, const void* h, size_t bytes);
+static int rb_build_plan(dogpup_solver* S) {
+  S->rb_phase.clear();
+  const int nsp = S->nsp, ng = S->ngroups, nt = S->guess_nterm;
+  const int PQ = 256 / nsp;
+  const int kMaxPairs = PQ * kRbPairsPerThread, kMaxTgt = PQ * kRbTgtPerThread;
   RbUpload up;
-  std::vector<std::pair<size_t, size_t>> mx_ranges;   // (phase, first index in newg)
-  for (size_t q = 0; q < ip; ++q) {
-    if (cache.mx_done[q]) continue;
-    mx_ranges.push_back({q, newg.size()});
-    for (int g : groups_of(phases[q])) newg.push_back(g - Wfull.g0);
-  }
-  const size_t o_glist = up.add(newg);
-  // 2. the pairs of every mesh and column block in launches of at most
-  // kMaxPairs
-  struct GramLaunch {
-    size_t o_offa, o_offb, o_pia, o_pib, o_out;
-    int na, nb, np, nrows, R;
-    std::vector<std::pair<int, int>> pairs;
-  };
-  std::vector<GramLaunch> gl;
-  size_t npair_total = 0;
-  for (auto& kv : fam_pairs) {
-    const auto& pr = kv.second;
-    for (size_t i0 = 0; i0 < pr.size(); i0 += kMaxPairs) {
-      GramLaunch L;
-      L.pairs.assign(pr.begin() + i0, pr.begin() + std::min(pr.size(), i0 + (size_t)kMaxPairs));
-      std::vector<int> la, lb;
-      for (auto& ab : L.pairs) {
-        la.push_back(ab.first);
-        lb.push_back(ab.second);
-      }
-      std::sort(la.begin(), la.end());
-      la.erase(std::unique(la.begin(), la.end()), la.end());
-      std::sort(lb.begin(), lb.end());
-      lb.erase(std::unique(lb.begin(), lb.end()), lb.end());
-      const size_t row_bytes = (la.size() + lb.size()) * NSP * 16;
-      if (row_bytes > (size_t)kRbLds) return DOGPUP_OK;
-      L.R = (int)std::min<size_t>(8, kRbLds / row_bytes);
-      std::vector<long long> oa, ob;
-      for (int g : la) oa.push_back(off_of(g));
-      for (int g : lb) ob.push_back(off_of(g));
-      std::vector<int> pia, pib;
-      for (auto& ab : L.pairs) {
-        pia.push_back((int)(std::lower_bound(la.begin(), la.end(), ab.first) - la.begin()));
-        pib.push_back((int)(std::lower_bound(lb.begin(), lb.end(), ab.second) - lb.begin()));
-      }
-      L.na = (int)la.size();
-      L.nb = (int)lb.size();
-      L.np = (int)L.pairs.size();
-      L.nrows = S->h_grows[L.pairs[0].first];
-      L.o_offa = up.add(oa);
-      L.o_offb = up.add(ob);
-      L.o_pia = up.add(pia);
-      L.o_pib = up.add(pib);
-      L.o_out = npair_total;
-      npair_total += L.np;
-      gl.push_back(std::move(L));
+  long long npair = 0;
+  int max_tgt = 0;
+  for (size_t c = 0; c < S->chunk_phases.size(); ++c) {
+    std::vector<dogpup_solver::RbPhase> plan(S->chunk_phases[c].size());
+    if (c >= S->chunk_rb.size() || !S->chunk_rb[c] || nt != kRbTerms) {
+      S->rb_phase.push_back(plan);
+      continue;
     }
+    const auto& phases = S->chunk_phases[c];
+    const int g0 = S->chunk_g[c];
+    const long long vb0 = S->h_gvb[g0];
+    auto groups_of = [&](const dogpup_solver::Phase& ph) {
+      std::vector<int> gs;
+      for (int w = ph.w0; w < ph.w0 + ph.nw; ++w)
+        if (gs.empty() || gs.back() != S->h_pwgrp[w]) gs.push_back(S->h_pwgrp[w]);
+      return gs;
+    };
+    auto family = [&](int g) { return (long long)S->h_gmesh[g] * 65536 + S->h_gcb[g]; };
+    auto off_of = [&](int g) { return S->h_gvb[g] - vb0; };
+    auto scale = [&](int g) { return S->h_fscale[S->m_fs0[S->h_gmesh[g]] + S->h_gf[g]]; };
+    auto spec = [&](int g, double* re, double* im) {
+      *re = 1.0;
+      *im = 0.0;
+      if (S->h_spec.empty()) return;
+      const int f = S->m_fs0[S->h_gmesh[g]] + S->h_gf[g];
+      *re = S->h_spec[2 * f];
+      *im = S->h_spec[2 * f + 1];
+    };
+    std::vector<char> mx_done(phases.size(), 0);
+    // computed pairs: (lo, hi) -> (pair id, computed as x_hi^H M x_lo)
+    std::map<long long, std::pair<long long, bool>> pid;
+    bool ok = true;
+    for (size_t ip = 1; ip < phases.size() && ok; ++ip) {
+      if (!phases[ip].guess) continue;
+      dogpup_solver::RbPhase& P = plan[ip];
+      const std::vector<int> tgt = groups_of(phases[ip]);
+      std::vector<char> is_new(ng, 0);
+      std::vector<int> glist;
+      for (size_t q = 0; q < ip; ++q) {
+        if (mx_done[q]) continue;
+        mx_done[q] = 1;
+        P.mx_phases.push_back((int)q);
+        for (int g : groups_of(phases[q])) {
+          is_new[g] = 1;
+          glist.push_back(g - g0);
+        }
+      }
+      P.o_glist = up.add(glist);
+      P.nglist = (int)glist.size();
+      std::map<long long, std::vector<int>> fam_tgt, fam_src;
+      std::map<long long, std::vector<std::pair<int, int>>> fam_pairs;
+      for (int t : tgt) {
+        const int* w = &S->h_gsrc[(size_t)t * nt];
+        const long long fk = family(t);
+        fam_tgt[fk].push_back(t);
+        for (int i = 0; i < nt && w[i] >= 0; ++i) {
+          fam_src[fk].push_back(w[i]);
+          for (int j = i; j < nt && w[j] >= 0; ++j) {
+            const int lo = std::min(w[i], w[j]), hi = std::max(w[i], w[j]);
+            if (pid.count((long long)lo * ng + hi)) continue;
+            if (!is_new[hi] && is_new[lo]) fam_pairs[fk].push_back({hi, lo});
+            else fam_pairs[fk].push_back({lo, hi});
+          }
+        }
+      }
+      for (auto& kv : fam_src) {
+        std::sort(kv.second.begin(), kv.second.end());
+        kv.second.erase(std::unique(kv.second.begin(), kv.second.end()), kv.second.end());
+        ok &= kv.second.size() * nsp * 16 <= (size_t)kRbLds;
+      }
+      // Gram launches, at most kMaxPairs pairs each
+      for (auto& kv : fam_pairs) {
+        auto& pr = kv.second;
+        std::sort(pr.begin(), pr.end());
+        pr.erase(std::unique(pr.begin(), pr.end()), pr.end());
+        for (size_t i0 = 0; i0 < pr.size() && ok; i0 += kMaxPairs) {
+          const size_t i1 = std::min(pr.size(), i0 + (size_t)kMaxPairs);
+          std::vector<int> la, lb;
+          for (size_t i = i0; i < i1; ++i) {
+            la.push_back(pr[i].first);
+            lb.push_back(pr[i].second);
+          }
+          std::sort(la.begin(), la.end());
+          la.erase(std::unique(la.begin(), la.end()), la.end());
+          std::sort(lb.begin(), lb.end());
+          lb.erase(std::unique(lb.begin(), lb.end()), lb.end());
+          const size_t row_bytes = (la.size() + lb.size()) * nsp * 16;
+          if (row_bytes > (size_t)kRbLds) {
+            ok = false;
+            break;
+          }
+          dogpup_solver::RbGram L;
+          L.R = (int)std::min<size_t>(8, kRbLds / row_bytes);
+          std::vector<long long> oa, ob;
+          for (int g : la) oa.push_back(off_of(g));
+          for (int g : lb) ob.push_back(off_of(g));
+          std::vector<int> pia, pib;
+          for (size_t i = i0; i < i1; ++i) {
+            pia.push_back((int)(std::lower_bound(la.begin(), la.end(), pr[i].first) - la.begin()));
+            pib.push_back((int)(std::lower_bound(lb.begin(), lb.end(), pr[i].second) - lb.begin()));
+            const int lo = std::min(pr[i].first, pr[i].second), hi = std::max(pr[i].first, pr[i].second);
+            pid[(long long)lo * ng + hi] = {npair + (long long)(i - i0), pr[i].first > pr[i].second};
+          }
+          L.na = (int)la.size();
+          L.nb = (int)lb.size();
+          L.np = (int)(i1 - i0);
+          L.nrows = S->h_grows[pr[i0].first];
+          L.o_offa = up.add(oa);
+          L.o_offb = up.add(ob);
+          L.o_pia = up.add(pia);
+          L.o_pib = up.add(pib);
+          L.out = npair;
+          npair += L.np;
+          P.gram.push_back(L);
+        }
+      }
+      if (!ok) break;
+      // the weight kernel's table per target: window size, columns, pair
+      // codes (2 * pair id + conjugate) of the window's Gram entries, the
+      // window's groups (chunk-local, for x^H b), spectra and scales
+      std::vector<int> tinfo;
+      std::vector<double> tspec;
+      std::vector<int> order_t;   // targets in the order of the guess launches
+      for (auto& kv : fam_tgt)
+        for (int t : kv.second) order_t.push_back(t);
+      for (int t : order_t) {
+        const int* win = &S->h_gsrc[(size_t)t * nt];
+        int n = 0;
+        while (n < nt && win[n] >= 0) ++n;
+        tinfo.push_back(n);
+        tinfo.push_back(S->h_gcols[t]);
+        for (int a = 0; a < kRbTerms; ++a)
+          for (int b = 0; b < kRbTerms; ++b) {
+            int code = -1;
+            if (a < n && b < n) {
+              const int lo = std::min(win[a], win[b]), hi = std::max(win[a], win[b]);
+              const auto& e = pid.at((long long)lo * ng + hi);
+              // stored value x_lo^H M x_hi, or its conjugate when computed
+              // the other way round; entry (a, b) is x_win[a]^H M x_win[b]
+              const bool conj = e.second != (win[a] > win[b]);
+              code = (int)(2 * e.first + (conj ? 1 : 0));
+            }
+            tinfo.push_back(code);
+          }
+        for (int a = 0; a < kRbTerms; ++a) tinfo.push_back(a < n ? win[a] - g0 : 0);
+        double re, im;
+        for (int a = 0; a < kRbTerms; ++a) {
+          if (a < n) spec(win[a], &re, &im);
+          else re = im = 0.0;
+          tspec.push_back(re);
+          tspec.push_back(im);
+        }
+        for (int a = 0; a < kRbTerms; ++a) tspec.push_back(a < n ? scale(win[a]) : 0.0);
+        spec(t, &re, &im);
+        tspec.push_back(re);
+        tspec.push_back(im);
+        tspec.push_back(scale(t));
+      }
+      P.o_tinfo = up.add(tinfo);
+      P.o_tspec = up.add(tspec);
+      P.ntgt = (int)order_t.size();
+      max_tgt = std::max(max_tgt, P.ntgt);
+      // guess launches: at most kMaxTgt targets of one mesh and column block
+      int tpos = 0;
+      for (auto& kv : fam_tgt) {
+        const std::vector<int>& src = fam_src[kv.first];
+        const auto& tg = kv.second;
+        for (size_t i0 = 0; i0 < tg.size(); i0 += kMaxTgt) {
+          const size_t i1 = std::min(tg.size(), i0 + (size_t)kMaxTgt);
+          std::vector<long long> oa, ot;
+          for (int g : src) oa.push_back(off_of(g));
+          std::vector<int> si;
+          for (size_t i = i0; i < i1; ++i) {
+            const int t = tg[i];
+            const int* win = &S->h_gsrc[(size_t)t * nt];
+            ot.push_back(off_of(t));
+            for (int q = 0; q < kRbTerms; ++q)
+              si.push_back(q < nt && win[q] >= 0
+                               ? (int)(std::lower_bound(src.begin(), src.end(), win[q]) - src.begin())
+                               : -1);
+          }
+          dogpup_solver::RbGuess L;
+          L.na = (int)src.size();
+          L.nt = (int)(i1 - i0);
+          L.nrows = S->h_grows[tg[i0]];
+          L.R = (int)std::min<size_t>(8, kRbLds / ((size_t)std::max(L.na, 1) * nsp * 16));
+          L.o_offa = up.add(oa);
+          L.o_offt = up.add(ot);
+          L.o_src = up.add(si);
+          L.w0 = tpos + (int)i0;
+          P.guess.push_back(L);
+        }
+        tpos += (int)tg.size();
+      }
+      P.ok = true;
+    }
+    if (!ok) {
+      S->chunk_rb[c] = 0;
+      plan.assign(phases.size(), dogpup_solver::RbPhase{});
+    }
+    S->rb_phase.push_back(plan);
   }
-  const size_t o_cdot = up.reserve(newg.size() * NSP * 16);
-  const size_t o_gram = up.reserve(npair_total * NSP * 16);
-  DP_TRY(S->rb_misc.ensure(up.bytes.size()));
-  DP_TRY(S->rb_part.ensure((size_t)kRbBlocks * kMaxPairs * NSP * 16));
-  char* dm = S->rb_misc.as<char>();
-  DP_HIP(hipMemcpyAsync(dm, up.bytes.data(), up.bytes.size(), hipMemcpyHostToDevice, st));
-  for (auto& mr : mx_ranges) {
-    const dogpup_solver::Phase& ph = phases[mr.first];
+  if (up.bytes.empty()) return DOGPUP_OK;
+  DP_TRY(upload(S->rb_plan, up.bytes.data(), up.bytes.size()));
+  const size_t max_groups = (size_t)ng;
+  DP_TRY(S->rb_cdot.ensure(max_groups * nsp * 16));
+  DP_TRY(S->rb_gram.ensure((size_t)std::max<long long>(npair, 1) * nsp * 16));
+  DP_TRY(S->rb_w.ensure((size_t)std::max(max_tgt, 1) * kRbTerms * nsp * 16));
+  DP_TRY(S->rb_part.ensure((size_t)kRbBlocks * kMaxPairs * nsp * 16));
+  return DOGPUP_OK;
+}
+
+// The launches of phase ip's Galerkin guesses (rb_build_plan): M x and
+// x^H b of the planned phases' solutions, the Gram entries, the weights on
+// the device, and the guesses, all on the stream without a host wait.
+// x64 (xc) holds every earlier phase's converged solutions; bperm receives
+// M x at the same offsets.
+template <int NSP>
+static int rb_run_phase(dogpup_solver* S, hipStream_t st, int c, const Work& Wfull, double* p1,
+                        const std::vector<dogpup_solver::Phase>& phases, size_t ip, const double* bc, double* xc) {
+  const dogpup_solver::RbPhase& P = S->rb_phase[c][ip];
+  const char* dm = S->rb_plan.as<char>();
+  double* cdot = S->rb_cdot.as<double>();
+  for (int q : P.mx_phases) {
+    const dogpup_solver::Phase& ph = phases[q];
     Work Wp = Wfull;
     Wp.wgrp = S->t_pwgrp.as<int>();
     Wp.wtile = S->t_pwtile.as<int>();
@@ -2914,13 +3074,13 @@ static int rb_guess_phase(dogpup_solver* S, hipStream_t st, const Work& Wfull, d
                                                         S->bperm.as<double>(), bc, S->order.as<int>(), p1, Wp,
                                                         S->halo_ilv ? 1 : 0);
     DP_LAUNCH_CHECK();
-    const size_t i0 = mr.second;
-    const size_t n = groups_of(ph).size();
-    k_rb_csum<<<dim3((unsigned)n), kScalThreads, 0, st>>>(p1, reinterpret_cast<double*>(dm + o_cdot) + 2 * i0 * NSP,
-                                                          reinterpret_cast<const int*>(dm + o_glist) + i0, NSP, Wp);
+  }
+  if (P.nglist > 0) {
+    k_rb_csum<<<dim3((unsigned)P.nglist), kScalThreads, 0, st>>>(
+        p1, cdot, reinterpret_cast<const int*>(dm + P.o_glist), NSP, Wfull);
     DP_LAUNCH_CHECK();
   }
-  for (auto& L : gl) {
+  for (const auto& L : P.gram) {
     const int ntile = (L.nrows + L.R - 1) / L.R, nblk = std::min(kRbBlocks, ntile);
     const size_t lds = (size_t)(L.na + L.nb) * L.R * NSP * 16;
     k_rb_gram<NSP><<<dim3((unsigned)nblk), 256, lds, st>>>(
@@ -2929,113 +3089,25 @@ static int rb_guess_phase(dogpup_solver* S, hipStream_t st, const Work& Wfull, d
         reinterpret_cast<const int*>(dm + L.o_pib), L.np, L.nrows, L.R, S->rb_part.as<double>());
     DP_LAUNCH_CHECK();
     const int n = L.np * NSP;
-    k_rb_reduce<<<dim3((unsigned)div_up(n, 256)), 256, 0, st>>>(
-        S->rb_part.as<double>(), reinterpret_cast<double*>(dm + o_gram) + 2 * L.o_out * NSP, n, nblk);
+    k_rb_reduce<<<dim3((unsigned)div_up(n, 256)), 256, 0, st>>>(S->rb_part.as<double>(),
+                                                               S->rb_gram.as<double>() + 2 * L.out * NSP, n, nblk);
     DP_LAUNCH_CHECK();
   }
-  std::vector<cd> h_cdot(newg.size() * NSP), h_gram(npair_total * NSP);
-  if (!h_cdot.empty())
-    DP_HIP(hipMemcpyAsync(h_cdot.data(), dm + o_cdot, h_cdot.size() * 16, hipMemcpyDeviceToHost, st));
-  if (!h_gram.empty())
-    DP_HIP(hipMemcpyAsync(h_gram.data(), dm + o_gram, h_gram.size() * 16, hipMemcpyDeviceToHost, st));
-  DP_HIP(hipStreamSynchronize(st));
-  for (size_t i = 0; i < newg.size(); ++i)
-    cache.cdot[newg[i] + Wfull.g0].assign(h_cdot.begin() + i * NSP, h_cdot.begin() + (i + 1) * NSP);
-  for (auto& mr : mx_ranges) cache.mx_done[mr.first] = 1;
-  for (auto& L : gl)
-    for (int p = 0; p < L.np; ++p) {
-      // stored as x_lo^H M x_hi (lo < hi); a pair computed the other way
-      // round gives its conjugate (M is real symmetric)
-      const auto& ab = L.pairs[p];
-      const size_t o = (L.o_out + p) * NSP;
-      std::vector<cd>& v = cache.gram[(long long)std::min(ab.first, ab.second) * ng + std::max(ab.first, ab.second)];
-      v.assign(h_gram.begin() + o, h_gram.begin() + o + NSP);
-      if (ab.first > ab.second)
-        for (cd& e : v) e = std::conj(e);
-    }
-  // 3. weights of every target column, and the guess launches
-  RbUpload up2;
-  struct GuessLaunch {
-    size_t o_offa, o_offt, o_src, o_w;
-    int na, nt, nrows, R;
-  };
-  std::vector<GuessLaunch> gsl;
-  for (auto& kv : fam_tgt) {
-    const std::vector<int>& src = fam_src[kv.first];
-    const auto& tg = kv.second;
-    for (size_t i0 = 0; i0 < tg.size(); i0 += kMaxTgt) {
-      const size_t i1 = std::min(tg.size(), i0 + (size_t)kMaxTgt);
-      std::vector<long long> oa, ot;
-      for (int g : src) oa.push_back(off_of(g));
-      std::vector<int> si;
-      std::vector<cd> w;
-      for (size_t i = i0; i < i1; ++i) {
-        const int t = tg[i];
-        const int* win = &S->h_gsrc[(size_t)t * nt];
-        int n = 0;
-        while (n < nt && win[n] >= 0) ++n;
-        ot.push_back(off_of(t));
-        for (int q = 0; q < kRbTerms; ++q)
-          si.push_back(q < n ? (int)(std::lower_bound(src.begin(), src.end(), win[q]) - src.begin()) : -1);
-        cd G[kRbTerms * kRbTerms], cq[kRbTerms], sj[kRbTerms], y[kRbTerms];
-        double fj[kRbTerms];
-        for (int q = 0; q < n; ++q) {
-          sj[q] = spec(win[q]);
-          fj[q] = scale(win[q]);
-        }
-        std::vector<cd> wt((size_t)kRbTerms * NSP, cd(0.0));
-        // the dots of the window (every one was computed above or before)
-        std::vector<const std::vector<cd>*> cdv(n), gv((size_t)n * n);
-        bool have = true;
-        for (int a = 0; a < n && have; ++a) {
-          auto ic = cache.cdot.find(win[a]);
-          have = ic != cache.cdot.end();
-          if (have) cdv[a] = &ic->second;
-          for (int b = 0; b < n && have; ++b) {
-            auto ig = cache.gram.find((long long)std::min(win[a], win[b]) * ng + std::max(win[a], win[b]));
-            have = ig != cache.gram.end();
-            if (have) gv[(size_t)a * n + b] = &ig->second;
-          }
-        }
-        for (int c = 0; have && c < S->h_gcols[t]; ++c) {
-          for (int a = 0; a < n; ++a) {
-            cq[a] = (*cdv[a])[c] / sj[a];
-            for (int b = 0; b < n; ++b) {
-              const cd v = (*gv[(size_t)a * n + b])[c];
-              G[a * n + b] = win[a] <= win[b] ? v : std::conj(v);
-            }
-          }
-          rb_weights(n, G, cq, sj, fj, spec(t), scale(t), y);
-          for (int q = 0; q < n; ++q) wt[(size_t)q * NSP + c] = y[q];
-        }
-        w.insert(w.end(), wt.begin(), wt.end());
-      }
-      GuessLaunch L;
-      L.na = (int)src.size();
-      L.nt = (int)(i1 - i0);
-      L.nrows = S->h_grows[tg[i0]];
-      L.R = (int)std::min<size_t>(8, kRbLds / ((size_t)std::max(L.na, 1) * NSP * 16));
-      L.o_offa = up2.add(oa);
-      L.o_offt = up2.add(ot);
-      L.o_src = up2.add(si);
-      L.o_w = up2.add(w);
-      gsl.push_back(L);
-    }
+  if (P.ntgt > 0) {
+    k_rb_weights<<<dim3(div_up((long long)P.ntgt * NSP, 64)), 64, 0, st>>>(
+        reinterpret_cast<const int*>(dm + P.o_tinfo), reinterpret_cast<const double*>(dm + P.o_tspec),
+        S->rb_gram.as<double>(), cdot, S->rb_w.as<double>(), P.ntgt, NSP);
+    DP_LAUNCH_CHECK();
   }
-  DP_TRY(S->rb_misc.ensure(up2.bytes.size()));
-  dm = S->rb_misc.as<char>();
-  DP_HIP(hipMemcpyAsync(dm, up2.bytes.data(), up2.bytes.size(), hipMemcpyHostToDevice, st));
-  for (auto& L : gsl) {
+  for (const auto& L : P.guess) {
     const int ntile = (L.nrows + L.R - 1) / L.R, nblk = std::min(kRbBlocks, ntile);
     const size_t lds = (size_t)std::max(L.na, 1) * L.R * NSP * 16;
     k_rb_guess<NSP><<<dim3((unsigned)nblk), 256, lds, st>>>(
         xc, reinterpret_cast<const long long*>(dm + L.o_offa), L.na, reinterpret_cast<const long long*>(dm + L.o_offt),
-        L.nt, reinterpret_cast<const int*>(dm + L.o_src), reinterpret_cast<const double*>(dm + L.o_w), L.nrows, L.R);
+        L.nt, reinterpret_cast<const int*>(dm + L.o_src), S->rb_w.as<double>() + 2 * (size_t)L.w0 * kRbTerms * NSP,
+        L.nrows, L.R);
     DP_LAUNCH_CHECK();
   }
-  // the host buffers above must outlive their copies
-  DP_HIP(hipStreamSynchronize(st));
-  *done = true;
   return DOGPUP_OK;
 }
 
@@ -3067,7 +3139,6 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
       DP_LAUNCH_CHECK();
       if (S->guess_self) DP_TRY(permute_block<NSP>(S, st, Wfull, x + goff, xc, 0));
     }
-    RbCache rb;
     for (size_t ip = 0; ip < phases.size(); ++ip) {
       const dogpup_solver::Phase& ph = phases[ip];
       C.W = Wfull;
@@ -3081,10 +3152,9 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
       C.phase = (int)ip;
       const double e64 = (double)ph.nodes * NSP;  // complex elements of the phase
       if (ph.guess) {
-        bool rb_done = false;
-        if (ip > 0 && c < (int)S->chunk_rb.size() && S->chunk_rb[c])
-          DP_TRY(rb_guess_phase<NSP>(S, st, Wfull, C.p1, phases, ip, bc, xc, rb, &rb_done));
-        if (!rb_done) {
+        const bool rb_done = c < (int)S->rb_phase.size() && ip < S->rb_phase[c].size() && S->rb_phase[c][ip].ok;
+        if (rb_done) DP_TRY(rb_run_phase<NSP>(S, st, c, Wfull, C.p1, phases, ip, bc, xc));
+        else {
           k_guess<NSP><<<C.vgrid(), 256, 0, st>>>(xc, C.W, S->t_gsrc.as<int>(), S->t_gw.as<double>(), S->guess_nterm,
                                                  x, S->order.as<int>());
           DP_LAUNCH_CHECK();
@@ -3673,6 +3743,7 @@ static int build_phases(dogpup_solver* S) {
   DP_TRY(upload(S->t_glevel, level.data(), level.size() * 4));
   S->h_gsrc = gsrc;
   S->h_pwgrp = pwg;
+  DP_TRY(rb_build_plan(S));
   S->clear_graphs();
   return DOGPUP_OK;
 }
