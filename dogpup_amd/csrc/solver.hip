@@ -31,7 +31,6 @@
 #include <cstring>
 #include <cstdlib>
 #include <algorithm>
-#include <complex>
 #include <map>
 #include <memory>
 #include <type_traits>
@@ -1777,9 +1776,9 @@ __global__ __launch_bounds__(kScalThreads) void k_plain_scal(BState s, int op,
 // A_j = K + i sigma_j M, so A_j x_j = b_j gives K x_j = s_j q - i sigma_j M x_j
 // and every entry of the small system follows from two kinds of dots:
 //   x_i^H A_k x_j = s_j (x_i^H q) + i (sigma_k - sigma_j) (x_i^H M x_j).
-// The kernels below compute M x_j with x_j^H b_j (k_rb_mx), the column-wise
-// M-Gram entries (k_rb_gram, k_rb_reduce) and the guesses from the host's
-// weights (k_rb_guess).
+// The kernels below compute M x_j with x_j^H b_j (k_rb_mx, k_rb_csum), the
+// column-wise M-Gram entries (k_rb_gram, k_rb_reduce), the weights of every
+// target column (k_rb_weights) and the guesses (k_rb_guess).
 constexpr int kRbTerms = 6;            // solutions in a guess's window
 constexpr int kRbPairsPerThread = 24;  // Gram pairs accumulated per thread
 constexpr int kRbTgtPerThread = 4;     // target groups per thread of k_rb_guess
