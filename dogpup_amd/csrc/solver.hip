@@ -2974,11 +2974,15 @@ static int rb_build_plan(dogpup_solver* S) {
             int code = -1;
             if (a < n && b < n) {
               const int lo = std::min(win[a], win[b]), hi = std::max(win[a], win[b]);
-              const auto& e = pid.at((long long)lo * ng + hi);
+              const auto e = pid.find((long long)lo * ng + hi);
+              if (e == pid.end()) {   // every window pair is planned above
+                ok = false;
+                break;
+              }
               // stored value x_lo^H M x_hi, or its conjugate when computed
               // the other way round; entry (a, b) is x_win[a]^H M x_win[b]
-              const bool conj = e.second != (win[a] > win[b]);
-              code = (int)(2 * e.first + (conj ? 1 : 0));
+              const bool conj = e->second.second != (win[a] > win[b]);
+              code = (int)(2 * e->second.first + (conj ? 1 : 0));
             }
             tinfo.push_back(code);
           }
@@ -2996,6 +3000,7 @@ static int rb_build_plan(dogpup_solver* S) {
         tspec.push_back(im);
         tspec.push_back(scale(t));
       }
+      if (!ok) break;
       P.o_tinfo = up.add(tinfo);
       P.o_tspec = up.add(tspec);
       P.ntgt = (int)order_t.size();
