@@ -215,11 +215,13 @@ def batch_leg(tol):
     log(f"batch: {len(meshes)} meshes, {out['nodes']} nodes, {nrhs} RHS; autotuning")
     t1 = time.time()
     best, table = autotune.tune(meshes, "s", "mixed", "cocg", use_cache=False, widths=(8, 16), tiles=(512, 1024),
-                                unrolls=(8,), iters=4)
+                                unrolls=(8,), iters=4, strides=(8, 16, 32), tol=tol)
     out["tune_s"] = time.time() - t1
     out["tuned"] = best
     out["tune_table_ms_per_iter"] = {f"nsp{r['nsp']}_tile{r['tile_rows']}_u{r['unroll']}": r["s_per_iter"] * 1e3
-                                     for r in table}
+                                     for r in table if "s_per_iter" in r}
+    out["tune_table_ms_per_solve"] = {f"cont_stride{r['cont_stride']}": r["s_per_solve"] * 1e3
+                                      for r in table if "s_per_solve" in r}
 
     def timed(B):
         b = B.rhs()
@@ -243,6 +245,7 @@ def batch_leg(tol):
     log(f"batch: tuned shapes {best}")
     B = batch.MeshBatch(meshes, "s", "mixed", nsp=best["nsp"], method="cocg",
                         tuning=(best["tile_rows"], best["unroll"], best["unroll"], 0))
+    B.set_cont_stride(best.get("cont_stride", 0))
     dt, fields, b = timed(B)
     out["mixed_s"] = dt
     out["rhs_per_s"] = nrhs / dt

@@ -54,6 +54,7 @@ SIGNATURES = {
     "dogpup_solver_create": [P(vp), i32, i32, i32, i32, i32, i32, vp, vp, i32, vp, vp, vp],
     "dogpup_batch_create": [P(vp), i32, vp, vp, vp, i32, i32, i32, vp, vp, i32, vp, vp, vp],
     "dogpup_solver_set_tuning": [vp, i32, i32, i32, i32],
+    "dogpup_solver_set_cont_stride": [vp, i32],
     "dogpup_solver_destroy": [vp],
     "dogpup_solver_set_matrix": [vp, vp, vp, vp, vp],
     "dogpup_solver_solve": [vp, vp, vp, f64, i32, vp, vp, vp, vp],

@@ -8,7 +8,12 @@ Krylov iterations of the real problem for each candidate:
 - row tile of the SpMV and vector passes: 256, 512 or 1024 rows.
 - x gathers in flight per row in the A and G SpMVs: 4 or 8.
 
-It keeps the fastest per RHS-iteration. Results are cached in a JSON file
+It keeps the fastest per RHS-iteration.  With ``strides`` (mixed mode) it then
+times whole solves of the chosen shape at ``tol`` with Galerkin frequency
+continuation for each level-0 spacing of the schedule (every stride-th
+frequency solved from zero): a wider spacing solves fewer frequencies from
+zero but runs more levels, each a synchronised phase with fewer groups, and
+the best spacing depends on the meshes' frequency counts. Results are cached in a JSON file
 (``$DOGPUP_TUNE_CACHE``, default ``~/.cache/dogpup_amd/tune.json``) under a
 signature of the problem: node, frequency and column counts per mesh, mode and
 method.
@@ -28,6 +33,7 @@ from .batch import MeshBatch
 TILES = (256, 512, 1024)
 UNROLLS = (4, 8)
 WIDTHS = (8, 16, 32)
+STRIDES = (8, 16, 32)
 
 
 def _cache_path():
@@ -55,12 +61,29 @@ def _time_iterations(B, iters):
     return dt / iters
 
 
+def _time_solve(B, tol):
+    """Seconds of a whole solve (after one that builds the schedule and
+    captures the graphs)."""
+    b = B.rhs()
+    B.solve(b, tol=tol)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    B.solve(b, tol=tol)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    del b
+    return dt
+
+
 def tune(meshes, which="s", mode="mixed", method="cocg", iters=6, widths=WIDTHS, tiles=TILES, unrolls=UNROLLS,
-         use_cache=True, verbose=False):
-    """Returns ({"nsp", "tile_rows", "unroll", "s_per_iter"}, table of every
-    candidate's seconds per iteration)."""
+         strides=None, tol=5e-8, use_cache=True, verbose=False):
+    """Returns ({"nsp", "tile_rows", "unroll", "s_per_iter"[, "cont_stride",
+    "s_per_solve"]}, table of every candidate: seconds per iteration, and
+    seconds per solve for the continuation spacings)."""
     meshes = list(meshes)
     key = signature(meshes, which, mode, method)
+    if strides and mode == "mixed":
+        key += json.dumps({"strides": list(strides), "tol": tol})
     path = _cache_path()
     cache = {}
     if use_cache and os.path.exists(path):
@@ -107,6 +130,22 @@ def tune(meshes, which="s", mode="mixed", method="cocg", iters=6, widths=WIDTHS,
         torch.cuda.empty_cache()
     if best is None:
         raise RuntimeError("no candidate shape fits in device memory")
+    if strides and mode == "mixed":
+        B = MeshBatch(meshes, which, mode=mode, nsp=best["nsp"], method=method,
+                      tuning=(best["tile_rows"], best["unroll"], best["unroll"], 0))
+        for st in strides:
+            B.set_cont_stride(st)
+            s = _time_solve(B, tol)
+            table.append({"nsp": best["nsp"], "tile_rows": best["tile_rows"], "unroll": best["unroll"],
+                          "cont_stride": st, "s_per_solve": s})
+            if verbose:
+                print(f"[tune] continuation stride {st:2d}: {s * 1e3:.1f} ms per solve", flush=True)
+            if "s_per_solve" not in best or s < best["s_per_solve"]:
+                best["cont_stride"], best["s_per_solve"] = st, s
+        del B
+        gc.collect()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
     if use_cache:
         cache[key] = {"best": best, "table": table}
         try:
@@ -119,7 +158,11 @@ def tune(meshes, which="s", mode="mixed", method="cocg", iters=6, widths=WIDTHS,
 
 
 def tuned_batch(meshes, which="s", mode="mixed", method="cocg", **kw):
-    """A MeshBatch with the tuned group width and kernel shapes."""
+    """A MeshBatch with the tuned group width, kernel shapes and (when tuned)
+    continuation spacing."""
     best, _ = tune(meshes, which, mode, method, **kw)
-    return MeshBatch(meshes, which, mode=mode, nsp=best["nsp"], method=method,
-                     tuning=(best["tile_rows"], best["unroll"], best["unroll"], 0))
+    B = MeshBatch(meshes, which, mode=mode, nsp=best["nsp"], method=method,
+                  tuning=(best["tile_rows"], best["unroll"], best["unroll"], 0))
+    if "cont_stride" in best:
+        B.set_cont_stride(best["cont_stride"])
+    return B

@@ -124,6 +124,11 @@ class MeshBatch:
     def set_tuning(self, tile_rows=0, unroll_a=0, unroll_g=0, graph_k=0):
         _lib.call("dogpup_solver_set_tuning", self.h, int(tile_rows), int(unroll_a), int(unroll_g), int(graph_k))
 
+    def set_cont_stride(self, stride=0):
+        """Level-0 spacing of the frequency continuation (a power of two >= 2;
+        0: the library's default), dogpup_solver_set_cont_stride."""
+        _lib.call("dogpup_solver_set_cont_stride", self.h, int(stride))
+
     def ensure_matrix(self):
         """Concatenated K, M and per-mesh frequency scales k*df/c; FSAI setup."""
         versions = [m._matrix_version for m in self.meshes]

@@ -2181,6 +2181,7 @@ struct dogpup_solver {
   // list, and per chunk whether every term lies in the chunk (else the
   // chunk takes the Lagrange guesses)
   bool cont_rb = true;
+  int cont_stride = 0;               // level-0 spacing; 0: the default of the guess kind
   std::vector<int> h_gsrc, h_pwgrp;
   std::vector<unsigned char> chunk_rb;
   // the Galerkin plan of every chunk and phase (rb_build_plan): launch
@@ -3550,16 +3551,16 @@ static_assert(kRbTerms <= kMaxGuessTerms, "guess terms");
 // Level of frequency index k: 0 for every stride-th and the last, then one
 // level per halving of the spacing.  DOGPUP_CONT_STRIDE (a power of two)
 // overrides the stride.
-static int cont_stride(bool rb) {
+static int cont_stride(const dogpup_solver* S) {
   static const int env = [] {
     const char* e = getenv("DOGPUP_CONT_STRIDE");
     const int v = e ? atoi(e) : 0;
     return (v >= 2 && !(v & (v - 1))) ? v : 0;
   }();
-  return env ? env : (rb ? 16 : 8);
+  if (S->cont_stride > 0) return S->cont_stride;
+  return env ? env : (S->cont_rb ? 16 : 8);
 }
-static int cont_level(int k, int nf, bool rb) {
-  const int st = cont_stride(rb);
+static int cont_level(int k, int nf, int st) {
   if (k % st == 0 || k == nf - 1) return 0;
   int L = 0, tz = 0;
   for (int v = st; v > 1; v >>= 1) ++L;
@@ -3654,12 +3655,12 @@ static int build_phases(dogpup_solver* S) {
         // the schedule: a later-level frequency interpolates the nearest
         // frequencies already solved, of lower levels in this chunk or of an
         // earlier chunk
-        const bool rb = S->cont_rb;
-        const int nfm = S->m_nf[m], k = S->h_gf[g], lk = cont_level(k, nfm, rb);
+        const int stride = cont_stride(S);
+        const int nfm = S->m_nf[m], k = S->h_gf[g], lk = cont_level(k, nfm, stride);
         auto grp = [&](int j) { return S->m_g0[m] + j * S->m_ncb[m] + S->h_gcb[g]; };
         auto avail = [&](int j) {
           const int sg = grp(j);
-          return sg < g0 || (sg < g1 && cont_level(j, nfm, rb) < lk);
+          return sg < g0 || (sg < g1 && cont_level(j, nfm, stride) < lk);
         };
         int sk[kMaxGuessTerms];
         double wk[2 * kMaxGuessTerms];
@@ -4205,6 +4206,18 @@ int dogpup_solver_set_tuning(dogpup_solver_t S, int tile_rows, int unroll_a, int
   if (tile_rows && tile_rows != S->tile_rows) {
     S->tile_rows = tile_rows;
     DP_TRY(build_work(S));
+  }
+  return DOGPUP_OK;
+}
+
+int dogpup_solver_set_cont_stride(dogpup_solver_t S, int stride) {
+  DP_ARG(S != nullptr, "NULL solver");
+  DP_ARG(stride == 0 || (stride >= 2 && !(stride & (stride - 1))), "stride must be a power of two >= 2 (got %d)",
+         stride);
+  if (stride != S->cont_stride) {
+    S->cont_stride = stride;
+    S->phases_dirty = true;
+    S->clear_graphs();
   }
   return DOGPUP_OK;
 }

@@ -174,6 +174,14 @@ int dogpup_batch_create(dogpup_solver_t* out, int nmesh, const int* h_row0, cons
  * between convergence polls.  Chosen by dogpup_amd.autotune. */
 int dogpup_solver_set_tuning(dogpup_solver_t s, int tile_rows, int unroll_a, int unroll_g, int graph_k);
 
+/* Level-0 spacing of the frequency continuation (see
+ * dogpup_solver_set_freq_continuation): every stride-th frequency and the
+ * last are solved from x0 = 0, each later level halves the spacing.  stride
+ * a power of two >= 2; 0 restores the default (16 with Galerkin guesses, 8
+ * with Lagrange guesses).  Chosen by dogpup_amd.autotune with the kernel
+ * shapes. */
+int dogpup_solver_set_cont_stride(dogpup_solver_t s, int stride);
+
 int dogpup_solver_destroy(dogpup_solver_t s);
 
 /* Set A (FP64 values on the handle's pattern; frequency k uses

@@ -92,3 +92,15 @@ def test_autotune_picks_a_measured_shape(tmp_path, monkeypatch):
     B = autotune.tuned_batch(meshes, "s", "mixed", "cocg")
     for f in B.solve(tol=1e-10):
         assert (f.status == 0).all()
+    # the continuation spacing: whole solves of the chosen shape per stride,
+    # the fastest kept and applied by tuned_batch; every spacing converges
+    best2, table2 = autotune.tune(meshes, "s", "mixed", "cocg", iters=3, widths=(8,), tiles=(512,), unrolls=(8,),
+                                  strides=(4, 8, 16), tol=1e-8)
+    solves = [r for r in table2 if "s_per_solve" in r]
+    assert [r["cont_stride"] for r in solves] == [4, 8, 16]
+    assert best2["s_per_solve"] == min(r["s_per_solve"] for r in solves)
+    assert best2["cont_stride"] == min(solves, key=lambda r: r["s_per_solve"])["cont_stride"]
+    B2 = autotune.tuned_batch(meshes, "s", "mixed", "cocg", iters=3, widths=(8,), tiles=(512,), unrolls=(8,),
+                              strides=(4, 8, 16), tol=1e-8)                  # served from the cache
+    for f in B2.solve(tol=1e-10):
+        assert (f.status == 0).all()
