@@ -2147,6 +2147,13 @@ struct dogpup_solver {
   std::vector<int> mesh_chb_tmp;   // first halo chunk of each mesh (while building)
   // values: a64re/a64im and g64 in reference order; the rest in solver order
   dogpup::Buf a64re, a64im, a64pre, a64pim, a32re, a32im, fscale, g64, gp64, gtp64, g32, gt32;
+  // FSAI straight into the solver layouts (every pattern row of at most 8
+  // entries): A positions of each row's lower triangle on its pattern (36 per
+  // row), each reference-order G entry's position in the G layout (ginv) and
+  // each G^T entry's position in the G layout (gtg); g64 (reference order,
+  // FP64) is then filled only on request
+  bool fsai_direct = false, g64_valid = false;
+  dogpup::Buf fsai_pos, ginv, gtg;
   dogpup::Buf ws, part1, part2, part3, state, rstate, bperm, xperm;
   int tile_rows = 512, lds_pad = 0;
   int unroll_a = 8, unroll_g = 4;  // gathers in flight per row (k_spmv: 4, 8 or 16)
@@ -2260,8 +2267,8 @@ __global__ void k_gather_real(const double* __restrict__ in, const int* __restri
 // One mesh's G values into solver order: for p in [p0, p0 + nnz) and every
 // frequency f < nf, out[base + f*nnz + p - p0] = in[base + f*nnz + map[p] - p0]
 // (complex; map stays within the mesh).
-template <typename TO>
-__global__ void k_gather_cplx(const double* __restrict__ in, const int* __restrict__ map, TO* __restrict__ out,
+template <typename TI, typename TO>
+__global__ void k_gather_cplx(const TI* __restrict__ in, const int* __restrict__ map, TO* __restrict__ out,
                               long long p0, long long nnz, int nf, long long base) {
   const long long total = nnz * nf;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
@@ -3826,7 +3833,7 @@ static int create_batch(dogpup_solver_t* out, int nmesh, const int* h_row0, cons
   permute(rA, cA, rAp, cAp, amap);
   permute(rG, cG, rGp, cGp, gmap);
   // G^T in solver order: counting sort of the permuted G by column
-  std::vector<int> rGT(nn + 1, 0), cGT(nnzG), gtmap(nnzG);
+  std::vector<int> rGT(nn + 1, 0), cGT(nnzG), gtmap(nnzG), gtg(nnzG);
   for (int v : cGp) rGT[v + 1]++;
   for (int i = 0; i < nn; ++i) rGT[i + 1] += rGT[i];
   {
@@ -3836,9 +3843,34 @@ static int create_batch(dogpup_solver_t* out, int nmesh, const int* h_row0, cons
         const int q = fill[cGp[p]]++;
         cGT[q] = i;
         gtmap[q] = gmap[p];
+        gtg[q] = p;
       }
   }
   auto up = [&](dogpup::Buf& b, const std::vector<int>& v) -> int { return upload(b, v.data(), v.size() * 4); };
+  // FSAI gather table: every pattern row of at most 8 entries, the A
+  // position of each lower-triangle pair found by the same ascending walk of
+  // A's row as the searching kernel
+  S->fsai_direct = true;
+  for (int r = 0; r < nn && S->fsai_direct; ++r) S->fsai_direct = rG[r + 1] - rG[r] <= 8;
+  if (S->fsai_direct) {
+    std::vector<int> pos((size_t)nn * 36, -1), ginv(nnzG);
+    for (int r = 0; r < nn; ++r) {
+      const int gs = rG[r], m = rG[r + 1] - gs;
+      for (int a = 0; a < m; ++a) {
+        const int ra = cG[gs + a], s1 = rA[ra + 1];
+        int p = rA[ra];
+        for (int b = 0; b <= a; ++b) {
+          const int cb = cG[gs + b];
+          while (p < s1 && cA[p] < cb) ++p;
+          if (p < s1 && cA[p] == cb) pos[(size_t)r * 36 + a * (a + 1) / 2 + b] = p;
+        }
+      }
+    }
+    for (int q = 0; q < nnzG; ++q) ginv[gmap[q]] = q;
+    DP_TRY(up(S->fsai_pos, pos));
+    DP_TRY(up(S->ginv, ginv));
+    DP_TRY(up(S->gtg, gtg));
+  }
   DP_TRY(up(S->pAr, rAp)); DP_TRY(up(S->pAc, cAp)); DP_TRY(up(S->amap, amap));
   DP_TRY(up(S->pGr, rGp)); DP_TRY(up(S->pGc, cGp)); DP_TRY(up(S->gmap, gmap));
   DP_TRY(up(S->pGTr, rGT)); DP_TRY(up(S->pGTc, cGT)); DP_TRY(up(S->gtmap, gtmap));
@@ -4001,6 +4033,21 @@ int dogpup_solver_destroy(dogpup_solver_t s) {
   return DOGPUP_OK;
 }
 
+// FSAI factors in the reference node order (pattern = the caller's tril(A))
+// into g64, FP64, one launch per mesh: mesh i's values for frequency f at
+// m_gbase[i] + f*nnzG_i + (p - pG0_i).
+static int fsai_reference_order(dogpup_solver* S, hipStream_t st) {
+  DP_TRY(S->g64.ensure((size_t)S->gvals * 16));
+  for (int i = 0; i < S->nmesh; ++i) {
+    FsaiOut fo{S->g64.as<double>() + 2 * (S->m_gbase[i] - S->m_pG0[i]), nullptr, S->m_nnzG[i], FSAI_OUT_SOLVER};
+    DP_TRY(fsai_launch(S->a64re.as<double>(), S->a64im.as<double>(), 0, S->rA0, S->cA0, S->rG0 + S->m_row0[i],
+                       S->cG0, S->m_row0[i + 1] - S->m_row0[i], S->m_nf[i], S->fscale.as<double>() + S->m_fs0[i],
+                       fo, st));
+  }
+  S->g64_valid = true;
+  return DOGPUP_OK;
+}
+
 int dogpup_solver_set_matrix(dogpup_solver_t S, const double* d_valA_re, const double* d_valA_im,
                              const double* d_fscale, void* stream) {
   DP_ARG(S && d_valA_re && d_valA_im && d_fscale, "NULL argument");
@@ -4020,17 +4067,15 @@ int dogpup_solver_set_matrix(dogpup_solver_t S, const double* d_valA_re, const d
   DP_HIP(hipMemcpy(S->t_gfsc.p, gf.data(), gf.size() * 8, hipMemcpyHostToDevice));
   S->h_fscale = hf;
   if (S->freq_cont) build_freq_cont(S);
-  // FSAI factors in the reference node order (pattern = the caller's tril(A)),
-  // one launch per mesh: mesh i's values for frequency f at
-  // m_gbase[i] + f*nnzG_i + (p - pG0_i)
+  // FSAI factors: straight into the solver layouts through the gather table
+  // (DOGPUP_FSAI_GATHER=1, rows longer than 8 or the pivoted kernel: in the
+  // reference order, then gathered)
   const size_t nG = (size_t)S->gvals;
-  DP_TRY(S->g64.ensure(nG * 16));
-  for (int i = 0; i < S->nmesh; ++i) {
-    FsaiOut fo{S->g64.as<double>() + 2 * (S->m_gbase[i] - S->m_pG0[i]), nullptr, S->m_nnzG[i], FSAI_OUT_SOLVER};
-    DP_TRY(fsai_launch(S->a64re.as<double>(), S->a64im.as<double>(), 0, S->rA0, S->cA0, S->rG0 + S->m_row0[i],
-                       S->cG0, S->m_row0[i + 1] - S->m_row0[i], S->m_nf[i], S->fscale.as<double>() + S->m_fs0[i],
-                       fo, st));
-  }
+  const char* eg = getenv("DOGPUP_FSAI_GATHER");
+  const char* ep = getenv("DOGPUP_FSAI_PIVOT");
+  const bool direct = S->fsai_direct && !(eg && atoi(eg)) && !(ep && atoi(ep));
+  S->g64_valid = false;
+  if (!direct) DP_TRY(fsai_reference_order(S, st));
   // values in solver node order
   DP_TRY(S->a64pre.ensure(nA * 8));
   DP_TRY(S->a64pim.ensure(nA * 8));
@@ -4040,9 +4085,10 @@ int dogpup_solver_set_matrix(dogpup_solver_t S, const double* d_valA_re, const d
   auto gather_g = [&](auto* tag, Buf& dst, const Buf& map) -> int {
     using TO = std::remove_pointer_t<decltype(tag)>;
     DP_TRY(dst.ensure(nG * 2 * sizeof(TO)));
+    if (direct) return DOGPUP_OK;
     for (int i = 0; i < S->nmesh; ++i) {
       const long long n = (long long)S->m_nnzG[i] * S->m_nf[i];
-      k_gather_cplx<TO><<<grid1(n), 256, 0, st>>>(S->g64.as<double>(), map.as<int>(), dst.as<TO>(), S->m_pG0[i],
+      k_gather_cplx<double, TO><<<grid1(n), 256, 0, st>>>(S->g64.as<double>(), map.as<int>(), dst.as<TO>(), S->m_pG0[i],
                                                   S->m_nnzG[i], S->m_nf[i], S->m_gbase[i]);
       DP_LAUNCH_CHECK();
     }
@@ -4059,6 +4105,27 @@ int dogpup_solver_set_matrix(dogpup_solver_t S, const double* d_valA_re, const d
   } else {
     DP_TRY(gather_g((double*)nullptr, S->gp64, S->gmap));
     DP_TRY(gather_g((double*)nullptr, S->gtp64, S->gtmap));
+  }
+  if (direct) {
+    // G in solver order from the FSAI kernel, then G^T gathered from it
+    auto direct_g = [&](auto* tag, Buf& g, Buf& gt) -> int {
+      using TO = std::remove_pointer_t<decltype(tag)>;
+      for (int i = 0; i < S->nmesh; ++i) {
+        FsaiDirect fo{g.as<TO>() + 2 * S->m_gbase[i], S->ginv.as<int>(), S->m_nnzG[i], S->m_pG0[i],
+                      std::is_same<TO, float>::value ? 1 : 0};
+        DP_TRY(fsai_pos_launch(S->a64re.as<double>(), S->a64im.as<double>(),
+                               S->fsai_pos.as<int>() + 36LL * S->m_row0[i], S->rG0 + S->m_row0[i],
+                               S->m_row0[i + 1] - S->m_row0[i], S->m_nf[i], S->fscale.as<double>() + S->m_fs0[i],
+                               fo, st));
+        const long long n = (long long)S->m_nnzG[i] * S->m_nf[i];
+        k_gather_cplx<TO, TO><<<grid1(n), 256, 0, st>>>(g.as<TO>(), S->gtg.as<int>(), gt.as<TO>(), S->m_pG0[i],
+                                                        S->m_nnzG[i], S->m_nf[i], S->m_gbase[i]);
+        DP_LAUNCH_CHECK();
+      }
+      return DOGPUP_OK;
+    };
+    if (S->mode == DOGPUP_MODE_C64_REFINE) DP_TRY(direct_g((float*)nullptr, S->g32, S->gt32));
+    else DP_TRY(direct_g((double*)nullptr, S->gp64, S->gtp64));
   }
   S->have_matrix = true;
   return DOGPUP_OK;
@@ -4249,7 +4316,9 @@ int dogpup_solver_stats(dogpup_solver_t S, double* h_out, int n) {
 int dogpup_solver_get_fsai(dogpup_solver_t S, double* d_out, void* stream) {
   DP_ARG(S && d_out, "NULL argument");
   DP_ARG(S->have_matrix, "no matrix set");
-  DP_HIP(hipMemcpyAsync(d_out, S->g64.p, (size_t)S->gvals * 16, hipMemcpyDeviceToDevice, as_stream(stream)));
+  hipStream_t st = as_stream(stream);
+  if (!S->g64_valid) DP_TRY(fsai_reference_order(S, st));
+  DP_HIP(hipMemcpyAsync(d_out, S->g64.p, (size_t)S->gvals * 16, hipMemcpyDeviceToDevice, st));
   return DOGPUP_OK;
 }
 

@@ -190,6 +190,35 @@ def test_solver_fsai_factors(setup):
     assert rel(got, valG) < 1e-12
 
 
+@pytest.mark.parametrize("mode", ["mixed", "fp64", "plain"])
+def test_solver_fsai_direct_matches_gather(setup, mode, monkeypatch):
+    """The solver's FSAI setup through the A-position table, written straight
+    into the solver's G and G^T layouts, against the reference-order factor
+    gathered into them (DOGPUP_FSAI_GATHER=1): the same factors, so the
+    solves agree bit for bit; the reference-order factors the handle returns
+    on request are the oracle's."""
+    from oracle import fsai as ofsai
+    D, A = setup["D"], setup["A"]
+    nf = setup["nf"]
+    S = D.solver(4, mode)
+    rng = np.random.default_rng(7)
+    b = torch.as_tensor(rng.standard_normal(S.empty_block().shape) + 1j * rng.standard_normal(S.empty_block().shape)).to("cuda")
+    print(f"[case] fsai direct vs gather {mode}", flush=True)
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("DOGPUP_FSAI_GATHER", flag)
+        S._version = None
+        S.ensure_matrix()
+        out[flag] = S.solve(b, tol=1e-10)
+        assert (out[flag].status == 0).all()
+    bits = [torch.view_as_real(out[f].data[:, :, :4].contiguous()).view(torch.int64) for f in ("0", "1")]
+    assert torch.equal(bits[0], bits[1])
+    assert (out["0"].iters == out["1"].iters).all()
+    fscale = np.arange(nf) * setup["df"] / D.c
+    rG, cG, valG = ofsai.fsaip_gen_batched(sp.csr_matrix(A.real) + 1j * sp.csr_matrix(A.imag / (setup["df"] / D.c)), nf, fscale=fscale)
+    assert rel(S.fsai_values().cpu().numpy().T, valG) < 1e-12
+
+
 def test_meas_and_transforms(setup):
     from dogpup_amd import transforms as tr
     from oracle import optode as oopt, solver as osol, transforms as otr
