@@ -21,9 +21,10 @@
 // Wd[t][(k,0)] = 2 w_k cos(f_k t), Wd[t][(k,1)] = -2 w_k sin(f_k t) and
 // X[(k,0)][e] = Re F_k(e), X[(k,1)][e] = Im F_k(e): one GEMM over all columns
 // on v_mfma_f64_16x16x4f64, FP64 products and accumulation, FP32 output.  A
-// persistent 1024-thread block per CU holds Wd (128 time bins per pass up to
+// persistent 512-thread block per CU holds Wd (128 time bins per pass up to
 // 64 frequencies, 64 beyond) in LDS; a wavefront keeps all of a pass's bins
-// of 16 columns in accumulator registers.
+// of 16 columns in accumulator registers, so every field value it loads
+// feeds 8 (4) MFMAs.
 //
 // MFMA form (k_jtg_mfma, up to 32 distinct source and 32 distinct detector
 // columns): per node and gate the link block is a real GEMM
@@ -260,27 +261,32 @@ __global__ void k_build_wd(double* __restrict__ Wd, const double* __restrict__ f
   Wd[i] = v;
 }
 
-// y[t0 + t][e] = sum_kk Wd[t][kk] X[kk][e] for t < min(th * NTT * 16, nt - t0).
+// y[t0 + t][e] = sum_kk Wd[t][kk] X[kk][e] for t < min(NTT * 16, nt - t0).
 // F: complex128 [nf][E]; out: float [nt][E].  A wavefront computes NTT time
-// tiles (16 bins each) of a 16-column tile on v_mfma_f64_16x16x4f64 (FP64
-// products and sums; only the stored value is rounded to FP32); th (1 or 2)
-// wavefronts share a column tile, one per block of NTT*16 bins, with Wd for
-// the pass's th*NTT*16 bins in LDS.  FP64 because a time bin of a curve far
-// from its source is a near-total cancellation of the 2nf terms: in FP32
-// such curves came out wrong by up to their own peak.
+// tiles (16 bins each, the whole pass) of a 16-column tile on
+// v_mfma_f64_16x16x4f64 (FP64 products and sums; only the stored value is
+// rounded to FP32), with Wd for the pass's NTT*16 bins in LDS.  FP64 because
+// a time bin of a curve far from its source is a near-total cancellation of
+// the 2nf terms: in FP32 such curves came out wrong by up to their own peak.
+// NTT = 8 on 512-thread blocks (2 waves per SIMD, 184 VGPRs) measured 9.8 ms
+// on 269,001 x 32 columns x 64 frequencies to 256 bins, against 11.0 ms for
+// NTT = 4 with two wavefronts per column tile on 1024-thread blocks: each
+// field value loaded feeds twice the MFMAs.
+constexpr int kFc2tdThreads = 512;
 template <int NTT>
-__global__ __launch_bounds__(1024) void k_fc2td_mfma64(const double* __restrict__ F, const double* __restrict__ Wd,
-                                                       float* __restrict__ out, long long E, int nf, int kks,
-                                                       int t0, int nt, int th) {
-  extern __shared__ double sWd[];   // th*NTT*16 rows x kks
-  for (int i = threadIdx.x; i < th * NTT * 16 * kks; i += 1024) sWd[i] = Wd[i];
+__global__ __launch_bounds__(kFc2tdThreads) void k_fc2td_mfma64(const double* __restrict__ F,
+                                                                const double* __restrict__ Wd,
+                                                                float* __restrict__ out, long long E, int nf,
+                                                                int kks, int t0, int nt) {
+  extern __shared__ double sWd[];   // NTT*16 rows x kks
+  for (int i = threadIdx.x; i < NTT * 16 * kks; i += kFc2tdThreads) sWd[i] = Wd[i];
   __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int half = wv % th, slot = wv / th, slots = 16 / th;
+  const int lane = threadIdx.x & 63, slot = threadIdx.x >> 6;
+  constexpr int slots = kFc2tdThreads / 64;
   const int i16 = lane & 15, kk = lane >> 4, kf = kk >> 1, im = kk & 1;
   const int nfp8 = (nf + 7) & ~7;   // Wd holds zero columns up to 2*nfp8
   const long long ntiles = (E + 15) / 16;
-  const double* wbase = sWd + (half * NTT * 16 + i16) * kks + kk;
+  const double* wbase = sWd + i16 * kks + kk;
   typedef double d4 __attribute__((ext_vector_type(4)));
   for (long long mt = (long long)blockIdx.x * slots + slot; mt < ntiles; mt += (long long)gridDim.x * slots) {
     const long long e = mt * 16 + i16;
@@ -314,29 +320,28 @@ __global__ __launch_bounds__(1024) void k_fc2td_mfma64(const double* __restrict_
       for (int i = 0; i < NTT; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int tt = t0 + (half * NTT + i) * 16 + kk + 4 * r;
+          const int tt = t0 + i * 16 + kk + 4 * r;
           if (tt < nt) out[(long long)tt * E + e] = (float)acc[i][r];
         }
     }
   }
 }
 
-// Shape of the fc2td passes for nf frequencies: th wavefronts of 4 time
-// tiles (64 bins) per column tile, so th*64 bins per pass (2 while the
-// pass's Wd fits the LDS, else 1), and the LDS row stride kks (doubles; 2
-// mod 32, so the 16 rows a wavefront reads start in different banks).
-constexpr int kFc2tdNtt = 4;
+// Shape of the fc2td passes for nf frequencies: ntt time tiles per pass (8
+// while the pass's Wd fits the LDS, else 4) and the LDS row stride kks
+// (doubles; 2 mod 32, so the 16 rows a wavefront reads start in different
+// banks).
 struct Fc2tdShape {
-  int th, kks;
-  int bins() const { return th * kFc2tdNtt * 16; }
+  int ntt, kks;
+  int bins() const { return ntt * 16; }
   size_t shm() const { return (size_t)bins() * kks * sizeof(double); }
 };
 static Fc2tdShape fc2td_shape(int nf) {
   const int nfp8 = (nf + 7) & ~7;
   Fc2tdShape s;
   s.kks = ((2 * nfp8 + 31) / 32) * 32 + 2;
-  s.th = 2;
-  if (s.shm() > 160 * 1024) s.th = 1;
+  s.ntt = 8;
+  if (s.shm() > 160 * 1024) s.ntt = 4;
   return s;
 }
 
@@ -346,15 +351,18 @@ static hipError_t fc2td_field_passes(const double* d_field, float* d_out, const 
                                      int nf, int nt, long long ncol, double* d_wd, int ncu, hipStream_t st) {
   const Fc2tdShape sh = fc2td_shape(nf);
   const int ntp = sh.bins();
-  const long long ntiles = (ncol + 15) / 16, slots = 16 / sh.th;
+  const long long ntiles = (ncol + 15) / 16, slots = kFc2tdThreads / 64;
   const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(ncu, (ntiles + slots - 1) / slots));
-  hipError_t e = hipFuncSetAttribute((const void*)k_fc2td_mfma64<kFc2tdNtt>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.shm());
+  const void* fn = sh.ntt == 8 ? (const void*)k_fc2td_mfma64<8> : (const void*)k_fc2td_mfma64<4>;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.shm());
   for (int t0 = 0; t0 < nt && e == hipSuccess; t0 += ntp) {
     k_build_wd<<<div_up((long long)ntp * sh.kks, 256), 256, 0, st>>>(d_wd, d_f, d_t, t0, nt, nf, ntp, sh.kks);
     e = hipGetLastError();
     if (e != hipSuccess) break;
-    k_fc2td_mfma64<kFc2tdNtt><<<grid, 1024, sh.shm(), st>>>(d_field, d_wd, d_out, ncol, nf, sh.kks, t0, nt, sh.th);
+    if (sh.ntt == 8)
+      k_fc2td_mfma64<8><<<grid, kFc2tdThreads, sh.shm(), st>>>(d_field, d_wd, d_out, ncol, nf, sh.kks, t0, nt);
+    else
+      k_fc2td_mfma64<4><<<grid, kFc2tdThreads, sh.shm(), st>>>(d_field, d_wd, d_out, ncol, nf, sh.kks, t0, nt);
     e = hipGetLastError();
   }
   return e;
