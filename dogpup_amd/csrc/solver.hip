@@ -421,10 +421,34 @@ __device__ inline void cmac2(const f4& v, float mr, float mi, float* ar, float* 
   ai[1] = fmaf(mi, v[2], ai[1]);
 }
 
+// N staged entries from p, unmasked (the tail of a whole-step row).
+template <int N, int LPX>
+__device__ inline void halo_tail(const unsigned char* __restrict__ scol, const float* __restrict__ sval,
+                                 const f4* __restrict__ xs, int cl, int p, float* ar, float* ai) {
+  int li[N];
+  float mr[N], mi[N];
+  f4 v[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    int iq = p + q;
+    asm volatile("" : "+v"(iq));
+    li[q] = scol[iq];
+  }
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    v[q] = xs[li[q] * LPX + cl];
+    mr[q] = sval[2 * (p + q)];
+    mi[q] = sval[2 * (p + q) + 1];
+  }
+#pragma unroll
+  for (int q = 0; q < N; ++q) cmac2(v[q], mr[q], mi[q], ar, ai);
+}
+
 // Row product of the x-halo kernels: ar + i ai = sum over the staged entries
 // [s0, s1) of value * x, x read from the LDS halo at slot * LPX + cl.  Four
-// entries per step.  Rows of 15 entries (A) take whole steps and a masked
-// tail step: the whole steps read their slots at constant offsets from p;
+// entries per step.  Rows of 15 entries (A) take whole steps and an
+// unmasked tail of 1-3 entries: the whole steps read their slots at
+// constant offsets from p;
 // the empty asm keeps each slot read a byte read (merged into one dword at a
 // row start that is not 4-aligned, the read measured 13 % slower on the
 // A pass).  Shorter rows of varying length (the FSAI factors, at most 8
@@ -453,6 +477,12 @@ __device__ inline void halo_row(const unsigned char* __restrict__ scol, const fl
 #pragma unroll
       for (int q = 0; q < 4; ++q) cmac2(v[q], mr[q], mi[q], ar, ai);
     }
+    // the last 1-3 entries without masks (a 15-entry row ends on 3)
+    const int rem = s1 - p;
+    if (rem == 3) halo_tail<3, LPX>(scol, sval, xs, cl, p, ar, ai);
+    else if (rem == 2) halo_tail<2, LPX>(scol, sval, xs, cl, p, ar, ai);
+    else if (rem == 1) halo_tail<1, LPX>(scol, sval, xs, cl, p, ar, ai);
+    return;
   }
   for (; p < s1; p += 4) {
     int li[4];
