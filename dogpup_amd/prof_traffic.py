@@ -5,7 +5,7 @@ runs of the same ``bench.py`` command, each writing ``*_counter_collection.csv``
 and the bench's JSON line):
 
     python -m dogpup_amd.prof_traffic FETCH.csv WRITE.csv --bench-json BENCH.json \
-        --solves 2 --json profiles/traffic_cocg_c64_slab269k_2048rhs.json
+        --from-last k_fsai_ldl8 --json profiles/traffic_cocg_c64_slab269k_2048rhs.json
 
 FETCH_SIZE and WRITE_SIZE are in KiB per dispatch; each is scaled by what it
 reads for the bench's plain 2 GiB device copy (``__amd_rocclr_copyBuffer``),
@@ -13,9 +13,13 @@ so that bytes = counter x factor.  The traffic of every dispatch of a class's
 kernels (0 G/G^T SpMV, 1 A SpMV, 2 Krylov vector passes, 3 FP64 refinement)
 is summed over the run and divided by the algorithmic bytes the bench's
 profiled step counted for that class (``extra.kernel_classes``) times the
-solves the run made (the timed step and the profiled step: 2; their work is
-identical).  That per-class ratio is what bench.py applies to its measured
-launches.  The largest dispatch of every kernel is listed too.
+solves the run made with that work (``--solves``).  With ``--from-last
+REGEX`` only the dispatches from the last one whose kernel name matches REGEX
+on are counted (``--from-last k_fsai_ldl8``: the bench's profiled step, which
+is the run's last FSAI setup and solve, without the bench's own residual
+checks of earlier steps), against one step's algorithmic bytes.  That
+per-class ratio is what bench.py applies to its measured launches.  The
+largest dispatch of every kernel is listed too.
 """
 from __future__ import annotations
 
@@ -42,12 +46,18 @@ def _short(name: str) -> str:
     return name
 
 
-def read_counter(path: str, counter: str):
+def read_counter(path: str, counter: str, from_last: str | None = None):
     rows = []
     with open(path, newline="") as fh:
         for r in csv.DictReader(fh):
             if r["Counter_Name"] == counter:
                 rows.append((int(r["Dispatch_Id"]), _short(r["Kernel_Name"]), float(r["Counter_Value"])))
+    rows.sort()
+    if from_last:
+        marks = [d for d, n, _ in rows if re.search(from_last, n)]
+        if not marks:
+            raise SystemExit(f"no dispatch of a kernel matching {from_last!r} in {path}")
+        rows = [r for r in rows if r[0] >= marks[-1]]
     return rows
 
 
@@ -74,14 +84,21 @@ def main(argv=None):
     ap.add_argument("fetch_csv")
     ap.add_argument("write_csv")
     ap.add_argument("--bench-json", required=True, help="the bench line of one of the two PMC runs")
-    ap.add_argument("--solves", type=int, default=2, help="solves of the profiled command with the same work")
+    ap.add_argument("--solves", type=int, default=None,
+                    help="solves of the profiled command with the same work (default 2, or 1 with --from-last)")
+    ap.add_argument("--from-last", default=None,
+                    help="count only the dispatches from the last one of a kernel matching this regex on")
     ap.add_argument("--copy-bytes", type=float, default=float(2 << 30))
     ap.add_argument("--json", required=True)
     a = ap.parse_args(argv)
-    fetch = read_counter(a.fetch_csv, "FETCH_SIZE")
-    write = read_counter(a.write_csv, "WRITE_SIZE")
-    cf = [v for _, n, v in fetch if n == COPY and v > 0.4 * a.copy_bytes / 1024][:6]
-    cw = [v for _, n, v in write if n == COPY and v > 0.4 * a.copy_bytes / 1024][:6]
+    if a.solves is None:
+        a.solves = 1 if a.from_last else 2
+    # the 2 GiB copy (calibration) comes early in the run, so it is looked
+    # for among all dispatches
+    cf = [v for _, n, v in read_counter(a.fetch_csv, "FETCH_SIZE") if n == COPY and v > 0.4 * a.copy_bytes / 1024][:6]
+    cw = [v for _, n, v in read_counter(a.write_csv, "WRITE_SIZE") if n == COPY and v > 0.4 * a.copy_bytes / 1024][:6]
+    fetch = read_counter(a.fetch_csv, "FETCH_SIZE", a.from_last)
+    write = read_counter(a.write_csv, "WRITE_SIZE", a.from_last)
     if not cf or not cw:
         raise SystemExit("no 2 GiB copy dispatch found for the calibration")
     f_fac = a.copy_bytes / (sum(cf) / len(cf))       # bytes per FETCH_SIZE unit
@@ -109,7 +126,7 @@ def main(argv=None):
         c["algorithmic_bytes"] = alg
         c["ratio"] = c["traffic_bytes"] / alg
     out = {"copy_calibration": {"bytes": a.copy_bytes, "fetch_factor": f_fac / 1024, "write_factor": w_fac / 1024},
-           "solves": a.solves, "kernels": kernels, "classes": classes}
+           "solves": a.solves, "from_last": a.from_last, "kernels": kernels, "classes": classes}
     with open(a.json, "w") as fh:
         json.dump(out, fh, indent=1)
     print("| class | traffic GB | algorithmic GB | traffic / algorithmic |")
