@@ -276,8 +276,12 @@ int dogpup_solver_stats(dogpup_solver_t s, double* h_out, int n);
 int dogpup_solver_set_profile(dogpup_solver_t s, int on);
 int dogpup_solver_profile(dogpup_solver_t s, double* h_out, int n);
 
-/* Copy the FSAI factors held by the handle (FP64, [freq][nnzG] interleaved
- * complex, 2*nfreq*nnzG doubles) into d_out. */
+/* The handle's FSAI factors in the caller's pattern order (FP64, [freq][nnzG]
+ * interleaved complex, 2*nfreq*nnzG doubles) into d_out.  When
+ * dogpup_solver_set_matrix wrote the factors straight into the solver's
+ * layout (every pattern row of at most 8 entries), the first call after it
+ * computes this FP64 copy (same kernel arithmetic), then keeps it until the
+ * next dogpup_solver_set_matrix. */
 int dogpup_solver_get_fsai(dogpup_solver_t s, double* d_out, void* stream);
 
 /* Layout transforms between MATLAB layout (NN x NF x NS column-major,
