@@ -192,9 +192,11 @@ def cpu_baseline(D, k=32, tol=1e-12):
 
 
 def batch_leg(tol):
-    """The 48-mesh uneven batch: host prep, autotuned shapes, one timed
-    mixed solve (FSAI setup + solve) of all 38,431 RHS, the same with default
-    shapes, and the mixed-vs-FP64 error of every instance."""
+    """The 48-mesh uneven batch: host prep, autotuned shapes (group width,
+    tile, continuation spacing, and a split into parts of their own group
+    width), one timed mixed solve (FSAI setup + solve) of all 38,431 RHS, the
+    same with the library's default shapes in one handle, and the
+    mixed-vs-FP64 error of every instance."""
     import gc
     import torch
     from dogpup_amd import autotune, batch, meshgen, mesh as dm
@@ -215,20 +217,22 @@ def batch_leg(tol):
     log(f"batch: {len(meshes)} meshes, {out['nodes']} nodes, {nrhs} RHS; autotuning")
     t1 = time.time()
     best, table = autotune.tune(meshes, "s", "mixed", "cocg", use_cache=False, widths=(8, 16), tiles=(512, 1024),
-                                unrolls=(8,), iters=4, strides=(8, 16, 32), tol=tol)
+                                unrolls=(8,), iters=4, strides=(8, 16, 32), splits=((32, 24), (32, 16)), tol=tol)
     out["tune_s"] = time.time() - t1
     out["tuned"] = best
     out["tune_table_ms_per_iter"] = {f"nsp{r['nsp']}_tile{r['tile_rows']}_u{r['unroll']}": r["s_per_iter"] * 1e3
                                      for r in table if "s_per_iter" in r}
     out["tune_table_ms_per_solve"] = {f"cont_stride{r['cont_stride']}": r["s_per_solve"] * 1e3
-                                      for r in table if "s_per_solve" in r}
+                                      for r in table if "cont_stride" in r}
+    out["tune_table_ms_per_solve"].update({f"split_{r['split'][0]}wide_above{r['split'][1]}": r["s_per_solve"] * 1e3
+                                           for r in table if "split" in r})
 
     def timed(B):
         b = B.rhs()
-        B._versions = None
+        B.reset_matrix()
         B.solve(b, tol=tol)                      # warm-up (graph capture)
         torch.cuda.synchronize()
-        B._versions = None                       # FSAI setup is part of the timed solve
+        B.reset_matrix()                         # FSAI setup is part of the timed solve
         t2 = time.perf_counter()
         f = B.solve(b, tol=tol)
         torch.cuda.synchronize()
@@ -243,8 +247,14 @@ def batch_leg(tol):
     gc.collect()
     torch.cuda.empty_cache()
     log(f"batch: tuned shapes {best}")
-    B = batch.MeshBatch(meshes, "s", "mixed", nsp=best["nsp"], method="cocg",
-                        tuning=(best["tile_rows"], best["unroll"], best["unroll"], 0))
+    tuning = (best["tile_rows"], best["unroll"], best["unroll"], 0)
+    if "split" in best:
+        parts = batch.SplitBatch.width_parts([len(sp["src"]) for sp in specs], wide=best["split"][0],
+                                             narrow=best["nsp"], threshold=best["split"][1])
+        B = batch.SplitBatch(meshes, "s", "mixed", parts=parts, method="cocg", tuning=tuning)
+        out["tuned_parts"] = [[len(i), w] for i, w in parts]
+    else:
+        B = batch.MeshBatch(meshes, "s", "mixed", nsp=best["nsp"], method="cocg", tuning=tuning)
     B.set_cont_stride(best.get("cont_stride", 0))
     dt, fields, b = timed(B)
     out["mixed_s"] = dt
@@ -253,17 +263,22 @@ def batch_leg(tol):
     out["tuning_speedup"] = dt_def / dt
     out["max_iters"] = int(max(f.iters.max() for f in fields))
     out["status"] = sorted({int(v) for f in fields for v in np.unique(f.status)})
-    xm = [f.data.clone() for f in fields]
-    del B, fields
+    xm = [f.to_reference_layout().clone() for f in fields]     # the part widths may differ from the FP64 run's
+    del B, fields, b
     gc.collect()
     torch.cuda.empty_cache()
     log("batch: FP64 reference solve")
     B64 = batch.MeshBatch(meshes, "s", "fp64", nsp=best["nsp"], method="cocg")
+    b = B64.rhs()
     t3 = time.perf_counter()
     f64 = B64.solve(b, tol=1e-12)
     torch.cuda.synchronize()
     out["fp64_s"] = time.perf_counter() - t3
-    errs = [float(torch.linalg.vector_norm(a - f.data) / torch.linalg.vector_norm(f.data)) for a, f in zip(xm, f64)]
+    errs = []
+    for a, f in zip(xm, f64):                  # one mesh's reference-layout copy at a time
+        r = f.to_reference_layout()
+        errs.append(float(torch.linalg.vector_norm(a - r) / torch.linalg.vector_norm(r)))
+        del r
     out["mixed_vs_fp64_rel_err_per_instance"] = errs
     out["mixed_vs_fp64_rel_err_max"] = max(errs)
     del B64, f64, xm, b

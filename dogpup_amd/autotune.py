@@ -13,7 +13,11 @@ times whole solves of the chosen shape at ``tol`` with Galerkin frequency
 continuation for each level-0 spacing of the schedule (every stride-th
 frequency solved from zero): a wider spacing solves fewer frequencies from
 zero but runs more levels, each a synchronised phase with fewer groups, and
-the best spacing depends on the meshes' frequency counts. Results are cached in a JSON file
+the best spacing depends on the meshes' frequency counts. With ``splits`` it
+also times whole solves of SplitBatch partitions (the meshes with more than a
+threshold of columns in wider groups, each part its own handle, solved one
+after the other) against the single handle, and keeps the fastest. Results
+are cached in a JSON file
 (``$DOGPUP_TUNE_CACHE``, default ``~/.cache/dogpup_amd/tune.json``) under a
 signature of the problem: node, frequency and column counts per mesh, mode and
 method.
@@ -28,7 +32,7 @@ import time
 import torch
 
 from . import _lib
-from .batch import MeshBatch
+from .batch import MeshBatch, SplitBatch
 
 TILES = (256, 512, 1024)
 UNROLLS = (4, 8)
@@ -75,15 +79,24 @@ def _time_solve(B, tol):
     return dt
 
 
+def _ncols(meshes, which):
+    return [sum(len((m.optode.s_bary if w == "s" else m.optode.d_bary)[0]) for w in which) for m in meshes]
+
+
 def tune(meshes, which="s", mode="mixed", method="cocg", iters=6, widths=WIDTHS, tiles=TILES, unrolls=UNROLLS,
-         strides=None, tol=5e-8, use_cache=True, verbose=False):
+         strides=None, splits=None, tol=5e-8, use_cache=True, verbose=False):
     """Returns ({"nsp", "tile_rows", "unroll", "s_per_iter"[, "cont_stride",
-    "s_per_solve"]}, table of every candidate: seconds per iteration, and
-    seconds per solve for the continuation spacings)."""
+    "s_per_solve"][, "split"]}, table of every candidate: seconds per
+    iteration, and seconds per solve for the continuation spacings and the
+    partitions).  ``splits``: (wide, threshold) pairs, SplitBatch.width_parts
+    with the tuned width as the narrow one; "split" is kept when one of them
+    solves faster than the single handle."""
     meshes = list(meshes)
     key = signature(meshes, which, mode, method)
     if strides and mode == "mixed":
         key += json.dumps({"strides": list(strides), "tol": tol})
+    if splits:
+        key += json.dumps({"splits": [list(x) for x in splits], "tol": tol})
     path = _cache_path()
     cache = {}
     if use_cache and os.path.exists(path):
@@ -146,6 +159,32 @@ def tune(meshes, which="s", mode="mixed", method="cocg", iters=6, widths=WIDTHS,
         gc.collect()
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
+    if splits:
+        tuning = (best["tile_rows"], best["unroll"], best["unroll"], 0)
+        if "s_per_solve" not in best:
+            B = MeshBatch(meshes, which, mode=mode, nsp=best["nsp"], method=method, tuning=tuning)
+            best["s_per_solve"] = _time_solve(B, tol)
+            table.append({"nsp": best["nsp"], "tile_rows": best["tile_rows"], "unroll": best["unroll"],
+                          "s_per_solve": best["s_per_solve"]})
+            del B
+        cols = _ncols(meshes, which)
+        for wide, thr in splits:
+            parts = SplitBatch.width_parts(cols, wide=wide, narrow=best["nsp"], threshold=thr)
+            if len(parts) < 2 and parts[0][1] == best["nsp"]:
+                continue                      # the single handle itself
+            S = SplitBatch(meshes, which, mode=mode, parts=parts, method=method, tuning=tuning)
+            if "cont_stride" in best:
+                S.set_cont_stride(best["cont_stride"])
+            s = _time_solve(S, tol)
+            table.append({"split": [wide, thr], "parts": [[len(i), w] for i, w in parts], "s_per_solve": s})
+            if verbose:
+                print(f"[tune] split {wide}-wide above {thr} columns: {s * 1e3:.1f} ms per solve", flush=True)
+            if s < best["s_per_solve"]:
+                best["split"], best["s_per_solve"] = [wide, thr], s
+            del S
+            gc.collect()
+            torch.cuda.synchronize()
+            torch.cuda.empty_cache()
     if use_cache:
         cache[key] = {"best": best, "table": table}
         try:
@@ -158,11 +197,18 @@ def tune(meshes, which="s", mode="mixed", method="cocg", iters=6, widths=WIDTHS,
 
 
 def tuned_batch(meshes, which="s", mode="mixed", method="cocg", **kw):
-    """A MeshBatch with the tuned group width, kernel shapes and (when tuned)
-    continuation spacing."""
+    """A MeshBatch (or, when a partition was tuned faster, a SplitBatch) with
+    the tuned group width, kernel shapes and (when tuned) continuation
+    spacing."""
+    meshes = list(meshes)
     best, _ = tune(meshes, which, mode, method, **kw)
-    B = MeshBatch(meshes, which, mode=mode, nsp=best["nsp"], method=method,
-                  tuning=(best["tile_rows"], best["unroll"], best["unroll"], 0))
+    tuning = (best["tile_rows"], best["unroll"], best["unroll"], 0)
+    if "split" in best:
+        wide, thr = best["split"]
+        parts = SplitBatch.width_parts(_ncols(meshes, which), wide=wide, narrow=best["nsp"], threshold=thr)
+        B = SplitBatch(meshes, which, mode=mode, parts=parts, method=method, tuning=tuning)
+    else:
+        B = MeshBatch(meshes, which, mode=mode, nsp=best["nsp"], method=method, tuning=tuning)
     if "cont_stride" in best:
         B.set_cont_stride(best["cont_stride"])
     return B

@@ -104,3 +104,52 @@ def test_autotune_picks_a_measured_shape(tmp_path, monkeypatch):
                               strides=(4, 8, 16), tol=1e-8)                  # served from the cache
     for f in B2.solve(tol=1e-10):
         assert (f.status == 0).all()
+
+
+def test_split_batch_equals_single_mesh_handles():
+    """A SplitBatch (parts of their own group width, one handle each) returns
+    the fields in the caller's mesh order, each equal bit for bit to a
+    handle of its mesh alone at its part's width; width_parts puts the
+    meshes above the threshold in the wide part; parts must cover every mesh
+    once."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dogpup_amd import batch
+    meshes = [D for D, _ in _batch_meshes()]                      # 3, 5 and 12 sources
+    parts = batch.SplitBatch.width_parts([3, 5, 12], wide=16, narrow=8, threshold=8)
+    assert parts == [([2], 16), ([0, 1], 8)]
+    print("[case] split batch", flush=True)
+    S = batch.SplitBatch(meshes, "s", "mixed", parts=parts, method="cocg")
+    fields = S.solve(tol=1e-10)
+    for i, (D, f) in enumerate(zip(meshes, fields)):
+        nsp = 16 if i == 2 else 8
+        assert f.nn == D.nn and f.nsp == nsp
+        assert (f.status == 0).all(), f.status
+        single = batch.MeshBatch([D], "s", "mixed", nsp=nsp, method="cocg").solve(tol=1e-10)[0]
+        assert torch.equal(single.data, f.data)
+        assert (single.iters == f.iters).all()
+    with pytest.raises(ValueError):
+        batch.SplitBatch(meshes, "s", "mixed", parts=[([0, 1], 8)])
+
+
+def test_autotune_keeps_the_faster_of_single_handle_and_split(tmp_path, monkeypatch):
+    """The tuner's partition search: whole solves of the single handle and of
+    a SplitBatch partition, the faster kept ("split" only when the partition
+    wins) and built by tuned_batch, whose solve converges."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dogpup_amd import autotune, batch
+    monkeypatch.setenv("DOGPUP_TUNE_CACHE", str(tmp_path / "tune.json"))
+    meshes = [D for D, _ in _batch_meshes()]
+    kw = dict(iters=3, widths=(8,), tiles=(512,), unrolls=(8,), splits=((16, 8),), tol=1e-8)
+    best, table = autotune.tune(meshes, "s", "mixed", "cocg", **kw)
+    single = [r for r in table if "s_per_solve" in r and "split" not in r]
+    split = [r for r in table if "split" in r]
+    assert len(single) == 1 and len(split) == 1
+    assert split[0]["parts"] == [[1, 16], [2, 8]]
+    assert best["s_per_solve"] == min(single[0]["s_per_solve"], split[0]["s_per_solve"])
+    assert ("split" in best) == (split[0]["s_per_solve"] < single[0]["s_per_solve"])
+    B = autotune.tuned_batch(meshes, "s", "mixed", "cocg", **kw)          # served from the cache
+    assert isinstance(B, batch.SplitBatch if "split" in best else batch.MeshBatch)
+    for f in B.solve(tol=1e-10):
+        assert (f.status == 0).all()
