@@ -34,24 +34,31 @@ DEV = "cuda"
 
 def target_spots(node, pos, w, kind="gauss", inmesh=None):
     """Sensitivity target spots (dMesh.m:250-282) on the grid points ``node``
-    (mesh nodes on the nodal basis): NW x NV, each row normalised to sum 1
-    over all grid points and negated, then zero at points outside the mesh
-    (``inmesh``).  As in the reference, the Gaussian width is
-    s = 2*sqrt(2 ln 2) * w."""
-    node = np.asarray(node, float)
-    pos = np.atleast_2d(np.asarray(pos, float))
+    (mesh nodes on the nodal basis): NW x NV float64 on the GPU, each row
+    normalised to sum 1 over all grid points and negated, then zero at
+    points outside the mesh (``inmesh``).  As in the reference, the Gaussian
+    width is s = 2*sqrt(2 ln 2) * w and the square spot is the box
+    |x - pos| <= w/2.  (On the host this was 1.2 s of a 3.2 s reconstruction
+    at 243 spots x 269k nodes.)"""
+    node = torch.as_tensor(np.asarray(node, float), dtype=torch.float64, device=DEV)
+    pos = torch.as_tensor(np.atleast_2d(np.asarray(pos, float)), dtype=torch.float64, device=DEV)
     if kind == "gauss":
         s = 2.0 * math.sqrt(2.0 * math.log(2.0)) * w
-        d2 = ((node[None, :, :] - pos[:, None, :]) ** 2).sum(axis=2)
-        t = np.exp(-d2 / (2.0 * s * s))
+        d2 = torch.zeros((pos.shape[0], node.shape[0]), dtype=torch.float64, device=DEV)
+        for a in range(3):                        # per axis: no NW x NV x 3 intermediate
+            d2 += (node[None, :, a] - pos[:, a, None]) ** 2
+        t = torch.exp(-d2 / (2.0 * s * s))
     elif kind == "square":
         h = w / 2.0
-        t = np.all(np.abs(node[None, :, :] - pos[:, None, :]) <= h, axis=2).astype(float)
+        t = torch.ones((pos.shape[0], node.shape[0]), dtype=torch.bool, device=DEV)
+        for a in range(3):
+            t &= (node[None, :, a] - pos[:, a, None]).abs() <= h
+        t = t.to(torch.float64)
     else:
         raise ValueError("Check spot type is gauss or square")
-    t = -(t / t.sum(axis=1, keepdims=True))
+    t = -(t / t.sum(dim=1, keepdim=True))
     if inmesh is not None:
-        t[:, ~np.asarray(inmesh, bool)] = 0.0
+        t[:, ~torch.as_tensor(np.asarray(inmesh, bool), device=DEV)] = 0.0
     return t
 
 
@@ -105,10 +112,12 @@ def wghtd_lm_recon(mesh, data_m, gates, pos_s=None, lam=(1e2, 1e-6), mode="mixed
     spots = None
     if pos_s is not None and len(pos_s):
         if basis is not None:
-            t = target_spots(basis.grid, pos_s, spot_fwhm, "gauss", basis.gridinmesh)[:, basis.in_idx]
+            t = target_spots(basis.grid, pos_s, spot_fwhm, "gauss", basis.gridinmesh)
+            t = t[:, torch.as_tensor(np.asarray(basis.in_idx, np.int64), device=DEV)]
         else:
             t = target_spots(mesh.node, pos_s, spot_fwhm, "gauss")
-        spots = torch.as_tensor(t, dtype=torch.float32, device=DEV)              # :35-37
+        spots = t.to(torch.float32)                                                # :35-37
+        del t
 
     def to_grid(v):
         """m2g * v restricted to the unknowns (in-mesh voxels or nodes)."""

@@ -22,6 +22,29 @@ def _problem(mua_field):
     return D, gates
 
 
+@pytest.mark.parametrize("kind,w", [("gauss", 1.5), ("square", 4.0)])
+def test_target_spots_matches_oracle(kind, w):
+    """lm.target_spots (GPU, FP64) against the oracle's restatement of
+    dMesh.m:250-282 on a jittered grid with an in-mesh mask: every entry to
+    1e-12 of the largest; each row's in-mesh entries sum to minus their
+    share of the spot."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dogpup_amd import lm
+    from oracle import lm as olm
+    rng = np.random.default_rng(3)
+    g = np.stack(np.meshgrid(np.arange(0.0, 20.0, 1.0), np.arange(0.0, 16.0, 1.0), np.arange(0.0, 8.0, 1.0),
+                             indexing="ij"), -1).reshape(-1, 3) + rng.uniform(-0.2, 0.2, (2560, 3))
+    inmesh = rng.uniform(size=len(g)) < 0.8
+    pos = rng.uniform([2.0, 2.0, 1.0], [18.0, 14.0, 7.0], (7, 3))
+    print(f"[case] target_spots {kind}", flush=True)
+    got = lm.target_spots(g, pos, w, kind, inmesh).cpu().numpy()
+    ref = olm.target_spots(g, pos, w, kind, inmesh)
+    assert got.shape == ref.shape == (7, len(g))
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.all(got[:, ~inmesh] == 0.0)
+
+
 def test_lm_reconstruction_mixed_and_fp64():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
