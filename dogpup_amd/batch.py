@@ -13,7 +13,6 @@ those calls' results.
 from __future__ import annotations
 
 import ctypes as C
-from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -190,16 +189,16 @@ class MeshBatch:
 
 
 class SplitBatch:
-    """A batch solved as several MeshBatch handles of their own group width:
-    meshes with many columns in wide groups (each matrix entry read once for
-    more RHS), the rest in narrow ones (little padding).  The parts' solves
-    run at the same time, one host thread each (every handle launches on its
-    own stream and captures its graphs thread-locally), or one after the
-    other.  On the 48-mesh uneven batch (8-32 sources per mesh), 32-wide
+    """A batch solved as several MeshBatch handles of their own group width,
+    one after the other: meshes with many columns in wide groups (each
+    matrix entry read once for more RHS), the rest in narrow ones (little
+    padding).  On the 48-mesh uneven batch (8-32 sources per mesh), 32-wide
     groups for the 18 meshes with more than 24 sources and 8-wide ones for
-    the other 30 measured 616 ms per solve one after the other and 577 ms at
-    the same time, against 688 ms in one 8-wide handle; the fields are the
-    same bit for bit either way.
+    the other 30 measured 616 ms per solve against 688 ms in one 8-wide
+    handle.  (The parts on one host thread each took 577 ms, but a capture
+    of a part's graph then fails now and then: another thread's device-wide
+    synchronisation, such as a handle destroyed by the garbage collector,
+    can fall inside it.)
 
     ``parts``: list of (mesh indices, nsp); default width_parts.  The other
     arguments are MeshBatch's and apply to every part.  solve() returns one
@@ -259,21 +258,15 @@ class SplitBatch:
         """Point-source right-hand sides, one batch-layout block per part."""
         return [B.rhs() for B in self.subs]
 
-    def solve(self, b=None, tol=1e-8, maxit=1000, concurrent=True):
-        """The parts' solves (b: one block per part, as rhs() returns, or
-        None), at the same time on one host thread each or, with
-        concurrent=False, one after the other; one FluenceField per mesh in
-        the caller's order."""
+    def solve(self, b=None, tol=1e-8, maxit=1000):
+        """The parts' solves one after the other (b: one block per part, as
+        rhs() returns, or None); one FluenceField per mesh in the caller's
+        order."""
         if b is None:
             b = [None] * len(self.subs)
         if len(b) != len(self.subs):
             raise ValueError("one right-hand-side block per part")
-        if concurrent and len(self.subs) > 1:
-            with ThreadPoolExecutor(len(self.subs)) as ex:
-                futs = [ex.submit(B.solve, bb, tol, maxit) for B, bb in zip(self.subs, b)]
-                results = [fu.result() for fu in futs]
-        else:
-            results = [B.solve(bb, tol=tol, maxit=maxit) for B, bb in zip(self.subs, b)]
+        results = [B.solve(bb, tol=tol, maxit=maxit) for B, bb in zip(self.subs, b)]
         out = [None] * len(self.meshes)
         stats = {"max_iters": 0.0, "group_iters": 0.0, "sweeps": 0.0, "inner_iters": 0.0}
         for (idx, _), B, fields in zip(self.parts, self.subs, results):
