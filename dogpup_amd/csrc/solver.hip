@@ -4082,7 +4082,18 @@ int dogpup_solver_set_matrix(dogpup_solver_t S, const double* d_valA_re, const d
                              const double* d_fscale, void* stream) {
   DP_ARG(S && d_valA_re && d_valA_im && d_fscale, "NULL argument");
   hipStream_t st = as_stream(stream);
-  S->clear_graphs();
+  // The captured graphs read the values below from these buffers and take
+  // their schedule from fscale: they stay valid across a call that keeps
+  // both (new optical properties at the same frequencies, as every LM step
+  // and every benchmark step makes), so only a change of either clears them.
+  // A call that fails leaves the handle without a matrix.
+  auto graph_bufs = [S]() {
+    return std::vector<void*>{S->a64pre.p, S->a64pim.p, S->a32re.p, S->a32im.p, S->g32.p,
+                              S->gt32.p,   S->gp64.p,   S->gtp64.p, S->t_gfsc.p, S->fscale.p};
+  };
+  const std::vector<void*> bufs0 = graph_bufs();
+  const bool had_matrix = S->have_matrix;
+  S->have_matrix = false;
   const size_t nA = S->nnzA;
   DP_TRY(S->a64re.ensure(nA * 8));
   DP_TRY(S->a64im.ensure(nA * 8));
@@ -4095,8 +4106,12 @@ int dogpup_solver_set_matrix(dogpup_solver_t S, const double* d_valA_re, const d
   DP_HIP(hipStreamSynchronize(st));
   for (int g = 0; g < S->ngroups; ++g) gf[g] = hf[S->m_fs0[S->h_gmesh[g]] + S->h_gf[g]];
   DP_HIP(hipMemcpy(S->t_gfsc.p, gf.data(), gf.size() * 8, hipMemcpyHostToDevice));
-  S->h_fscale = hf;
-  if (S->freq_cont) build_freq_cont(S);
+  const bool same_fscale = had_matrix && hf == S->h_fscale;
+  if (!same_fscale) {
+    S->clear_graphs();
+    S->h_fscale = hf;
+    if (S->freq_cont) build_freq_cont(S);
+  }
   // FSAI factors: straight into the solver layouts through the gather table
   // (DOGPUP_FSAI_GATHER=1, rows longer than 8 or the pivoted kernel: in the
   // reference order, then gathered)
@@ -4157,6 +4172,7 @@ int dogpup_solver_set_matrix(dogpup_solver_t S, const double* d_valA_re, const d
     if (S->mode == DOGPUP_MODE_C64_REFINE) DP_TRY(direct_g((float*)nullptr, S->g32, S->gt32));
     else DP_TRY(direct_g((double*)nullptr, S->gp64, S->gtp64));
   }
+  if (graph_bufs() != bufs0) S->clear_graphs();
   S->have_matrix = true;
   return DOGPUP_OK;
 }

@@ -219,6 +219,53 @@ def test_solver_fsai_direct_matches_gather(setup, mode, monkeypatch):
     assert rel(S.fsai_values().cpu().numpy().T, valG) < 1e-12
 
 
+@pytest.mark.parametrize("mode", ["mixed", "fp64", "plain"])
+def test_graphs_kept_across_new_properties(setup, mode):
+    """A handle keeps its captured graphs when dogpup_solver_set_matrix brings
+    new optical properties at the same frequency scales: its next solve is
+    the solve of a fresh handle with those properties, bit for bit; a call
+    with other frequency scales clears them, and the solve is again a fresh
+    handle's."""
+    from dogpup_amd import _lib, meshgen, mesh as dm
+    m = setup["m"]
+    tpsf = meshgen.gaussian_tpsf(setup["t"], 250e-12)
+
+    def make(mua):
+        D = dm.DMesh(m.node, m.elem, m.nr, np.stack([mua, m.musp], 1))
+        D.add_optode(dm.DOptode(setup["src"], setup["det"], setup["link"], 1e3, tpsf, setup["t"], setup["nf"]))
+        return D
+    mua1 = m.mua * 1.3
+    print(f"[case] graphs kept, {mode}", flush=True)
+    D = make(m.mua.copy())
+    D.solve_point_sources("s", mode=mode, tol=1e-10)                     # captures the graphs
+    D.update_properties(np.stack([mua1, m.musp], 1))
+    again = D.solve_point_sources("s", mode=mode, tol=1e-10)              # same fscale: graphs kept
+    fresh = make(mua1).solve_point_sources("s", mode=mode, tol=1e-10)
+    assert (again.status == 0).all()
+    assert torch.equal(again.data[:, :, :4], fresh.data[:, :, :4])
+    assert (again.iters == fresh.iters).all()
+    # other frequency scales through the C ABI: graphs cleared
+    def with_scale(Dm, f):
+        S = Dm.solver(4, mode)
+        S.ensure_matrix()
+        if mode == "mixed":
+            S.set_continuation(True, Dm.optode.s_fpsf)          # as solve_point_sources sets it
+        _lib.call("dogpup_solver_set_matrix", S.h, _lib.ptr(Dm.K), _lib.ptr(Dm.M),
+                  _lib.ptr(torch.as_tensor(f, dtype=torch.float64, device="cuda")), _lib.stream_ptr())
+        nodes, w = Dm._bary_dev("s")
+        spec = torch.as_tensor(Dm.optode.s_fpsf, dtype=torch.complex128, device="cuda")
+        b = S.empty_block()
+        _lib.call("dogpup_point_sources", _lib.ptr(torch.view_as_real(b)), Dm.nn, S.nf, 4, S.nsp,
+                  _lib.ptr(nodes), _lib.ptr(w), _lib.ptr(torch.view_as_real(spec)), _lib.stream_ptr())
+        return S.solve(b, tol=1e-10)
+    f2 = np.arange(setup["nf"]) * setup["df"] / D.c * 1.25
+    a2 = with_scale(D, f2)
+    b2 = with_scale(make(mua1), f2)
+    assert (a2.status == 0).all()
+    assert torch.equal(a2.data[:, :, :4], b2.data[:, :, :4])
+    assert not torch.equal(a2.data[:, :, :4], again.data[:, :, :4])
+
+
 def test_meas_and_transforms(setup):
     from dogpup_amd import transforms as tr
     from oracle import optode as oopt, solver as osol, transforms as otr
