@@ -187,7 +187,11 @@ int dogpup_solver_destroy(dogpup_solver_t s);
 /* Set A (FP64 values on the handle's pattern; frequency k uses
  * valA_re + i*fscale[k]*valA_im) and compute the FSAI factors of every
  * frequency (FP64), G^T, and their complex64 copies when the mode needs them.
- * d_fscale: nfreq doubles (the reference uses k = 0..NF-1). */
+ * d_fscale: nfreq doubles (the reference uses k = 0..NF-1).  A call with the
+ * previous call's fscale (new optical properties at the same frequencies)
+ * keeps the handle's captured iteration graphs and continuation schedule;
+ * other scales rebuild them at the next solve.  After a failed call the
+ * handle has no matrix until a call succeeds. */
 int dogpup_solver_set_matrix(dogpup_solver_t s, const double* d_valA_re, const double* d_valA_im,
                              const double* d_fscale, void* stream);
 
