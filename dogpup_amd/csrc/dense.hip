@@ -10,6 +10,7 @@
 // precisions): lane l supplies A[l%16][l/16] and B[l/16][l%16] (K index
 // l/16); its accumulator register r holds D[4*(l/16)+r][l%16] for FP32 and
 // D[l/16+4*r][l%16] for FP64 (measured on gfx950 by an exact-integer test).
+#include <algorithm>
 #include <type_traits>
 
 #include "common.hpp"
@@ -29,11 +30,14 @@ constexpr int GK = 16;   // K step
 // v_mfma_f32_16x16x4f32 with FP32 accumulation.  Output FP64 (OUT_D) or FP32.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+// Split K (P != nullptr): block z of grid.z sums k in [z*ks, min((z+1)*ks, K))
+// and stores its unscaled partial tile, FP64, at P[(z*M + m)*N + n];
+// k_gemm_reduce then adds the slices in the order z = 0, 1, ...
 template <bool TRANS_B, bool OUT_D, bool F64>
 __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, const float* __restrict__ B,
                                                   void* __restrict__ Cv, int M, int N, int K, long long lda,
                                                   long long ldb, long long ldc, float alpha,
-                                                  int lower_only) {
+                                                  int lower_only, double* __restrict__ P, int ks) {
   using E = typename std::conditional<F64, double, float>::type;
   using V4 = typename std::conditional<F64, f64x4, f32x4>::type;
   __shared__ E As[GT][GK + 1];
@@ -47,14 +51,16 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, c
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = V4{0, 0, 0, 0};
-  for (int k0 = 0; k0 < K; k0 += GK) {
+  const int kb = P ? blockIdx.z * ks : 0;
+  const int ke = P ? min(K, kb + ks) : K;
+  for (int k0 = kb; k0 < ke; k0 += GK) {
     {
       const int r = t >> 2, kq = (t & 3) * 4;
       const int gm = m0 + r;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int gk = k0 + kq + q;
-        As[r][kq + q] = (gm < M && gk < K) ? (E)A[(long long)gm * lda + gk] : (E)0;
+        As[r][kq + q] = (gm < M && gk < ke) ? (E)A[(long long)gm * lda + gk] : (E)0;
       }
     }
     if (TRANS_B) {
@@ -63,7 +69,7 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, c
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int gk = k0 + kq + q;
-        Bs[r][kq + q] = (gn < N && gk < K) ? (E)B[(long long)gn * ldb + gk] : (E)0;
+        Bs[r][kq + q] = (gn < N && gk < ke) ? (E)B[(long long)gn * ldb + gk] : (E)0;
       }
     } else {
       const int kk = t >> 4, nq = (t & 15) * 4;
@@ -71,7 +77,7 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, c
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int gn = n0 + nq + q;
-        Bs[nq + q][kk] = (gn < N && gk < K) ? (E)B[(long long)gk * ldb + gn] : (E)0;
+        Bs[nq + q][kk] = (gn < N && gk < ke) ? (E)B[(long long)gk * ldb + gn] : (E)0;
       }
     }
     __syncthreads();
@@ -102,11 +108,29 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, c
         const int gm = m0 + wm + 16 * i + rr;
         const int gn = n0 + wn + 16 * j + (lane & 15);
         if (gm < M && gn < N && (!lower_only || gn <= gm)) {
+          if (P) {
+            P[((long long)blockIdx.z * M + gm) * N + gn] = (double)acc[i][j][r];
+            continue;
+          }
           const E v = (E)alpha * acc[i][j][r];
           if (OUT_D) reinterpret_cast<double*>(Cv)[(long long)gm * ldc + gn] = (double)v;
           else reinterpret_cast<float*>(Cv)[(long long)gm * ldc + gn] = (float)v;
         }
       }
+}
+
+// C[m][n] = alpha * sum_z P[z][m][n], z in order (the split-K slices).
+template <bool OUT_D>
+__global__ void k_gemm_reduce(const double* __restrict__ P, int S, int M, int N, void* __restrict__ Cv,
+                              long long ldc, double alpha, int lower_only) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)M * N) return;
+  const int m = (int)(i / N), n = (int)(i % N);
+  if (lower_only && n > m) return;
+  double v = 0.0;
+  for (int z = 0; z < S; ++z) v += P[(long long)z * M * N + i];
+  if (OUT_D) reinterpret_cast<double*>(Cv)[(long long)m * ldc + n] = alpha * v;
+  else reinterpret_cast<float*>(Cv)[(long long)m * ldc + n] = (float)(alpha * v);
 }
 
 // out[j] = sum_i A[i][j]^2 (column sums of squares of a rows x cols FP32
@@ -293,8 +317,29 @@ int dogpup_gemm_f32(const float* d_A, const float* d_B, void* d_C, int M, int N,
   dim3 grid(div_up(N, GT), div_up(M, GT));
   DP_ARG(grid.y <= 65535, "M too large");
   hipStream_t st = as_stream(stream);
-#define DP_GEMM(TB, OD, F6) \
-  k_gemm_f32<TB, OD, F6><<<grid, 256, 0, st>>>(d_A, d_B, d_C, M, N, K, lda, ldb, ldc, alpha, lower_only)
+  // Split K when the tiles alone cannot fill the GPU (normal matrices of a
+  // few thousand rows over a quarter-million columns): slices of at least
+  // kSplitMinK, enough of them for about kSplitBlocks blocks.  A block walks
+  // K in 16-wide steps with each step's loads waited for, so the blocks in
+  // flight per CU are what hides the load latency.
+  constexpr long long kSplitBlocks = 4096, kSplitMinK = 2048;
+  const long long tiles = lower_only ? (long long)grid.y * (grid.y + 1) / 2 : (long long)grid.x * grid.y;
+  int S = (int)std::min<long long>((kSplitBlocks + tiles - 1) / tiles, std::max<long long>(1, K / kSplitMinK));
+  S = std::max(1, std::min(S, 64));
+  double* P = nullptr;
+  int ks = K;
+  if (S > 1) {
+    ks = (int)(((long long)K + S - 1) / S);
+    ks = (ks + GK - 1) / GK * GK;
+    S = (int)(((long long)K + ks - 1) / ks);
+    // every entry the reduction reads (n <= m when lower_only) is written by
+    // each slice's tile, so the buffer needs no clearing
+    DP_HIP(hipMallocAsync((void**)&P, (size_t)S * M * N * sizeof(double), st));
+    grid.z = S;
+  }
+#define DP_GEMM(TB, OD, F6)                                                                                  \
+  k_gemm_f32<TB, OD, F6><<<grid, 256, 0, st>>>(d_A, d_B, d_C, M, N, K, lda, ldb, ldc, alpha, lower_only, P, \
+                                               ks)
   if (compute_f64) {
     if (trans_b) { if (out_f64) DP_GEMM(true, true, true); else DP_GEMM(true, false, true); }
     else { if (out_f64) DP_GEMM(false, true, true); else DP_GEMM(false, false, true); }
@@ -304,6 +349,13 @@ int dogpup_gemm_f32(const float* d_A, const float* d_B, void* d_C, int M, int N,
   }
 #undef DP_GEMM
   DP_LAUNCH_CHECK();
+  if (P) {
+    const long long n = (long long)M * N;
+    if (out_f64) k_gemm_reduce<true><<<div_up(n, 256), 256, 0, st>>>(P, S, M, N, d_C, ldc, alpha, lower_only);
+    else k_gemm_reduce<false><<<div_up(n, 256), 256, 0, st>>>(P, S, M, N, d_C, ldc, alpha, lower_only);
+    DP_LAUNCH_CHECK();
+    DP_HIP(hipFreeAsync(P, st));
+  }
   return DOGPUP_OK;
 }
 

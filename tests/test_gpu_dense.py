@@ -34,6 +34,28 @@ def test_mfma_gemm_layout_exact():
         assert np.array_equal(np.tril(Cl), np.tril(ref))
 
 
+def test_mfma_gemm_split_k_exact():
+    """Shapes whose K is split over several blocks (few output tiles over a
+    long K: 9 slices for 70 x 33 x 20000, 4 for the lower half of 130 x 130
+    x 9000) with small-integer operands: every partial and total is exact,
+    so the slices' sum equals the product exactly, in both precisions and in
+    the lower-only form."""
+    from dogpup_amd import recon
+    rng = np.random.default_rng(4)
+    for M, N, K in [(70, 33, 20000), (130, 130, 9000)]:
+        A = rng.integers(-4, 5, size=(M, K)).astype(np.float32)
+        B = rng.integers(-4, 5, size=(N, K)).astype(np.float32)
+        ref = A.astype(np.float64) @ B.T.astype(np.float64)
+        for exact in (False, True):
+            C = recon.gemm_nt(A, B, exact=exact).cpu().numpy()
+            assert np.array_equal(C, ref), (M, N, K, exact)
+        Bn = np.ascontiguousarray(B.T)
+        assert np.array_equal(recon.gemm_nn(A, Bn).cpu().numpy(), (A.astype(np.float64) @ Bn).astype(np.float32))
+        Cl = recon.gemm_nt(A, A, lower_only=True).cpu().numpy()
+        refl = A.astype(np.float64) @ A.T.astype(np.float64)
+        assert np.array_equal(np.tril(Cl), np.tril(refl))
+
+
 def test_cholesky_solve():
     from dogpup_amd import recon
     rng = np.random.default_rng(1)
