@@ -266,35 +266,70 @@ __global__ void k_trsv_update(const double* __restrict__ L, double* __restrict__
   if (idx >= (long long)nb * nrhs) return;
   const int r = (int)(idx / nrhs), c = (int)(idx % nrhs);
   const int row = k0 + r;
-  double s = 0.0;
-  if (!trans) {
-    for (int l = 0; l < k0; ++l) s += L[(long long)row * n + l] * B[(long long)l * nrhs + c];
-  } else {
-    for (int l = k0 + nb; l < n; ++l) s += L[(long long)l * n + row] * B[(long long)l * nrhs + c];
+  // eight partial sums, so that eight products' loads are in flight at once
+  // (one running sum waited for each load in turn)
+  constexpr int U = 8;
+  double ps[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) ps[u] = 0.0;
+  const int l0 = trans ? k0 + nb : 0, l1 = trans ? n : k0;
+  int l = l0;
+  for (; l + U <= l1; l += U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double lv = trans ? L[(long long)(l + u) * n + row] : L[(long long)row * n + l + u];
+      ps[u] += lv * B[(long long)(l + u) * nrhs + c];
+    }
   }
+  for (; l < l1; ++l) {
+    const double lv = trans ? L[(long long)l * n + row] : L[(long long)row * n + l];
+    ps[0] += lv * B[(long long)l * nrhs + c];
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) s += ps[u];
   B[(long long)row * nrhs + c] -= s;
 }
 
-__global__ void k_trsv_diag(const double* __restrict__ L, double* __restrict__ B, int n, int nrhs, int k0,
+__global__ __launch_bounds__(256) void k_trsv_diag(const double* __restrict__ L, double* __restrict__ B, int n,
+                                                   int nrhs, int k0,
                             int trans) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nrhs) return;
   const int nb = min(NB, n - k0);
+  // the thread's column of the block in registers (NB is a compile-time
+  // bound and every loop is unrolled), with the same operations in the same
+  // order as the in-memory form
+  double y[NB];
+#pragma unroll
+  for (int r = 0; r < NB; ++r) y[r] = r < nb ? B[(long long)(k0 + r) * nrhs + c] : 0.0;
   if (!trans) {
-    for (int r = 0; r < nb; ++r) {
-      const int row = k0 + r;
-      double s = B[(long long)row * nrhs + c];
-      for (int l = 0; l < r; ++l) s -= L[(long long)row * n + k0 + l] * B[(long long)(k0 + l) * nrhs + c];
-      B[(long long)row * nrhs + c] = s / L[(long long)row * n + row];
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+      if (r < nb) {
+        const int row = k0 + r;
+        double s = y[r];
+#pragma unroll
+        for (int l = 0; l < r; ++l) s -= L[(long long)row * n + k0 + l] * y[l];
+        y[r] = s / L[(long long)row * n + row];
+      }
     }
   } else {
-    for (int r = nb - 1; r >= 0; --r) {
-      const int row = k0 + r;
-      double s = B[(long long)row * nrhs + c];
-      for (int l = r + 1; l < nb; ++l) s -= L[(long long)(k0 + l) * n + row] * B[(long long)(k0 + l) * nrhs + c];
-      B[(long long)row * nrhs + c] = s / L[(long long)row * n + row];
+#pragma unroll
+    for (int r = NB - 1; r >= 0; --r) {
+      if (r < nb) {
+        const int row = k0 + r;
+        double s = y[r];
+#pragma unroll
+        for (int l = r + 1; l < NB; ++l)
+          if (l < nb) s -= L[(long long)(k0 + l) * n + row] * y[l];
+        y[r] = s / L[(long long)row * n + row];
+      }
     }
   }
+#pragma unroll
+  for (int r = 0; r < NB; ++r)
+    if (r < nb) B[(long long)(k0 + r) * nrhs + c] = y[r];
 }
 
 // A[i][i] += shift

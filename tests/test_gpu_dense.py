@@ -59,16 +59,17 @@ def test_mfma_gemm_split_k_exact():
 def test_cholesky_solve():
     from dogpup_amd import recon
     rng = np.random.default_rng(1)
-    for n in [5, 32, 33, 100]:
+    for n in [5, 32, 33, 100, 300]:
         X = rng.normal(size=(n, n))
         A = X @ X.T + n * np.eye(n)
         L = torch.as_tensor(np.tril(A), device="cuda").contiguous()
         recon.chol_factor(L, 0.5)
         Lh = np.tril(L.cpu().numpy())
         assert np.allclose(Lh @ Lh.T, A + 0.5 * np.eye(n), rtol=1e-12, atol=1e-10)
-        B = rng.normal(size=(n, 3))
-        Xs = recon.chol_solve(L, B).cpu().numpy()
-        assert np.allclose((A + 0.5 * np.eye(n)) @ Xs, B, rtol=1e-9, atol=1e-9)
+        for nrhs in (3, 243):
+            B = rng.normal(size=(n, nrhs))
+            Xs = recon.chol_solve(L, B).cpu().numpy()
+            assert np.allclose((A + 0.5 * np.eye(n)) @ Xs, B, rtol=1e-9, atol=1e-9)
 
 
 def test_tik_inv_and_reshape_J_vs_oracle():
