@@ -72,6 +72,9 @@ def build_workload(small=False):
         nf = 8
     else:
         m, src, det, link = meshgen.slab_80x80x40_h1(seed=0)
+        # nodes on the solver's Morton curve (iso2mesh's sortmesh does the
+        # same for locality): the solves then skip their row permutations
+        m = meshgen.sorted_mesh(m)
         nf = 64
     t = np.arange(0, 1001) * 20e-12                     # T = 20 ns -> 50 MHz spacing
     tpsf = meshgen.gaussian_tpsf(t, 250e-12)
@@ -79,8 +82,9 @@ def build_workload(small=False):
     opt = dm.DOptode(src, det, link, 1e3, tpsf, t, nf)
     D.add_optode(opt)
     D.dxyz = m.h                                         # voxel side of the nodal basis
+    ident = bool(np.array_equal(D.solver_order, np.arange(D.nn)))
     log(f"mesh {D.nn} nodes, {D.elem.shape[0]} tets, nnz(A) {D.nnzA}, nnz(G) {D.nnzG}, "
-        f"host prep {time.time() - t0:.1f} s")
+        f"host prep {time.time() - t0:.1f} s, node order = solver order: {ident}")
     return D, opt, t
 
 

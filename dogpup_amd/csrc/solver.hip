@@ -2164,6 +2164,10 @@ struct dogpup_solver {
   const int *rA = nullptr, *cA = nullptr, *rG = nullptr, *cG = nullptr;
   const int *rGT = nullptr, *cGT = nullptr;
   dogpup::Buf pAr, pAc, pGr, pGc, pGTr, pGTc, amap, gmap, gtmap, order;
+  // the caller's node order is the solver's (a mesh already on its Morton
+  // curve): the solves then work in the caller's b and x, with no row
+  // permutation in or out
+  bool order_identity = false;
   // x-halo lists of the three solver-order patterns (A, G, G^T) per 64-row
   // chunk of each mesh (k_spmv_halo); halo_ok[m]: every chunk of pattern m
   // fits the LDS halo; use_halo: the complex64 SpMVs take the halo form
@@ -2797,9 +2801,10 @@ static int solve_c128(dogpup_solver* S, hipStream_t st, const double* b, double*
     const int gc = C.gcount, g0 = S->chunk_g[c];
     Vecs<double> V = carve<double>(S->ws, (size_t)C.nodes * NSP * 16, false, false);
     const size_t goff = (size_t)C.W.vb0 * NSP * 2;
-    DP_TRY(permute_block<NSP>(S, st, C.W, b + goff, S->bperm.as<double>(), 0));
-    V.x = S->xperm.as<double>();
-    const double* bc = S->bperm.as<double>();
+    const bool ident = S->order_identity;
+    if (!ident) DP_TRY(permute_block<NSP>(S, st, C.W, b + goff, S->bperm.as<double>(), 0));
+    V.x = ident ? x + goff : S->xperm.as<double>();
+    const double* bc = ident ? b + goff : S->bperm.as<double>();
     k_fill_tol<<<div_up(gc * NSP, 256), 256, 0, st>>>(C.bs.tol, rhs_tol_at(S, g0), tol, gc * NSP);
     DP_LAUNCH_CHECK();
     int its = 0;
@@ -2816,7 +2821,7 @@ static int solve_c128(dogpup_solver* S, hipStream_t st, const double* b, double*
     DP_HIP(hipMemcpyAsync(part.data(), C.p1, part.size() * 8, hipMemcpyDeviceToHost, st));
     std::vector<double> bn(gc * NSP);
     DP_HIP(hipMemcpyAsync(bn.data(), C.bs.bnorm, bn.size() * 8, hipMemcpyDeviceToHost, st));
-    DP_TRY(permute_block<NSP>(S, st, C.W, V.x, x + goff, 1));
+    if (!ident) DP_TRY(permute_block<NSP>(S, st, C.W, V.x, x + goff, 1));
     DP_TRY(export_results(S, st, g0, gc, C.bs.iters, C.bs.relres, C.bs.status, h_iters, h_relres, h_status));
     // relres = |b - A x| / |b| from the residual pass's partials
     for (int g = 0; g < gc; ++g) {
@@ -3170,7 +3175,9 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
     // order) is first written by the first sweep's correction, or by the
     // guess of its phase
     const double* bc = b + goff;
-    double* xc = S->xperm.as<double>();
+    const bool ident = S->order_identity;
+    double* xc = ident ? x + goff : S->xperm.as<double>();
+    const int* border = ident ? nullptr : S->order.as<int>();   // b's rows in the caller's order
     const bool cocg = S->method == DOGPUP_METHOD_COCG;
     const Work Wfull = C.W;
     std::vector<dogpup_solver::Phase> one{{0, Wfull.w0, Wfull.nw, false, C.nodes}};
@@ -3179,7 +3186,7 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
       // groups of later phases take no part until their phase
       k_fill_int<<<div_up(gc * NSP, 256), 256, 0, st>>>(R.status, ST_SKIP, gc * NSP);
       DP_LAUNCH_CHECK();
-      if (S->guess_self) DP_TRY(permute_block<NSP>(S, st, Wfull, x + goff, xc, 0));
+      if (S->guess_self && !ident) DP_TRY(permute_block<NSP>(S, st, Wfull, x + goff, xc, 0));
     }
     for (size_t ip = 0; ip < phases.size(); ++ip) {
       const dogpup_solver::Phase& ph = phases[ip];
@@ -3209,7 +3216,7 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
         // COCG starts from r alone (p = G^T G r is written by its start), so
         // only BiCGSTAB needs p and rhat
         k_residual64<NSP><<<C.vgrid(), 256, 0, st>>>(
-            S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), bc, S->order.as<int>(),
+            S->rA, S->cA, S->a64pre.as<double>(), S->a64pim.as<double>(), bc, border,
             from0 ? nullptr : xc, V.r, cocg ? nullptr : V.rhat, cocg ? nullptr : V.p, V.x,
             sweep == 0 ? nullptr : R.status, 1, C.p1, C.p2, C.W, S->halo_ilv ? 1 : 0, p3);
         DP_LAUNCH_CHECK();
@@ -3255,7 +3262,7 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
       }
     }
     C.W = Wfull;
-    DP_TRY(permute_block<NSP>(S, st, C.W, xc, x + goff, 1));
+    if (!ident) DP_TRY(permute_block<NSP>(S, st, C.W, xc, x + goff, 1));
     DP_TRY(export_results(S, st, g0, gc, R.iters, R.relres, R.status, h_iters, h_relres, h_status));
   }
   return DOGPUP_OK;
@@ -3271,9 +3278,10 @@ static int solve_plain(dogpup_solver* S, hipStream_t st, const double* b, double
     const size_t vbytes = (size_t)C.nodes * NSP * 16;
     Vecs<double> V = carve<double>(S->ws, vbytes, false, true);
     const size_t goff = (size_t)W.vb0 * NSP * 2;
-    DP_TRY(permute_block<NSP>(S, st, W, b + goff, S->bperm.as<double>(), 0));
-    V.x = S->xperm.as<double>();
-    const double* bc = S->bperm.as<double>();
+    const bool ident = S->order_identity;
+    if (!ident) DP_TRY(permute_block<NSP>(S, st, W, b + goff, S->bperm.as<double>(), 0));
+    V.x = ident ? x + goff : S->xperm.as<double>();
+    const double* bc = ident ? b + goff : S->bperm.as<double>();
     unsigned long long* dmax = reinterpret_cast<unsigned long long*>(C.bs.counter);
     const dim3 vg = C.vgrid();
     auto maxerr = [&](int op, const double* pa, double* out) -> int {
@@ -3343,7 +3351,7 @@ static int solve_plain(dogpup_solver* S, hipStream_t st, const double* b, double
       k_axpby<double, NSP><<<vg, 256, 0, st>>>(V.p, V.r, V.p, C.bs.beta, 2, W, nullptr);  // p = r + beta p
       DP_LAUNCH_CHECK();
     }
-    DP_TRY(permute_block<NSP>(S, st, W, V.x, x + goff, 1));
+    if (!ident) DP_TRY(permute_block<NSP>(S, st, W, V.x, x + goff, 1));
     S->stats[0] = std::max(S->stats[0], (double)std::min(it, maxit));
     S->stats[1] += (double)std::min(it, maxit) * gc;
     // status: converged if the lockstep loop met tol
@@ -3905,6 +3913,8 @@ static int create_batch(dogpup_solver_t* out, int nmesh, const int* h_row0, cons
   DP_TRY(up(S->pGr, rGp)); DP_TRY(up(S->pGc, cGp)); DP_TRY(up(S->gmap, gmap));
   DP_TRY(up(S->pGTr, rGT)); DP_TRY(up(S->pGTc, cGT)); DP_TRY(up(S->gtmap, gtmap));
   DP_TRY(up(S->order, order));
+  S->order_identity = true;
+  for (int i = 0; i < nn && S->order_identity; ++i) S->order_identity = order[i] == i;
   S->rA = S->pAr.as<int>(); S->cA = S->pAc.as<int>();
   S->rG = S->pGr.as<int>(); S->cG = S->pGc.as<int>();
   S->rGT = S->pGTr.as<int>(); S->cGT = S->pGTc.as<int>();

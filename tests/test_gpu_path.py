@@ -190,6 +190,33 @@ def test_solver_fsai_factors(setup):
     assert rel(got, valG) < 1e-12
 
 
+@pytest.mark.parametrize("mode,tol,bound,method", [("mixed", 1e-10, 1e-7, "cocg"), ("fp64", 1e-12, 1e-9, "cocg"),
+                                                   ("plain", 1e-12, 1e-9, "bicgstab")])
+def test_solver_on_morton_sorted_mesh_vs_direct(setup, mode, tol, bound, method):
+    """A mesh whose nodes already follow the solver's Morton order
+    (meshgen.sorted_mesh): the handle works in the caller's b and x with no
+    row permutation, and every form still solves to the oracle's direct
+    solution of that mesh."""
+    from dogpup_amd import meshgen, mesh as dm
+    from oracle import fem, optode as oopt, solver as osol
+    ms = meshgen.sorted_mesh(setup["m"])
+    D = dm.DMesh(ms.node, ms.elem, ms.nr, np.stack([ms.mua, ms.musp], 1))
+    assert np.array_equal(D.solver_order, np.arange(D.nn))
+    D.add_optode(dm.DOptode(setup["src"], setup["det"], setup["link"], 1e3,
+                            meshgen.gaussian_tpsf(setup["t"], 250e-12), setup["t"], setup["nf"]))
+    om = fem.make_mesh(ms.node, ms.elem, ms.nr, ms.mua, ms.musp)
+    A = fem.full_system(om, setup["df"])
+    sid, sw = D.optode.s_bary
+    ref = osol.direct(A, oopt.source_matrix(om, sid, sw, D.optode.s_fpsf))
+    print(f"[case] sorted mesh {mode} {method}", flush=True)
+    phi = D.solve_point_sources("s", mode=mode, tol=tol, method=method)
+    got = phi.to_reference_layout().cpu().numpy()
+    assert (phi.status == 0).all(), phi.status
+    for k in range(setup["nf"]):
+        for c in range(ref.shape[2]):
+            assert rel(got[:, k, c], ref[:, k, c]) < bound, (mode, k, c)
+
+
 @pytest.mark.parametrize("mode", ["mixed", "fp64", "plain"])
 def test_solver_fsai_direct_matches_gather(setup, mode, monkeypatch):
     """The solver's FSAI setup through the A-position table, written straight
