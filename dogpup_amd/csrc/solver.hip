@@ -29,6 +29,7 @@
 // of all meshes, so meshes of very different sizes fill one launch together.
 #include <cmath>
 #include <cstring>
+#include <cstdint>
 #include <cstdlib>
 #include <algorithm>
 #include <map>
@@ -2166,8 +2167,11 @@ struct dogpup_solver {
   dogpup::Buf pAr, pAc, pGr, pGc, pGTr, pGTc, amap, gmap, gtmap, order;
   // the caller's node order is the solver's (a mesh already on its Morton
   // curve): the solves then work in the caller's b and x, with no row
-  // permutation in or out
-  bool order_identity = false;
+  // permutation in or out, unless b and x overlap (ident_solve: this solve
+  // does).  The FP64 fused form's captured iteration then updates the
+  // caller's x in place, so its graphs hold that address (graph_x).
+  bool order_identity = false, ident_solve = false;
+  const double* graph_x = nullptr;
   // x-halo lists of the three solver-order patterns (A, G, G^T) per 64-row
   // chunk of each mesh (k_spmv_halo); halo_ok[m]: every chunk of pattern m
   // fits the LDS halo; use_halo: the complex64 SpMVs take the halo form
@@ -2801,7 +2805,7 @@ static int solve_c128(dogpup_solver* S, hipStream_t st, const double* b, double*
     const int gc = C.gcount, g0 = S->chunk_g[c];
     Vecs<double> V = carve<double>(S->ws, (size_t)C.nodes * NSP * 16, false, false);
     const size_t goff = (size_t)C.W.vb0 * NSP * 2;
-    const bool ident = S->order_identity;
+    const bool ident = S->ident_solve;
     if (!ident) DP_TRY(permute_block<NSP>(S, st, C.W, b + goff, S->bperm.as<double>(), 0));
     V.x = ident ? x + goff : S->xperm.as<double>();
     const double* bc = ident ? b + goff : S->bperm.as<double>();
@@ -3175,7 +3179,7 @@ static int solve_refine(dogpup_solver* S, hipStream_t st, const double* b, doubl
     // order) is first written by the first sweep's correction, or by the
     // guess of its phase
     const double* bc = b + goff;
-    const bool ident = S->order_identity;
+    const bool ident = S->ident_solve;
     double* xc = ident ? x + goff : S->xperm.as<double>();
     const int* border = ident ? nullptr : S->order.as<int>();   // b's rows in the caller's order
     const bool cocg = S->method == DOGPUP_METHOD_COCG;
@@ -3278,7 +3282,7 @@ static int solve_plain(dogpup_solver* S, hipStream_t st, const double* b, double
     const size_t vbytes = (size_t)C.nodes * NSP * 16;
     Vecs<double> V = carve<double>(S->ws, vbytes, false, true);
     const size_t goff = (size_t)W.vb0 * NSP * 2;
-    const bool ident = S->order_identity;
+    const bool ident = S->ident_solve;
     if (!ident) DP_TRY(permute_block<NSP>(S, st, W, b + goff, S->bperm.as<double>(), 0));
     V.x = ident ? x + goff : S->xperm.as<double>();
     const double* bc = ident ? b + goff : S->bperm.as<double>();
@@ -4198,6 +4202,18 @@ int dogpup_solver_solve(dogpup_solver_t S, const double* d_b, double* d_x, doubl
   DP_HIP(hipStreamWaitEvent(S->own, S->ev_in, 0));
   hipStream_t st = S->own;
   for (double& v : S->stats) v = 0.0;
+  // b and x in place of each other (or overlapping) take the permutation
+  // buffers, which read b before x is written
+  {
+    const uintptr_t bytes = (uintptr_t)(S->h_gvb.back() + S->h_grows.back()) * (uintptr_t)S->nsp * 16;
+    const uintptr_t xa = reinterpret_cast<uintptr_t>(d_x), ba = reinterpret_cast<uintptr_t>(d_b);
+    S->ident_solve = S->order_identity && (xa + bytes <= ba || ba + bytes <= xa);
+  }
+  const double* xg = S->ident_solve ? d_x : nullptr;
+  if (S->mode == DOGPUP_MODE_C128 && xg != S->graph_x) {
+    S->clear_graphs();
+    S->graph_x = xg;
+  }
   int rc;
   switch (S->mode) {
     case DOGPUP_MODE_C128:

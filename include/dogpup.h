@@ -199,7 +199,10 @@ int dogpup_solver_set_matrix(dogpup_solver_t s, const double* d_valA_re, const d
  * tol: relative residual ||b - A x|| / ||b|| per RHS (FP64 true residual in
  * C64_REFINE mode); maxit: BiCGSTAB iterations (all sweeps together in
  * C64_REFINE).  Outputs (host, nfreq*ncols each, index freq*ncols + col):
- * iterations, final relative residual, status.  Synchronous. */
+ * iterations, final relative residual, status.  Synchronous.  d_x may be
+ * d_b (in place).  With the node order the identity the solve iterates in
+ * d_x itself (no row permutation), and a later solve into another buffer
+ * recaptures the FP64 fused form's iteration graph. */
 int dogpup_solver_solve(dogpup_solver_t s, const double* d_b, double* d_x, double tol, int maxit,
                         int* h_iters, double* h_relres, int* h_status, void* stream);
 

@@ -217,6 +217,38 @@ def test_solver_on_morton_sorted_mesh_vs_direct(setup, mode, tol, bound, method)
             assert rel(got[:, k, c], ref[:, k, c]) < bound, (mode, k, c)
 
 
+@pytest.mark.parametrize("mode,method", [("mixed", "cocg"), ("fp64", "cocg"), ("fp64", "bicgstab"),
+                                         ("plain", "bicgstab")])
+def test_sorted_mesh_new_x_buffer_and_in_place(setup, mode, method):
+    """On a mesh in the solver's order the solves iterate in the caller's x,
+    so the FP64 fused form's captured iteration holds that address: a second
+    solve into another buffer must recapture, not replay into the first one.
+    A solve with x = b (in place) reads b before writing x.  Each of the three
+    solves gives the first one's field bit for bit."""
+    from dogpup_amd import meshgen, mesh as dm
+    ms = meshgen.sorted_mesh(setup["m"])
+    D = dm.DMesh(ms.node, ms.elem, ms.nr, np.stack([ms.mua, ms.musp], 1))
+    D.add_optode(dm.DOptode(setup["src"], setup["det"], setup["link"], 1e3,
+                            meshgen.gaussian_tpsf(setup["t"], 250e-12), setup["t"], setup["nf"]))
+    S = D.solver(4, mode, method)
+    rng = np.random.default_rng(11)
+    shp = S.empty_block().shape
+    b = torch.as_tensor(rng.standard_normal(shp) + 1j * rng.standard_normal(shp)).to("cuda")
+    tol = 1e-10 if mode == "mixed" else 1e-12
+    print(f"[case] sorted mesh, new x and in place: {mode} {method}", flush=True)
+    f1 = S.solve(b, tol=tol)
+    x1 = f1.data.clone()
+    f2 = S.solve(b, tol=tol, x=torch.full_like(b, 7.0))   # another buffer, held while x1 is alive
+    bi = b.clone()
+    f3 = S.solve(bi, tol=tol, x=bi)
+    torch.cuda.synchronize()
+    assert (f1.status == 0).all(), f1.status
+    real = lambda t: torch.view_as_real(t[..., :4]).contiguous().view(torch.int64)
+    assert torch.equal(real(f2.data), real(x1)), "second buffer"
+    assert torch.equal(real(f3.data), real(x1)), "in place"
+    assert np.array_equal(f2.iters, f1.iters) and np.array_equal(f3.iters, f1.iters)
+
+
 @pytest.mark.parametrize("mode", ["mixed", "fp64", "plain"])
 def test_solver_fsai_direct_matches_gather(setup, mode, monkeypatch):
     """The solver's FSAI setup through the A-position table, written straight
