@@ -36,7 +36,23 @@ def sixteen():
     A = fem.full_system(om, df)
     sid, sw = opt.s_bary
     phi = osol.direct(A, oopt.source_matrix(om, sid, sw, opt.s_fpsf))
-    return dict(D=D, phi=phi)
+    return dict(D=D, phi=phi, m=m, src=src, det=det, t=t, nf=nf)
+
+
+def _sorted_mesh_of(sixteen):
+    """The fixture's mesh renumbered along the solver's Morton curve (a
+    handle on it iterates in the caller's x) and the direct solution in that
+    node order: the sources sit at the same points, so the fields are the
+    same nodal values renumbered."""
+    from dogpup_amd import meshgen, meshio, mesh as dm
+    m = sixteen["m"]
+    ms = meshgen.sorted_mesh(m)
+    order = meshio.sortmesh(m.node, m.elem)[2]
+    D = dm.DMesh(ms.node, ms.elem, ms.nr, np.stack([ms.mua, ms.musp], 1))
+    D.add_optode(dm.DOptode(sixteen["src"], sixteen["det"], meshgen.all_pairs(9, 4), 1e3,
+                            meshgen.gaussian_tpsf(sixteen["t"], 250e-12), sixteen["t"], sixteen["nf"]))
+    assert np.array_equal(D.solver_order, np.arange(D.nn))
+    return D, sixteen["phi"][order]
 
 
 @pytest.mark.parametrize("method", ["cocg", "bicgstab"])
@@ -135,3 +151,43 @@ def test_continuation_across_workspace_chunks(sixteen, monkeypatch):
         out[cont] = fw.iters.sum()
     print("iterations without / with continuation:", out[False], out[True], flush=True)
     assert out[True] < 0.8 * out[False]
+
+
+def test_sorted_mesh_chunked_continuation_and_warm_start(sixteen, monkeypatch):
+    """On a mesh in the solver's order the refinement form iterates in the
+    caller's x: chunks of a small workspace read the earlier chunks'
+    solutions from it and write their own in place, and a warm start reads
+    each group's entry before its guess overwrites it.  Every column matches
+    the direct solve, with and without continuation, and the warm start
+    from a perturbed solution needs fewer iterations."""
+    from dogpup_amd import _lib
+    monkeypatch.setenv("DOGPUP_WORKSPACE_GB", "0.015")       # about 6 of the 16 groups per chunk
+    D, phi = _sorted_mesh_of(sixteen)
+    out = {}
+    for cont in (False, True):
+        print(f"[case] sorted mesh, chunked, continuation={cont}", flush=True)
+        fw = D.solve_point_sources("s", mode="mixed", tol=1e-10, cont=cont)
+        assert (fw.status == 0).all()
+        got = fw.to_reference_layout().cpu().numpy()
+        for k in range(got.shape[1]):
+            assert rel(got[:, k], phi[:, k]) < 1e-7, (cont, k)
+        out[cont] = fw
+    assert out[True].iters.sum() < 0.8 * out[False].iters.sum()
+    fw = out[True]
+    S = D.solver(fw.ncols, "mixed")
+    nf = S.nf
+    src = np.arange(nf, dtype=np.int32)[:, None].copy()               # each frequency: itself
+    w = np.zeros((nf, 1, 2))
+    w[..., 0] = 1.0
+    _lib.call("dogpup_solver_set_guess", S.h, 1, _lib.ptr(src), _lib.ptr(w))
+    S._cont_key = None
+    b = S.empty_block(zero=False)
+    spec = torch.as_tensor(D.optode.s_fpsf, dtype=torch.complex128, device="cuda")
+    nodes, wq = D._bary_dev("s")
+    _lib.call("dogpup_point_sources", _lib.ptr(torch.view_as_real(b)), D.nn, nf, fw.ncols, S.nsp,
+              _lib.ptr(nodes), _lib.ptr(wq), _lib.ptr(torch.view_as_real(spec)), _lib.stream_ptr())
+    print("[case] sorted mesh, chunked warm start", flush=True)
+    x = fw.data * (1.0 + 1e-3 * torch.randn_like(fw.data.real))
+    f3 = S.solve(b, tol=1e-10, x=x)
+    assert (f3.status == 0).all() and f3.iters.max() < fw.iters.max()
+    assert rel(f3.to_reference_layout().cpu().numpy(), phi) < 1e-7
