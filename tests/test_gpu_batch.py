@@ -14,13 +14,17 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-def _batch_meshes():
+def _batch_meshes(morton=False):
+    """Three uneven slabs; morton=True renumbers each along the solver's
+    Morton curve first (the batch then iterates in the caller's x)."""
     from dogpup_amd import meshgen, mesh as dm
     t = np.arange(0, 501) * 20e-12
     specs = [((20.0, 20.0, 10.0), 2.0, 3, 4), ((14.0, 10.0, 8.0), 1.0, 5, 6), ((30.0, 24.0, 12.0), 3.0, 12, 3)]
     out = []
     for k, (ext, h, ns, nf) in enumerate(specs):
         m = small_slab(seed=k + 1, extent=ext, h=h)
+        if morton:
+            m = meshgen.sorted_mesh(m)
         nx = int(np.ceil(np.sqrt(ns)))
         ny = -(-ns // nx)
         src = meshgen.top_grid(m.extent, nx, ny, 0.25 * ext[0], 0.25 * ext[1], 0.5 * ext[0] / max(nx - 1, 1),
@@ -32,13 +36,15 @@ def _batch_meshes():
     return out
 
 
-@pytest.mark.parametrize("mode,method", [("mixed", "cocg"), ("mixed", "bicgstab"), ("fp64", "cocg")])
-def test_batch_equals_single_mesh_solves(mode, method):
+@pytest.mark.parametrize("mode,method,morton", [("mixed", "cocg", False), ("mixed", "bicgstab", False),
+                                                ("fp64", "cocg", False), ("mixed", "cocg", True),
+                                                ("fp64", "cocg", True)])
+def test_batch_equals_single_mesh_solves(mode, method, morton):
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     from dogpup_amd import batch, mesh as dm
-    meshes = [D for D, _ in _batch_meshes()]
-    print(f"[case] batch {mode} {method}", flush=True)
+    meshes = [D for D, _ in _batch_meshes(morton)]
+    print(f"[case] batch {mode} {method} morton={morton}", flush=True)
     B = batch.MeshBatch(meshes, "s", mode=mode, nsp=8, method=method)
     tol = 1e-10 if mode == "mixed" else 1e-12
     fields = B.solve(tol=tol)
@@ -59,12 +65,19 @@ def test_batch_equals_single_mesh_solves(mode, method):
         assert (single.iters == f.iters).all()
 
 
-def test_batch_fp64_vs_direct():
+@pytest.mark.parametrize("morton", [False, True])
+def test_batch_fp64_vs_direct(morton):
+    """Every mesh of an FP64 batch against the oracle's direct solve; with
+    Morton-sorted meshes the handle's node order is the identity and the
+    solve runs in the caller's b and x."""
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     from dogpup_amd import batch
     from oracle import fem, optode as oopt, solver as osol, transforms as otr
-    pairs = _batch_meshes()
+    pairs = _batch_meshes(morton)
+    if morton:
+        assert all(np.array_equal(D.solver_order, np.arange(D.nn)) for D, _ in pairs)
+    print(f"[case] fp64 batch vs direct, morton={morton}", flush=True)
     B = batch.MeshBatch([D for D, _ in pairs], "s", mode="fp64", nsp=8, method="cocg")
     fields = B.solve(tol=1e-12)
     for (D, m), f in zip(pairs, fields):
