@@ -210,12 +210,13 @@ def batch_leg(tol):
     tpsf = meshgen.gaussian_tpsf(t, 250e-12)
     meshes = []
     for sp in specs:
-        m = sp["mesh"]
+        m = meshgen.sorted_mesh(sp["mesh"])       # node order = solver order, as for the 269k-node mesh
         D = dm.DMesh(m.node, m.elem, m.nr, np.stack([m.mua, m.musp], 1))
         D.add_optode(dm.DOptode(sp["src"], sp["det"], sp["link"], 1e3, tpsf, t, sp["nf"]))
         meshes.append(D)
     nrhs = sum(len(sp["src"]) * sp["nf"] for sp in specs)
     out = {"instances": len(meshes), "nodes": int(sum(D.nn for D in meshes)), "rhs": nrhs,
+           "node_order_is_solver_order": all(np.array_equal(D.solver_order, np.arange(D.nn)) for D in meshes),
            "node_rhs": float(sum(D.nn * len(sp["src"]) * sp["nf"] for D, sp in zip(meshes, specs))),
            "host_prep_s": time.time() - t0}
     log(f"batch: {len(meshes)} meshes, {out['nodes']} nodes, {nrhs} RHS; autotuning")
