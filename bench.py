@@ -417,6 +417,7 @@ def fd_leg(n_nodes=50, seed=0, delta=1e-5):
     from dogpup_amd.transforms import td2fc
     t = np.arange(0, 1001) * 20e-12
     m, src, det, _, = meshgen.slab_80x80x40_h1(seed=0)
+    m = meshgen.sorted_mesh(m)                    # node order = solver order, as in the other legs
     src = src[[0, 9, 18, 27]]
     det = det[[4, 13, 22, 31]]
     D = dm.DMesh(m.node, m.elem, m.nr, np.stack([m.mua, m.musp], 1))
@@ -480,6 +481,7 @@ def lm_leg(max_iter=10):
     out = {}
     t0 = time.time()
     mf, src, det, link, inc = meshgen.slab_80x80x40_two_inclusions(0.75, seed=0)
+    mf = meshgen.sorted_mesh(mf)
     Df = dm.DMesh(mf.node, mf.elem, mf.nr, np.stack([mf.mua, mf.musp], 1))
     Df.add_optode(dm.DOptode(src, det, link, 1e3, tpsf, t, 64))
     log(f"LM data mesh {Df.nn} nodes (host prep {time.time() - t0:.1f} s); FP64 forward solve")
@@ -491,6 +493,7 @@ def lm_leg(max_iter=10):
     gc.collect()
     torch.cuda.empty_cache()
     m, src, det, link, inc = meshgen.slab_80x80x40_two_inclusions(1.0, seed=0)
+    m = meshgen.sorted_mesh(m)                    # node order = solver order (no row permutations)
     truth = m.mua.copy()
     xs = np.arange(8.0, 73.0, 8.0)
     pos_s = np.stack(np.meshgrid(xs, xs, [30.0, 25.0, 20.0], indexing="ij"), -1).reshape(-1, 3)
