@@ -194,8 +194,9 @@ class SplitBatch:
     matrix entry read once for more RHS), the rest in narrow ones (little
     padding).  On the 48-mesh uneven batch (8-32 sources per mesh), 32-wide
     groups for the 18 meshes with more than 24 sources and 8-wide ones for
-    the other 30 measured 616 ms per solve against 688 ms in one 8-wide
-    handle.  (The parts on one host thread each took 577 ms, but a capture
+    the other 30 measured 606 ms per solve against 655 ms in one 8-wide
+    handle.  (The parts on one host thread each took 577 ms against 616 ms
+    one after the other on an earlier build, but a capture
     of a part's graph then fails now and then: another thread's device-wide
     synchronisation, such as a handle destroyed by the garbage collector,
     can fall inside it.)
