@@ -6,7 +6,8 @@
 // GEMM tiles: a 256-thread block (4 wavefronts) computes a 64 x 64 tile of
 // C; each wavefront owns a 32 x 32 quarter as 2 x 2 tiles of
 // v_mfma_f32_16x16x4_f32 (or v_mfma_f64_16x16x4_f64).  Operand tiles of
-// 64 x 16 are staged through LDS per K step.  MFMA 16x16x4 lane layout (both
+// 64 x 16 per K step alternate between two LDS buffers, the next step's
+// loads in registers while the current one multiplies.  MFMA 16x16x4 lane layout (both
 // precisions): lane l supplies A[l%16][l/16] and B[l/16][l%16] (K index
 // l/16); its accumulator register r holds D[4*(l/16)+r][l%16] for FP32 and
 // D[l/16+4*r][l%16] for FP64 (measured on gfx950 by an exact-integer test).
@@ -40,8 +41,10 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, c
                                                   int lower_only, double* __restrict__ P, int ks) {
   using E = typename std::conditional<F64, double, float>::type;
   using V4 = typename std::conditional<F64, f64x4, f32x4>::type;
-  __shared__ E As[GT][GK + 1];
-  __shared__ E Bs[GT][GK + 1];
+  // two LDS buffers: step s is stored into buffer s % 2, so one barrier per
+  // step orders the stores after every wave's reads of the step before
+  __shared__ E As[2][GT][GK + 1];
+  __shared__ E Bs[2][GT][GK + 1];
   const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
   if (lower_only && n0 > m0 + GT - 1) return;  // tile entirely above the diagonal
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -53,49 +56,61 @@ __global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict__ A, c
     for (int j = 0; j < 2; ++j) acc[i][j] = V4{0, 0, 0, 0};
   const int kb = P ? blockIdx.z * ks : 0;
   const int ke = P ? min(K, kb + ks) : K;
-  for (int k0 = kb; k0 < ke; k0 += GK) {
-    {
-      const int r = t >> 2, kq = (t & 3) * 4;
-      const int gm = m0 + r;
+  // Software pipeline: the K step after the current one is loaded into
+  // registers (ra, rb) before the current step's MFMAs, so its load latency
+  // overlaps them; out-of-range elements are zeros.  Thread t covers A row
+  // t/4 (and B row t/4, or B k-row t/16 when not transposed), 4 elements.
+  const int ar = t >> 2, akq = (t & 3) * 4;
+  const int bkk = t >> 4, bnq = (t & 15) * 4;
+  float ra[4], rb[4];
+  auto gload = [&](int k0) __attribute__((always_inline)) {
+    const int gm = m0 + ar;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int gk = k0 + kq + q;
-        As[r][kq + q] = (gm < M && gk < ke) ? (E)A[(long long)gm * lda + gk] : (E)0;
-      }
+    for (int q = 0; q < 4; ++q) {
+      const int gk = k0 + akq + q;
+      ra[q] = (gm < M && gk < ke) ? A[(long long)gm * lda + gk] : 0.f;
     }
     if (TRANS_B) {
-      const int r = t >> 2, kq = (t & 3) * 4;
-      const int gn = n0 + r;
+      const int gn = n0 + ar;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int gk = k0 + kq + q;
-        Bs[r][kq + q] = (gn < N && gk < ke) ? (E)B[(long long)gn * ldb + gk] : (E)0;
+        const int gk = k0 + akq + q;
+        rb[q] = (gn < N && gk < ke) ? B[(long long)gn * ldb + gk] : 0.f;
       }
     } else {
-      const int kk = t >> 4, nq = (t & 15) * 4;
-      const int gk = k0 + kk;
+      const int gk = k0 + bkk;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int gn = n0 + nq + q;
-        Bs[nq + q][kk] = (gn < N && gk < ke) ? (E)B[(long long)gk * ldb + gn] : (E)0;
+        const int gn = n0 + bnq + q;
+        rb[q] = (gn < N && gk < ke) ? B[(long long)gk * ldb + gn] : 0.f;
       }
     }
+  };
+  if (kb < ke) gload(kb);
+  int buf = 0;
+  for (int k0 = kb; k0 < ke; k0 += GK, buf ^= 1) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      As[buf][ar][akq + q] = (E)ra[q];
+      if (TRANS_B) Bs[buf][ar][akq + q] = (E)rb[q];
+      else Bs[buf][bnq + q][bkk] = (E)rb[q];
+    }
     __syncthreads();
+    if (k0 + GK < ke) gload(k0 + GK);
 #pragma unroll
     for (int kk = 0; kk < GK; kk += 4) {
       const int ka = kk + (lane >> 4);
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        const E a = As[wm + 16 * i + (lane & 15)][ka];
+        const E a = As[buf][wm + 16 * i + (lane & 15)][ka];
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          const E b = Bs[wn + 16 * j + (lane & 15)][ka];
+          const E b = Bs[buf][wn + 16 * j + (lane & 15)][ka];
           if constexpr (F64) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i][j], 0, 0, 0);
           else acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[i][j], 0, 0, 0);
         }
       }
     }
-    __syncthreads();
   }
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -355,8 +370,8 @@ int dogpup_gemm_f32(const float* d_A, const float* d_B, void* d_C, int M, int N,
   // Split K when the tiles alone cannot fill the GPU (normal matrices of a
   // few thousand rows over a quarter-million columns): slices of at least
   // kSplitMinK, enough of them for about kSplitBlocks blocks.  A block walks
-  // K in 16-wide steps with each step's loads waited for, so the blocks in
-  // flight per CU are what hides the load latency.
+  // K in 16-wide steps and has only the next step's loads in flight, so the
+  // blocks in flight per CU hide the rest of the load latency.
   constexpr long long kSplitBlocks = 4096, kSplitMinK = 2048;
   const long long tiles = lower_only ? (long long)grid.y * (grid.y + 1) / 2 : (long long)grid.x * grid.y;
   int S = (int)std::min<long long>((kSplitBlocks + tiles - 1) / tiles, std::max<long long>(1, K / kSplitMinK));
