@@ -3936,31 +3936,51 @@ static int create_batch(dogpup_solver_t* out, int nmesh, const int* h_row0, cons
       return std::min(r + kChunkRows, h_row0[mesh_of[r] + 1]);
     };
     const std::vector<int>* pats[3][2] = {{&rAp, &cAp}, {&rGp, &cGp}, {&rGT, &cGT}};
+    // per chunk: its distinct columns ascending (the halo), and each
+    // nonzero's position in that list; stamp[col] = the last chunk that
+    // listed col, slot[col] = its position there
+    std::vector<int> stamp(nn), slot(nn);
     for (int m = 0; m < 3; ++m) {
       const std::vector<int>& rp = *pats[m][0];
       const std::vector<int>& cp = *pats[m][1];
       std::vector<int> hp(nch + 1, 0), hc;
       std::vector<unsigned char> hl(cp.size(), 0), mok(nmesh, 1);
-      std::vector<int> cols;
-      // a mesh fits when every chunk's halo and nonzeros fit the LDS
-      for (int c = 0; c < nch; ++c) {
-        const int e0 = rp[chunk_row0[c]], e1 = rp[chunk_end(c)];
-        cols.assign(cp.begin() + e0, cp.begin() + e1);
-        std::sort(cols.begin(), cols.end());
-        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
-        if (cols.empty() || (int)cols.size() > kHaloCap || e1 - e0 > kChunkCap) mok[mesh_of[chunk_row0[c]]] = 0;
-      }
-      for (int c = 0; c < nch; ++c) {
-        if (mok[mesh_of[chunk_row0[c]]]) {
+      std::vector<int> cols, mcols, mptr;
+      std::fill(stamp.begin(), stamp.end(), -1);
+      hc.reserve(cp.size() / 3);
+      // a mesh fits when every chunk's halo and nonzeros fit the LDS; its
+      // chunks are consecutive, so each mesh is listed and then kept or
+      // dropped as a whole
+      for (int i = 0; i < nmesh; ++i) {
+        const int c0 = mesh_chb[i], c1 = i + 1 < nmesh ? mesh_chb[i + 1] : nch;
+        mcols.clear();
+        mptr.assign(1, 0);
+        for (int c = c0; c < c1 && mok[i]; ++c) {
           const int e0 = rp[chunk_row0[c]], e1 = rp[chunk_end(c)];
-          cols.assign(cp.begin() + e0, cp.begin() + e1);
-          std::sort(cols.begin(), cols.end());
-          cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+          cols.clear();
           for (int q = e0; q < e1; ++q)
-            hl[q] = (unsigned char)(std::lower_bound(cols.begin(), cols.end(), cp[q]) - cols.begin());
-          hc.insert(hc.end(), cols.begin(), cols.end());
+            if (stamp[cp[q]] != c) {
+              stamp[cp[q]] = c;
+              cols.push_back(cp[q]);
+            }
+          if (cols.empty() || (int)cols.size() > kHaloCap || e1 - e0 > kChunkCap) {
+            mok[i] = 0;
+            break;
+          }
+          std::sort(cols.begin(), cols.end());
+          for (int k = 0; k < (int)cols.size(); ++k) slot[cols[k]] = k;
+          for (int q = e0; q < e1; ++q) hl[q] = (unsigned char)slot[cp[q]];
+          mcols.insert(mcols.end(), cols.begin(), cols.end());
+          mptr.push_back((int)mcols.size());
         }
-        hp[c + 1] = (int)hc.size();
+        if (mok[i]) {
+          for (int c = c0; c < c1; ++c) hp[c + 1] = (int)hc.size() + mptr[c - c0 + 1];
+          hc.insert(hc.end(), mcols.begin(), mcols.end());
+        } else {
+          const int e0 = rp[chunk_row0[c0]], e1 = rp[chunk_end(c1 - 1)];
+          std::fill(hl.begin() + e0, hl.begin() + e1, (unsigned char)0);
+          for (int c = c0; c < c1; ++c) hp[c + 1] = (int)hc.size();
+        }
       }
       S->halo_ok[m] = std::find(mok.begin(), mok.end(), 1) != mok.end();
       S->mesh_ok_tmp[m] = mok;
