@@ -463,6 +463,66 @@ def fd_leg(n_nodes=50, seed=0, delta=1e-5):
             "nodal_product_vs_exact_max_rel": float(rel_nodal.max())}
 
 
+def shard_leg(D, tol, method, x_ref, ngpu=8):
+    """One GPU's share of the job split over `ngpu` GPUs, in three layouts of
+    ngpu shards of NS x NF / ngpu right-hand sides each: contiguous frequency
+    blocks x every source; source blocks x every frequency; and source blocks
+    x frequency halves.  Each shard has its own handle (own G, own
+    continuation levels over its frequencies: a shard needs nothing from
+    another during its solve).  Every shard is run here on this one GPU, one
+    after the other: a warm-up solve (graph capture), then one timed step of
+    FSAI setup + mixed solve, the headline step without fc2td (which in the
+    split job runs after an exchange by node range).  Each shard's field is
+    compared with the full solve's columns."""
+    import copy
+    import torch
+    opt = D.optode
+    nf, ns = opt.Nf, len(opt.s_bary[0])
+    fall, sall = np.arange(nf), np.arange(ns)
+    layouts = {
+        "frequency_blocks": [(sall, fall[r * nf // ngpu:(r + 1) * nf // ngpu]) for r in range(ngpu)],
+        "source_blocks": [(sall[r * ns // ngpu:(r + 1) * ns // ngpu], fall) for r in range(ngpu)],
+        "source_blocks_x_frequency_halves": [(sall[q * 2 * ns // ngpu:(q + 1) * 2 * ns // ngpu],
+                                              fall[h * nf // 2:(h + 1) * nf // 2])
+                                             for q in range(ngpu // 2) for h in range(2)],
+    }
+    out = {}
+    try:
+        for name, shards in layouts.items():
+            res = []
+            for sidx, freqs in shards:
+                sub = copy.copy(opt)
+                sub.s_bary = (opt.s_bary[0][sidx], opt.s_bary[1][sidx])
+                D.optode = sub
+                fq = None if len(freqs) == nf else freqs
+                D.solve_point_sources("s", mode="mixed", tol=tol, method=method, freqs=fq)
+                S = D.solver(len(sidx), "mixed", method, fq)
+                S._version = None                       # FSAI setup is part of the step
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                fw = D.solve_point_sources("s", mode="mixed", tol=tol, method=method, freqs=fq)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                assert fw.data.shape[0] == len(freqs) and fw.nsp >= len(sidx), "one column group per frequency"
+                ref = x_ref[torch.as_tensor(freqs, device=x_ref.device)][:, :, int(sidx[0]):int(sidx[-1]) + 1]
+                err = rel_err_per_rhs(fw.data[:, :, :len(sidx)].permute(0, 2, 1).reshape(-1, D.nn),
+                                      ref.permute(0, 2, 1).reshape(-1, D.nn))
+                res.append({"sources": [int(sidx[0]), int(sidx[-1])], "freqs": [int(freqs[0]), int(freqs[-1])],
+                            "group_width": int(fw.nsp), "s": dt, "max_iters": int(fw.iters.max()),
+                            "total_iters": int(fw.iters.sum()), "vs_full_solve_rel_diff_max": float(err.max())})
+                del fw
+                D.optode = opt
+                release(D)
+            ts = [r["s"] for r in res]
+            out[name] = {"shards": res, "max_s": max(ts), "sum_s": sum(ts)}
+    finally:
+        D.optode = opt
+    out["rhs_per_shard"] = ns * nf // ngpu
+    out["note"] = (f"one GPU's share of the 2048-RHS job split over {ngpu} GPUs, in three layouts; every shard "
+                   f"timed on this one GPU (step = FSAI setup + mixed solve, no fc2td); no multi-GPU run is made")
+    return out
+
+
 def lm_leg(max_iter=10):
     """Config 5: weighted LM absorption reconstruction.  Data: FP64 solve on
     a 0.75 mm mesh of the slab with two inclusions, 32 x 32 pairs x 8 gates,
@@ -548,6 +608,8 @@ def main(argv=None):
                     help="skip the exact-derivative J_TG^T ybar check against central differences at 50 nodes of "
                          "the 269k mesh (config 3, about half a minute)")
     ap.add_argument("--no-profile", action="store_true", help="skip the separate profiled step (per-class times)")
+    ap.add_argument("--no-shard", action="store_true",
+                    help="skip the shards (one GPU's share of the job split over 8 GPUs: by frequency, by source, by both)")
     ap.add_argument("--method", default="cocg", choices=["bicgstab", "cocg"],
                     help="Krylov method of the headline mixed solve")
     ap.add_argument("--no-alt", action="store_true",
@@ -650,6 +712,10 @@ def main(argv=None):
             extra[f"{name}_vs_galerkin_rel_diff_max"] = float(err.max())
             del phn
             release(D)
+
+    if not a.no_shard and not a.small:
+        log("shards: one GPU's share of the job split over 8 GPUs, three layouts")
+        extra["shards"] = shard_leg(D, a.tol, a.method, x_mixed)
 
     names = ["G/G^T SpMV", "A SpMV", "Krylov vector passes", "FP64 refinement passes"]
     achieved = byts = nl = ms = None

@@ -264,17 +264,18 @@ class DMesh:
         return _i32(nodes), _f64(w)
 
     # -- solver -----------------------------------------------------------------
-    def solver(self, ncols, mode="mixed", method=None):
-        key = (ncols, mode)
+    def solver(self, ncols, mode="mixed", method=None, freqs=None):
+        fk = None if freqs is None else tuple(int(k) for k in np.asarray(freqs).reshape(-1))
+        key = (ncols, mode) if fk is None else (ncols, mode, fk)
         s = self._solvers.get(key)
         if s is None:
-            s = FluenceSolver(self, ncols, MODES[mode], nsp=group_width(ncols, mode))
+            s = FluenceSolver(self, ncols, MODES[mode], nsp=group_width(ncols, mode), freqs=freqs)
             self._solvers[key] = s
         s.set_method(default_method(mode) if method is None else method)
         return s
 
     def solve_point_sources(self, which="s", mode="mixed", tol=None, maxit=1000, extra=None,
-                            method=None, adj_tol=None, cont=None):
+                            method=None, adj_tol=None, cont=None, freqs=None):
         """Solve for the sources ('s'), detectors ('d') or both ('sd', forward
         and adjoint batched in one launch: columns 0..NS-1 sources,
         NS..NS+ND-1 detectors).  method: 'cocg' (conjugate orthogonal CG, the
@@ -289,17 +290,22 @@ class DMesh:
         every right-hand side carries: True / None / 'galerkin' (the
         default) combines the six nearest by the Galerkin solution in their
         span, 'lagrange' interpolates the four nearest; False starts every
-        right-hand side from 0, as the reference does."""
+        right-hand side from 0, as the reference does.
+        freqs: indices into the optode's frequency axis to solve (a frequency
+        shard, such as one GPU's share when the frequencies are split across
+        GPUs); None solves every frequency.  The field then holds those
+        frequencies in the given order."""
         parts = [self._bary_dev(w) for w in which]
         nodes = torch.cat([p[0] for p in parts])
         w = torch.cat([p[1] for p in parts])
         ncols = nodes.shape[0]
-        S = self.solver(ncols, mode, method)
+        S = self.solver(ncols, mode, method, freqs)
+        s_fpsf = np.asarray(self.optode.s_fpsf)[S.freqs]
         if MODES[mode] == _lib.MODE_C64_REFINE:
-            S.set_continuation(cont, self.optode.s_fpsf)
+            S.set_continuation(cont, s_fpsf)
         elif cont:
             raise ValueError("frequency continuation is for the mixed mode")
-        spec = torch.as_tensor(self.optode.s_fpsf, dtype=torch.complex128, device=DEV)
+        spec = torch.as_tensor(s_fpsf, dtype=torch.complex128, device=DEV)
         b = S.empty_block(zero=False)               # dogpup_point_sources clears it
         _lib.call("dogpup_point_sources", _lib.ptr(torch.view_as_real(b)), self.nn, S.nf, ncols, S.nsp,
                   _lib.ptr(nodes), _lib.ptr(w), _lib.ptr(torch.view_as_real(spec)), _lib.stream_ptr())
@@ -336,11 +342,16 @@ class FluenceSolver:
     """Device-resident batched FSAI-BiCGSTAB for one mesh, column count and
     mode (wraps dogpup_solver_*)."""
 
-    def __init__(self, mesh: DMesh, ncols: int, mode: int, order=None, nsp=None):
+    def __init__(self, mesh: DMesh, ncols: int, mode: int, order=None, nsp=None, freqs=None):
         if mesh.optode is None:
             raise ValueError("add an optode first (the frequency axis comes from it)")
         self.mesh = mesh
-        self.nf = mesh.optode.Nf
+        nfa = mesh.optode.Nf
+        # frequency k of the handle is index freqs[k] of the optode's axis
+        self.freqs = np.arange(nfa) if freqs is None else np.asarray(freqs, np.int64).reshape(-1)
+        if self.freqs.size == 0 or self.freqs.min() < 0 or self.freqs.max() >= nfa:
+            raise ValueError(f"freqs must be indices of the optode's {nfa} frequencies")
+        self.nf = int(self.freqs.size)
         self.ncols = ncols
         self.nsp = int(nsp) if nsp else group_width(ncols, "mixed" if mode == _lib.MODE_C64_REFINE else "fp64")
         self.ncb = -(-ncols // self.nsp)
@@ -402,7 +413,7 @@ class FluenceSolver:
         m = self.mesh
         if self._version == m._matrix_version:
             return
-        fscale = _f64(np.arange(self.nf) * m.optode.df / m.c)    # A_k = K + i k df/c M
+        fscale = _f64(self.freqs * m.optode.df / m.c)    # A_k = K + i k df/c M
         _lib.call("dogpup_solver_set_matrix", self.h, _lib.ptr(m.K), _lib.ptr(m.M), _lib.ptr(fscale),
                   _lib.stream_ptr())
         torch.cuda.current_stream().synchronize()

@@ -85,6 +85,26 @@ def test_frequency_continuation_vs_direct(sixteen, method):
     assert out["galerkin"].sum() < out["lagrange"].sum()
 
 
+@pytest.mark.parametrize("mode", ["mixed", "fp64"])
+def test_frequency_shards_vs_direct(sixteen, mode):
+    """A handle over a subset of the optode's frequencies (one GPU's share when
+    the frequencies are split across GPUs): two contiguous halves and a
+    strided shard, each against the oracle's direct solve at its frequencies
+    (dMesh.m:1238-1348), with the fields in the shard's frequency order."""
+    D, phi = sixteen["D"], sixteen["phi"]
+    for freqs in (np.arange(0, 8), np.arange(8, 16), np.array([13, 1, 9, 5])):
+        print(f"[case] {mode} shard {freqs.tolist()}", flush=True)
+        fw = D.solve_point_sources("s", mode=mode, tol=1e-10, freqs=freqs)
+        assert fw.nf == len(freqs) and (fw.status == 0).all(), fw.status
+        assert (fw.relres < 1e-10).all()
+        got = fw.to_reference_layout().cpu().numpy()
+        for j, k in enumerate(freqs):
+            for s in range(got.shape[2]):
+                assert rel(got[:, j, s], phi[:, k, s]) < 1e-7, (freqs.tolist(), k, s)
+    with pytest.raises(ValueError):
+        D.solve_point_sources("s", mode=mode, freqs=[0, sixteen["nf"]])
+
+
 def test_warm_start_and_guess_terms(sixteen):
     """Explicit terms through the C ABI: a warm start from the previous
     solution converges at its first FP64 residual (0 iterations); a cycle
