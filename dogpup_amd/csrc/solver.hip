@@ -3822,15 +3822,16 @@ static int create_batch(dogpup_solver_t* out, int nmesh, const int* h_row0, cons
   if (mode == DOGPUP_MODE_C128_PLAIN) S->method = DOGPUP_METHOD_BICGSTAB;
   S->nnzA = nnzA; S->nnzG = nnzG;
   S->rA0 = d_rptrA; S->cA0 = d_cptrA; S->rG0 = d_rptrG; S->cG0 = d_cptrG;
-  // Row tile of the kernels (a multiple of 64 rows).  256 in the refinement
+  // Row tile of the kernels (a multiple of 64 rows).  384 in the refinement
   // form, whose continuation levels launch few groups at a time, so that
-  // more and shorter work items fill the GPU (129.5-129.9 against
-  // 131.8-132.1 ms per step at 512 on the 269k-node, 2048-RHS solve + fc2td;
-  // 136.6 ms at 128); 512 in the FP64 forms.  A batch and a single-mesh
+  // more and shorter work items fill the GPU (ms per step of the 269k-node,
+  // 2048-RHS solve + fc2td, same box: 192: 130.5-130.8, 256: 128.9-129.0,
+  // 320: 129.1-129.2, 384: 126.6-126.8, 448: 128.1-128.2, 512: 131.8-132.1);
+  // 512 in the FP64 forms.  A batch and a single-mesh
   // handle of the same form share the default, so that their solves agree
   // bit for bit; a batch's tile is for the tuner to pick (48 uneven meshes in
   // one default handle: 0.714 s at 256, 0.668-0.691 s at 512).
-  const int def_tile = mode == DOGPUP_MODE_C64_REFINE ? 256 : 512;
+  const int def_tile = mode == DOGPUP_MODE_C64_REFINE ? 384 : 512;
   S->tile_rows = def_tile;
   if (const char* e = getenv("DOGPUP_TILE")) S->tile_rows = atoi(e);
   if (S->tile_rows < 64 || S->tile_rows % 64) S->tile_rows = def_tile;
